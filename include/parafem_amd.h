@@ -1,0 +1,200 @@
+/*
+ * parafem_amd -- MI355X (gfx950) element-by-element PCG for ParaFEM's
+ * p121 (hex20 elasticity) and p123 (hex8 Laplace) solves.
+ *
+ * C ABI.  Plain pointers and sizes only.  Every entry returns an int status
+ * (PF_OK = 0).  Conventions follow ParaFEM's Fortran arrays:
+ *   - integer arrays are int32, equation and node ids are 1-based and 0 in a
+ *     steering array means "restrained freedom";
+ *   - matrices are column-major: storkm_pp(ntot,ntot,nels) is passed as
+ *     nels consecutive ntot x ntot column-major blocks, g_g_pp(ntot,nels) as
+ *     nels consecutive ntot-vectors, g_coord_pp(nod,3,nels) as nels
+ *     consecutive (nod,3) column-major blocks.
+ * Host pointers ("host") are read or written only during the call.  Device
+ * pointers ("dev") must be device memory of the context's GPU (from
+ * pf_dev_alloc or any HIP allocator); work on them is ordered on the
+ * context's stream.  One context per host thread; no global state other than
+ * what each context owns.
+ *
+ * The reference has no such library: its drivers call module procedures of
+ * the same names from Fortran (src/modules/readme.txt:26-38 describes the
+ * same-name library swap).  Each declaration cites the reference routine it
+ * replaces; paths are relative to ParaFEM's source tree.
+ */
+#ifndef PARAFEM_AMD_H
+#define PARAFEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status */
+#define PF_OK              0
+#define PF_ERR_ARG         1  /* bad argument (null, size, order of calls)   */
+#define PF_ERR_HIP         2  /* HIP runtime error                            */
+#define PF_ERR_OOM         3  /* device allocation failed                     */
+#define PF_ERR_STATE       4  /* required set-up call missing                 */
+#define PF_NOT_CONVERGED   5  /* PCG hit its iteration limit                  */
+#define PF_ERR_BREAKDOWN   6  /* non-finite or zero curvature p.u / up        */
+#define PF_ERR_NODEVICE    7  /* no HIP device / device index out of range    */
+
+/* Storage precision of element matrices. */
+#define PF_STORE_FP64      0  /* storkm_pp as REAL(iwp) (precision.f90:19)    */
+#define PF_STORE_FP32      1  /* FP32 hi + FP32 lo split (K ~= hi + lo)       */
+
+/* PCG stopping rule. */
+#define PF_STOP_CHECON     0  /* checon_par: max|dx|/max|x| <= tol
+                                 (src/modules/mpi/maths.f90:999-1069)         */
+#define PF_STOP_RELRES     1  /* ||r||_2 <= tol * ||r0||_2                    */
+
+/* PCG arithmetic. */
+#define PF_PREC_FP64       0  /* FP64 everywhere (the reference's arithmetic) */
+#define PF_PREC_MIXED      1  /* FP32 element products, FP64 vectors, dots,
+                                 recurrences and true-residual refinement     */
+
+typedef struct pf_ctx pf_ctx;
+
+typedef struct {
+    int    limit;        /* iteration limit (p121.f90:103 "iters==limit")     */
+    double tol;          /* tolerance for the stopping rule                   */
+    int    stop_mode;    /* PF_STOP_*                                          */
+    int    precision;    /* PF_PREC_*                                          */
+    int    check_every;  /* host polls convergence every n iterations (0=16)  */
+    int    refine_every; /* mixed: true-residual replacement period (0=auto)  */
+} pf_pcg_opts;
+
+typedef struct {
+    int    iters;        /* completed iterations, as the reference counts     */
+    int    converged;    /* 1 if the stopping rule was met                    */
+    double crit;         /* last value of the stopping criterion              */
+    double rr0;          /* ||r0||_2^2                                         */
+    double rr;           /* ||r||_2^2 of the recurrence residual at exit      */
+    int    refinements;  /* mixed: number of true-residual replacements        */
+    int    status;       /* PF_OK, PF_NOT_CONVERGED or PF_ERR_BREAKDOWN        */
+} pf_pcg_result;
+
+/* ------------------------------------------------------ context, memory */
+/* Library version (major*10000 + minor*100 + patch). */
+int pf_version(void);
+/* Human-readable text for a status code; never NULL. */
+const char *pf_status_string(int status);
+/* Replaces FIND_PE_PROCS (src/modules/mpi/mp_interface.f90:42-91): one
+ * process drives one GPU, so the "process" is a context on a device. */
+int pf_create(int device, pf_ctx **ctx);
+/* Replaces SHUTDOWN (mp_interface.f90:97-106); frees all device state. */
+int pf_destroy(pf_ctx *ctx);
+/* Order the context's work on a caller's hipStream_t (NULL = the context's
+ * own stream).  The stream must outlive its use by the context. */
+int pf_set_stream(pf_ctx *ctx, void *hip_stream);
+/* The hipStream_t the context currently enqueues on. */
+void *pf_get_stream(pf_ctx *ctx);
+/* Wait for all work the context enqueued. */
+int pf_sync(pf_ctx *ctx);
+/* Device buffers for callers without an allocator of their own (Fortran). */
+int pf_dev_alloc(pf_ctx *ctx, size_t bytes, void **dptr);
+int pf_dev_free(pf_ctx *ctx, void *dptr);
+int pf_memcpy_h2d(pf_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+int pf_memcpy_d2h(pf_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+/* Bytes of device memory the context holds. */
+int pf_mem_used(pf_ctx *ctx, size_t *bytes);
+
+/* ------------------------------------------------------------- set-up */
+/* Element coordinates g_coord_pp(nod,3,nels) (host), as read by
+ * READ_G_COORD_PP (src/modules/mpi/input.f90:288-445).  nod = 8 or 20. */
+int pf_mesh_set(pf_ctx *ctx, int nod, int nels, const double *g_coord_pp);
+/* Element steering g_g_pp(ntot,nels) (host) and neq, as built by
+ * REARRANGE + FIND_G3 / FIND_G4 (src/modules/shared/new_library.f90:
+ * 3059-3271; p121.f90:44-48).  Builds the device gather map and the
+ * transposed (equation -> element slots, ascending element) scatter map that
+ * MAKE_GGL's 1-rank tables stand for (src/modules/mpi/gather_scatter.f90:
+ * 1387-1780).  Integer maps are exact; no equation may be missing. */
+int pf_steer_set(pf_ctx *ctx, int ntot, int nels, int neq, const int *g_g_pp);
+/* Element stiffness of p121 (p121.f90:54-64): SAMPLE, SHAPE_DER, jacobian,
+ * DETERMINANT, INVERT, BEEMAT, DEEMAT, K += B^T D B det w over 2x2x2 Gauss
+ * points, fused in one kernel that writes storkm_pp to HBM.  e_elem (host,
+ * nels, nullable) gives a per-element Young's modulus, else e. */
+int pf_assemble_elastic(pf_ctx *ctx, double e, double v, const double *e_elem, int store);
+/* Element conductivity of p123 (p123.f90:70-84):
+ * storkc = kx*kcx + ky*kcy + kz*kcz; k_elem (host, nullable) scales all three
+ * per element. */
+int pf_assemble_laplace(pf_ctx *ctx, double kx, double ky, double kz, const double *k_elem, int store);
+/* Upload / download storkm_pp(ntot,ntot,nels) (host, FP64).  Upload lets a
+ * caller keep its own assembly (the driver's loop A) and use the device PCG. */
+int pf_storkm_set(pf_ctx *ctx, const double *storkm_pp, int store);
+int pf_storkm_get(pf_ctx *ctx, double *storkm_pp);
+/* Assembled diagonal of the element matrices (host, neq): the
+ * diag_precon_tmp -> SCATTER step of p121.f90:65-69 before inversion. */
+int pf_diag_get(pf_ctx *ctx, double *diag);
+/* p123 fixed freedoms by penalty (p123.f90:94-132): adds penalty to the
+ * assembled diagonal of equations no_f (host, 1-based) and makes the PCG
+ * replace u(no_f(i)) by p(no_f(i))*store(i) after each scatter
+ * (p123.f90:141-145), store(i) = diag + penalty.  Call after assembly.
+ * store_out (host, nullable, nfix) receives the store values. */
+int pf_fixed_set(pf_ctx *ctx, int nfix, const int *no_f, double penalty, double *store_out);
+
+/* ----------------------------------------- drop-in primitives (device) */
+/* GATHER(p_pp, pmul_pp) (src/modules/mpi/gather_scatter.f90:547-688):
+ * pmul(:,e) = p(g_g(:,e)), 0 for restrained freedoms.  p: dev neq,
+ * pmul: dev ntot*nels. */
+int pf_gather(pf_ctx *ctx, const double *p_dev, double *pmul_dev);
+/* utemp(:,e) = MATMUL(storkm(:,:,e), pmul(:,e)) (p121.f90:93-97). */
+int pf_elem_matvec(pf_ctx *ctx, const double *pmul_dev, double *utemp_dev);
+/* SCATTER(u_pp, utemp_pp) (gather_scatter.f90:694-850): u(eq) += sum of
+ * utemp over the element slots of eq in ascending element order. */
+int pf_scatter(pf_ctx *ctx, double *u_dev, const double *utemp_dev);
+/* SCATTER_NOADD(utemp_pp, u_pp) (gather_scatter.f90:856-993): assignment. */
+int pf_scatter_noadd(pf_ctx *ctx, const double *utemp_dev, double *u_dev);
+/* DOT_PRODUCT_P(a, b) (src/modules/mpi/maths.f90:168-216); result to host. */
+int pf_dot_product_p(pf_ctx *ctx, const double *a_dev, const double *b_dev, int64_t n, double *out);
+/* SUM_P(a) (maths.f90:271-315); result to host. */
+int pf_sum_p(pf_ctx *ctx, const double *a_dev, int64_t n, double *out);
+/* checon_par(loads, tol, converged, oldlds) (maths.f90:999-1069):
+ * converged = max|xnew-x|/max|xnew| <= tol, then x = xnew. */
+int pf_checon_par(pf_ctx *ctx, const double *xnew_dev, double tol, int *converged, double *x_dev, int64_t n);
+
+/* ----------------------------------------------------- the PCG solve */
+/* The PCG loop of p121.f90:86-104 / p123.f90:133-151 on device-resident
+ * state: d = dinv*r, p = d, x = 0, then per iteration gather -> element
+ * products -> scatter, alpha, x/r/d updates, beta, p update and the stopping
+ * rule.  r0 is the loaded right-hand side (host neq).  x receives xnew (host
+ * neq).  Returns PF_NOT_CONVERGED if the limit was reached. */
+int pf_pcg_solve(pf_ctx *ctx, const double *r0_host, double *x_host,
+                 const pf_pcg_opts *opts, pf_pcg_result *res);
+/* Split form for callers that keep vectors on the device: begin (r0 host or
+ * dev), run n more iterations without host synchronisation (iterations after
+ * convergence are no-ops on the device), poll (synchronises), read x. */
+int pf_pcg_begin(pf_ctx *ctx, const double *r0, int r0_on_device, const pf_pcg_opts *opts);
+int pf_pcg_run(pf_ctx *ctx, int n_iters);
+int pf_pcg_poll(pf_ctx *ctx, pf_pcg_result *res);
+int pf_pcg_x(pf_ctx *ctx, double *x, int x_on_device);
+/* y = K x over the assembled operator in FP64 (host vectors, neq), with the
+ * p123 fixed-freedom rows applied.  Used to report true residuals. */
+int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);
+
+/* ------------------------------------------------------------ timing */
+/* No counterpart in the reference (its timing is CPU_TIME around phases,
+ * src/modules/shared/timing.f90:35-62).  While enabled, pf_pcg_run records a
+ * HIP event pair on the context's stream around every kernel it launches.
+ * pf_pcg_timing_get synchronises and returns, per kernel of an iteration
+ * (0 element products, 1 scatter + p.u, 2 x/r/d update, 3 p update), the
+ * summed milliseconds and the launch count since the last enable. */
+int pf_pcg_timing_enable(pf_ctx *ctx, int on);
+int pf_pcg_timing_get(pf_ctx *ctx, double ms[4], int counts[4]);
+
+/* ------------------------------------------------- layout introspection */
+/* Grid size G (blocks of 256 threads) the FP64 reductions use for length n.
+ * It fixes the summation order of every dot product: thread g = 256*b + t
+ * adds a_i*b_i for i = g, g + 256*G, ... in turn (s = s + a*b, no fused
+ * multiply-add); each 64-lane wavefront folds its lanes with offsets
+ * 32,16,8,4,2,1 (v[l] = v[l] + v[l+off]); a block adds its 4 wavefront sums
+ * left to right; the last block to finish folds the G block sums the same
+ * way (thread t sums entries t, t+256, ..., then wavefronts, then blocks). */
+int pf_reduce_grid(int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARAFEM_AMD_H */

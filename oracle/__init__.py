@@ -1,0 +1,277 @@
+"""CPU oracle for the EBE-PCG path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings to ``liboracle.so`` (built from ``pf_oracle.c`` by
+``oracle/Makefile``), a plain-C FP64 restatement of ParaFEM's p121/p123
+element assembly and PCG.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s CPU-baseline leg use it, and only as the checker; the product
+(``parafem_amd``) never imports it.
+
+Every function below names the reference routine it restates; the C file
+cites the reference line ranges.
+
+Pinning status (how each function is checked in ``tests/test_oracle.py``):
+  sample_hex8, deemat6, shape_der, beemat6 -- against the reference's own
+      new_library.f90 compiled in oracle/_ref (when buildable) and against
+      closed forms (Gauss-Legendre, Hooke's law, partition of unity,
+      finite differences of the shape functions);
+  determinant3, invert3 -- closed forms / numpy on random matrices;
+  elastic_km, laplace_kc -- symmetry, rigid-body null space, patch test,
+      the hex8 unit-cube closed form, numpy restatement of B^T D B;
+  gather/scatter -- adjointness and brute force;
+  pcg -- the reference's committed fixtures (xx3-tiny: 1640 equations, 79
+      iterations, displacements; p121 demo: 98360 equations, total load
+      -100, 295 iterations, displacement field) and scipy direct solves.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+STOP_CHECON = 0
+STOP_RELRES = 1
+DOT_SEQ = 0        # DOT_PRODUCT order at one rank (the reference)
+DOT_THREADS = 1    # per-thread contiguous chunks (like per-rank partial sums)
+DOT_DEVICE = 2     # the device library's fixed reduction order
+
+
+def build() -> None:
+    import subprocess
+    subprocess.run(["make", "-B", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        d = C.POINTER(C.c_double)
+        i = C.POINTER(C.c_int)
+        L.or_sample_hex8.argtypes = [d, d]
+        L.or_deemat6.argtypes = [d, C.c_double, C.c_double]
+        L.or_shape_der.argtypes = [d, d, C.c_int, C.c_int, C.c_int]
+        L.or_shape_der.restype = C.c_int
+        L.or_determinant3.argtypes = [d]
+        L.or_determinant3.restype = C.c_double
+        L.or_invert3.argtypes = [d]
+        L.or_beemat6.argtypes = [d, d, C.c_int]
+        L.or_elastic_km.argtypes = [d, C.c_int, d, d]
+        L.or_laplace_kc.argtypes = [d, C.c_int, C.c_double, C.c_double, C.c_double, d]
+        L.or_assemble_elastic.argtypes = [C.c_int, C.c_int, d, d, C.c_double, C.c_double, d, C.c_int]
+        L.or_assemble_laplace.argtypes = [C.c_int, C.c_int, d, d, C.c_double, C.c_double, C.c_double, d, C.c_int]
+        L.or_gather.argtypes = [C.c_int, C.c_int, i, d, d]
+        L.or_scatter.argtypes = [C.c_int, C.c_int, i, d, d]
+        L.or_elem_matvec.argtypes = [C.c_int, C.c_int, d, d, d, C.c_int]
+        L.or_dot.argtypes = [C.c_long, d, d, C.c_int]
+        L.or_dot.restype = C.c_double
+        L.or_sum.argtypes = [C.c_long, d]
+        L.or_sum.restype = C.c_double
+        L.or_checon.argtypes = [C.c_long, d, C.c_double, d]
+        L.or_checon.restype = C.c_int
+        L.or_pcg.argtypes = [C.c_int, C.c_int, C.c_int, i, d, d, d, C.c_int, i, d,
+                             C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, d, i, d]
+        L.or_dot_dev.argtypes = [C.c_long, d, d]
+        L.or_dot_dev.restype = C.c_double
+        L.or_pcg.restype = C.c_int
+        L.or_diag.argtypes = [C.c_int, C.c_int, C.c_int, i, d, d]
+        L.or_apply.argtypes = [C.c_int, C.c_int, C.c_int, i, d, d, d, C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def sample_hex8():
+    """SAMPLE('hexahedron', points, weights), nip = 8 -> (points (8,3), w (8,))."""
+    pts = np.zeros(24)
+    w = np.zeros(8)
+    lib().or_sample_hex8(_d(pts), _d(w))
+    return pts.reshape(3, 8).T.copy(), w
+
+
+def deemat6(e: float, v: float):
+    """DEEMAT for nst = 6 -> (6,6)."""
+    out = np.zeros(36)
+    lib().or_deemat6(_d(out), e, v)
+    return out.reshape(6, 6).T.copy()
+
+
+def shape_der(points: np.ndarray, i: int, nod: int):
+    """SHAPE_DER(der, points, i) -> der (3, nod); i is 1-based."""
+    pts = _f64(np.asarray(points).T)           # column-major (nip,3)
+    der = np.zeros(3 * nod)
+    rc = lib().or_shape_der(_d(der), _d(pts), points.shape[0], i, nod)
+    if rc:
+        raise ValueError(nod)
+    return der.reshape(nod, 3).T.copy()
+
+
+def determinant3(jac):
+    a = _f64(np.asarray(jac).T)
+    return lib().or_determinant3(_d(a))
+
+
+def invert3(jac):
+    a = _f64(np.asarray(jac).T)
+    lib().or_invert3(_d(a))
+    return a.reshape(3, 3).T.copy()
+
+
+def beemat6(deriv):
+    nod = deriv.shape[1]
+    dv = _f64(np.asarray(deriv).T)
+    bee = np.zeros(6 * 3 * nod)
+    lib().or_beemat6(_d(bee), _d(dv), nod)
+    return bee.reshape(3 * nod, 6).T.copy()
+
+
+def elastic_km(coord, dee):
+    """One element of p121.f90:56-64; coord (nod,3) -> km (ntot,ntot)."""
+    nod = coord.shape[0]
+    c = _f64(np.asarray(coord).T)
+    dd = _f64(np.asarray(dee).T)
+    km = np.zeros(9 * nod * nod)
+    lib().or_elastic_km(_d(c), nod, _d(dd), _d(km))
+    return km.reshape(3 * nod, 3 * nod).T.copy()
+
+
+def laplace_kc(coord, kx=1.0, ky=1.0, kz=1.0):
+    nod = coord.shape[0]
+    c = _f64(np.asarray(coord).T)
+    kc = np.zeros(nod * nod)
+    lib().or_laplace_kc(_d(c), nod, kx, ky, kz, _d(kc))
+    return kc.reshape(nod, nod).T.copy()
+
+
+def assemble_elastic(g_coord, e, v, e_elem=None, nthreads=0):
+    """storkm_pp for every element.  g_coord is (nels, 3, nod) -- the
+    reference's (nod,3,nels) column-major memory.  Returns (nels, ntot, ntot)
+    whose [e] slice in memory order is storkm_pp(:,:,e) (column-major)."""
+    g = _f64(g_coord)
+    nels, _, nod = g.shape
+    ntot = 3 * nod
+    out = np.empty((nels, ntot, ntot))
+    ee = None if e_elem is None else _f64(e_elem)
+    lib().or_assemble_elastic(nels, nod, _d(g), None if ee is None else _d(ee), e, v, _d(out), nthreads)
+    return out
+
+
+def assemble_laplace(g_coord, kx=1.0, ky=1.0, kz=1.0, k_elem=None, nthreads=0):
+    g = _f64(g_coord)
+    nels, _, nod = g.shape
+    out = np.empty((nels, nod, nod))
+    kk = None if k_elem is None else _f64(k_elem)
+    lib().or_assemble_laplace(nels, nod, _d(g), None if kk is None else _d(kk), kx, ky, kz, _d(out), nthreads)
+    return out
+
+
+def gather(g_g, p):
+    g = _i32(g_g)
+    nels, ntot = g.shape
+    out = np.zeros((nels, ntot))
+    pp = _f64(p)
+    lib().or_gather(nels, ntot, _i(g), _d(pp), _d(out))
+    return out
+
+
+def scatter(g_g, utemp, neq, u=None):
+    g = _i32(g_g)
+    nels, ntot = g.shape
+    out = np.zeros(neq) if u is None else _f64(u).copy()
+    ut = _f64(utemp)
+    lib().or_scatter(nels, ntot, _i(g), _d(ut), _d(out))
+    return out
+
+
+def elem_matvec(storkm, pmul, nthreads=0):
+    k = _f64(storkm)
+    nels, ntot, _ = k.shape
+    pm = _f64(pmul)
+    out = np.empty((nels, ntot))
+    lib().or_elem_matvec(nels, ntot, _d(k), _d(pm), _d(out), nthreads)
+    return out
+
+
+def dot(a, b, nthreads=1):
+    a = _f64(a)
+    b = _f64(b)
+    return lib().or_dot(a.size, _d(a), _d(b), nthreads)
+
+
+def dot_device(a, b=None):
+    """Sum of a*b (or of a) in the device library's reduction order."""
+    a = _f64(a)
+    bb = None if b is None else _f64(b)
+    return lib().or_dot_dev(a.size, _d(a), None if bb is None else _d(bb))
+
+
+def sum_p(a):
+    a = _f64(a)
+    return lib().or_sum(a.size, _d(a))
+
+
+def checon(xnew, tol, x):
+    """checon_par: returns (converged, x updated to xnew)."""
+    xn = _f64(xnew)
+    xo = _f64(x).copy()
+    c = lib().or_checon(xn.size, _d(xn), tol, _d(xo))
+    return bool(c), xo
+
+
+def diag(g_g, storkm, neq):
+    g = _i32(g_g)
+    k = _f64(storkm)
+    out = np.zeros(neq)
+    lib().or_diag(g.shape[0], g.shape[1], neq, _i(g), _d(k), _d(out))
+    return out
+
+
+def apply(g_g, storkm, x, nthreads=0):
+    g = _i32(g_g)
+    k = _f64(storkm)
+    xx = _f64(x)
+    y = np.zeros(xx.size)
+    lib().or_apply(g.shape[0], g.shape[1], xx.size, _i(g), _d(k), _d(xx), _d(y), nthreads)
+    return y
+
+
+def pcg(g_g, storkm, r0, diag_assembled, tol, limit, stop=STOP_CHECON,
+        no_f=None, store=None, nthreads=1, history=False, dot_order=DOT_SEQ):
+    """The p121/p123 PCG loop.  Returns (x, iters, converged[, hist])."""
+    g = _i32(g_g)
+    k = _f64(storkm)
+    nels, ntot = g.shape
+    r = _f64(r0)
+    dg = _f64(diag_assembled)
+    neq = r.size
+    x = np.zeros(neq)
+    it = np.zeros(1, dtype=np.int32)
+    nfix = 0 if no_f is None else len(no_f)
+    nf = _i32(no_f if nfix else np.zeros(1))
+    st = _f64(store if nfix else np.zeros(1))
+    hist = np.zeros(max(limit, 1))
+    conv = lib().or_pcg(nels, ntot, neq, _i(g), _d(k), _d(r), _d(dg), nfix, _i(nf), _d(st),
+                        tol, limit, stop, nthreads, dot_order, _d(x), _i(it), _d(hist))
+    if history:
+        return x, int(it[0]), bool(conv), hist[: int(it[0])]
+    return x, int(it[0]), bool(conv)

@@ -1,0 +1,414 @@
+"""Python binding of libparafem_amd (ctypes over the C ABI in include/).
+
+The names mirror the reference's Fortran call surface for the EBE-PCG path
+(``GATHER``, ``SCATTER``, ``SCATTER_NOADD``, ``DOT_PRODUCT_P``, ``SUM_P``,
+``checon_par``; ``src/modules/mpi/gather_scatter.f90``,
+``src/modules/mpi/maths.f90``) with the same argument meaning.  Arrays use
+the reference's memory order (Fortran column-major): a numpy array of shape
+(nels, ntot) holds ``g_g_pp(ntot, nels)``, (nels, 3, nod) holds
+``g_coord_pp(nod, 3, nels)`` and (nels, ntot, ntot) holds
+``storkm_pp(ntot, ntot, nels)`` where element e's block is the column-major
+ntot x ntot matrix, i.e. ``K[e].T`` is the mathematical matrix (K is
+symmetric, so the two agree).
+
+The shared library is required: importing works without it, but creating a
+:class:`Context` raises :class:`LibraryNotBuilt` -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libparafem_amd.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "parafem_amd.h")
+
+PF_OK = 0
+PF_ERR_ARG = 1
+PF_ERR_HIP = 2
+PF_ERR_OOM = 3
+PF_ERR_STATE = 4
+PF_NOT_CONVERGED = 5
+PF_ERR_BREAKDOWN = 6
+PF_ERR_NODEVICE = 7
+
+STORE = {"fp64": 0, "fp32": 1}
+STOP = {"checon": 0, "relres": 1}
+PREC = {"fp64": 0, "mixed": 1}
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+class PFError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: status {status} ({status_string(status)})")
+
+
+class PcgOpts(C.Structure):
+    _fields_ = [("limit", C.c_int), ("tol", C.c_double), ("stop_mode", C.c_int),
+                ("precision", C.c_int), ("check_every", C.c_int), ("refine_every", C.c_int)]
+
+
+class PcgResult(C.Structure):
+    _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("crit", C.c_double),
+                ("rr0", C.c_double), ("rr", C.c_double), ("refinements", C.c_int),
+                ("status", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libparafem_amd.so (raises LibraryNotBuilt when absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryNotBuilt(f"{LIB_PATH} is missing: run `python -m parafem_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, dbl, sz = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_size_t
+    pd, pi = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    sig = {
+        "pf_version": (i32, []),
+        "pf_status_string": (C.c_char_p, [i32]),
+        "pf_create": (i32, [i32, C.POINTER(vp)]),
+        "pf_destroy": (i32, [vp]),
+        "pf_set_stream": (i32, [vp, vp]),
+        "pf_get_stream": (vp, [vp]),
+        "pf_sync": (i32, [vp]),
+        "pf_dev_alloc": (i32, [vp, sz, C.POINTER(vp)]),
+        "pf_dev_free": (i32, [vp, vp]),
+        "pf_memcpy_h2d": (i32, [vp, vp, vp, sz]),
+        "pf_memcpy_d2h": (i32, [vp, vp, vp, sz]),
+        "pf_mem_used": (i32, [vp, C.POINTER(sz)]),
+        "pf_mesh_set": (i32, [vp, i32, i32, pd]),
+        "pf_steer_set": (i32, [vp, i32, i32, i32, pi]),
+        "pf_assemble_elastic": (i32, [vp, dbl, dbl, pd, i32]),
+        "pf_assemble_laplace": (i32, [vp, dbl, dbl, dbl, pd, i32]),
+        "pf_storkm_set": (i32, [vp, pd, i32]),
+        "pf_storkm_get": (i32, [vp, pd]),
+        "pf_diag_get": (i32, [vp, pd]),
+        "pf_fixed_set": (i32, [vp, i32, pi, dbl, pd]),
+        "pf_gather": (i32, [vp, vp, vp]),
+        "pf_elem_matvec": (i32, [vp, vp, vp]),
+        "pf_scatter": (i32, [vp, vp, vp]),
+        "pf_scatter_noadd": (i32, [vp, vp, vp]),
+        "pf_dot_product_p": (i32, [vp, vp, vp, i64, pd]),
+        "pf_sum_p": (i32, [vp, vp, i64, pd]),
+        "pf_checon_par": (i32, [vp, vp, dbl, pi, vp, i64]),
+        "pf_pcg_solve": (i32, [vp, pd, pd, C.POINTER(PcgOpts), C.POINTER(PcgResult)]),
+        "pf_pcg_begin": (i32, [vp, vp, i32, C.POINTER(PcgOpts)]),
+        "pf_pcg_run": (i32, [vp, i32]),
+        "pf_pcg_poll": (i32, [vp, C.POINTER(PcgResult)]),
+        "pf_pcg_x": (i32, [vp, vp, i32]),
+        "pf_apply": (i32, [vp, pd, pd]),
+        "pf_reduce_grid": (i32, [i64]),
+        "pf_pcg_timing_enable": (i32, [vp, i32]),
+        "pf_pcg_timing_get": (i32, [vp, pd, pi]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/parafem_amd.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void\s*\*|const char\s*\*)\s*(pf_\w+)\s*\(", txt, re.M)))
+
+
+def status_string(s: int) -> str:
+    try:
+        return lib().pf_status_string(int(s)).decode()
+    except LibraryNotBuilt:
+        return "library not built"
+
+
+def _check(rc: int, what: str, ok=(PF_OK,)):
+    if rc not in ok:
+        raise PFError(rc, what)
+    return rc
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pi(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class DeviceArray:
+    """A device buffer of float64 owned by a context (pf_dev_alloc)."""
+
+    def __init__(self, ctx: "Context", n: int):
+        self.ctx = ctx
+        self.n = int(n)
+        p = C.c_void_p()
+        _check(lib().pf_dev_alloc(ctx.h, max(self.n, 1) * 8, C.byref(p)), "pf_dev_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx, a):
+        a = _f64(a).reshape(-1)
+        d = cls(ctx, a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = _f64(a).reshape(-1)
+        assert a.size == self.n
+        _check(lib().pf_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), "pf_memcpy_h2d")
+
+    def download(self):
+        out = np.empty(self.n)
+        _check(lib().pf_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), "pf_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().pf_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class SolveResult:
+    iters: int
+    converged: bool
+    crit: float
+    rr0: float
+    rr: float
+    refinements: int
+    status: int
+
+
+class Context:
+    """One GPU's EBE-PCG state (pf_create .. pf_destroy)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        _check(L.pf_create(int(device), C.byref(h)), "pf_create")
+        self.h = h.value
+        self.nels = self.ntot = self.neq = self.nod = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pf_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ plumbing
+    @property
+    def stream(self) -> int:
+        return lib().pf_get_stream(self.h) or 0
+
+    def set_stream(self, stream_handle: int | None):
+        _check(lib().pf_set_stream(self.h, stream_handle or None), "pf_set_stream")
+
+    def sync(self):
+        _check(lib().pf_sync(self.h), "pf_sync")
+
+    def mem_used(self) -> int:
+        n = C.c_size_t()
+        _check(lib().pf_mem_used(self.h, C.byref(n)), "pf_mem_used")
+        return n.value
+
+    # -------------------------------------------------------------- set-up
+    def mesh_set(self, g_coord: np.ndarray):
+        """g_coord_pp as (nels, 3, nod)."""
+        g = _f64(g_coord)
+        nels, three, nod = g.shape
+        assert three == 3
+        _check(lib().pf_mesh_set(self.h, nod, nels, _pd(g)), "pf_mesh_set")
+        self.nels, self.nod = nels, nod
+
+    def steer_set(self, g_g: np.ndarray, neq: int):
+        """g_g_pp as (nels, ntot), 1-based equations, 0 = restrained."""
+        g = _i32(g_g)
+        nels, ntot = g.shape
+        _check(lib().pf_steer_set(self.h, ntot, nels, int(neq), _pi(g)), "pf_steer_set")
+        self.nels, self.ntot, self.neq = nels, ntot, int(neq)
+
+    def assemble_elastic(self, e: float, v: float, e_elem=None, store: str = "fp64"):
+        ee = None if e_elem is None else _f64(e_elem)
+        _check(lib().pf_assemble_elastic(self.h, float(e), float(v), None if ee is None else _pd(ee),
+                                         STORE[store]), "pf_assemble_elastic")
+
+    def assemble_laplace(self, kx=1.0, ky=1.0, kz=1.0, k_elem=None, store: str = "fp64"):
+        kk = None if k_elem is None else _f64(k_elem)
+        _check(lib().pf_assemble_laplace(self.h, float(kx), float(ky), float(kz),
+                                         None if kk is None else _pd(kk), STORE[store]),
+               "pf_assemble_laplace")
+
+    def storkm_set(self, storkm: np.ndarray, store: str = "fp64"):
+        k = _f64(storkm)
+        assert k.shape == (self.nels, self.ntot, self.ntot)
+        _check(lib().pf_storkm_set(self.h, _pd(k), STORE[store]), "pf_storkm_set")
+
+    def storkm_get(self) -> np.ndarray:
+        out = np.empty((self.nels, self.ntot, self.ntot))
+        _check(lib().pf_storkm_get(self.h, _pd(out)), "pf_storkm_get")
+        return out
+
+    def diag_get(self) -> np.ndarray:
+        out = np.empty(self.neq)
+        _check(lib().pf_diag_get(self.h, _pd(out)), "pf_diag_get")
+        return out
+
+    def fixed_set(self, no_f, penalty: float = 1.0e20) -> np.ndarray:
+        """p123 penalty fixed freedoms; returns store_pp."""
+        nf = _i32(no_f)
+        store = np.zeros(max(nf.size, 1))
+        _check(lib().pf_fixed_set(self.h, nf.size, _pi(nf) if nf.size else None, float(penalty),
+                                  _pd(store)), "pf_fixed_set")
+        return store[: nf.size]
+
+    # ------------------------------------------------------ drop-in layer
+    @staticmethod
+    def _need(buf: DeviceArray, n: int):
+        """Host-side size check of a device buffer before a kernel uses it."""
+        if not isinstance(buf, DeviceArray) or buf.n != n or not buf.ptr:
+            raise ValueError(f"device buffer of {n} float64 expected, got {getattr(buf, 'n', None)}")
+
+    def gather(self, p_pp: DeviceArray, pmul_pp: DeviceArray):
+        """GATHER(p_pp, pmul_pp)."""
+        self._need(p_pp, self.neq); self._need(pmul_pp, self.ntot * self.nels)
+        _check(lib().pf_gather(self.h, p_pp.ptr, pmul_pp.ptr), "pf_gather")
+
+    def elem_matvec(self, pmul_pp: DeviceArray, utemp_pp: DeviceArray):
+        self._need(pmul_pp, self.ntot * self.nels); self._need(utemp_pp, self.ntot * self.nels)
+        _check(lib().pf_elem_matvec(self.h, pmul_pp.ptr, utemp_pp.ptr), "pf_elem_matvec")
+
+    def scatter(self, u_pp: DeviceArray, utemp_pp: DeviceArray):
+        """SCATTER(u_pp, utemp_pp): u_pp accumulates."""
+        self._need(u_pp, self.neq); self._need(utemp_pp, self.ntot * self.nels)
+        _check(lib().pf_scatter(self.h, u_pp.ptr, utemp_pp.ptr), "pf_scatter")
+
+    def scatter_noadd(self, utemp_pp: DeviceArray, u_pp: DeviceArray):
+        """SCATTER_NOADD(utemp_pp, u_pp): assignment (note the argument order)."""
+        self._need(u_pp, self.neq); self._need(utemp_pp, self.ntot * self.nels)
+        _check(lib().pf_scatter_noadd(self.h, utemp_pp.ptr, u_pp.ptr), "pf_scatter_noadd")
+
+    def dot_product_p(self, a: DeviceArray, b: DeviceArray) -> float:
+        self._need(b, a.n)
+        out = C.c_double()
+        _check(lib().pf_dot_product_p(self.h, a.ptr, b.ptr, a.n, C.byref(out)), "pf_dot_product_p")
+        return out.value
+
+    def sum_p(self, a: DeviceArray) -> float:
+        out = C.c_double()
+        _check(lib().pf_sum_p(self.h, a.ptr, a.n, C.byref(out)), "pf_sum_p")
+        return out.value
+
+    def checon_par(self, xnew: DeviceArray, tol: float, x: DeviceArray) -> bool:
+        """checon_par(loads, tol, converged, oldlds): returns converged, x = xnew."""
+        self._need(x, xnew.n)
+        conv = C.c_int()
+        _check(lib().pf_checon_par(self.h, xnew.ptr, float(tol), C.byref(conv), x.ptr, xnew.n),
+               "pf_checon_par")
+        return bool(conv.value)
+
+    # ----------------------------------------------------------- the solve
+    @staticmethod
+    def _opts(limit, tol, stop, precision, check_every=0, refine_every=0):
+        return PcgOpts(int(limit), float(tol), STOP[stop], PREC[precision], int(check_every),
+                       int(refine_every))
+
+    def pcg_solve(self, r0, limit: int, tol: float, stop: str = "checon", precision: str = "fp64",
+                  check_every: int = 0, refine_every: int = 0):
+        """The p121/p123 PCG loop; returns (xnew, SolveResult)."""
+        r = _f64(r0)
+        assert r.size == self.neq
+        x = np.zeros(self.neq)
+        o = self._opts(limit, tol, stop, precision, check_every, refine_every)
+        res = PcgResult()
+        rc = lib().pf_pcg_solve(self.h, _pd(r), _pd(x), C.byref(o), C.byref(res))
+        _check(rc, "pf_pcg_solve", ok=(PF_OK, PF_NOT_CONVERGED, PF_ERR_BREAKDOWN))
+        return x, self._res(res)
+
+    @staticmethod
+    def _res(res):
+        return SolveResult(res.iters, bool(res.converged), res.crit, res.rr0, res.rr,
+                           res.refinements, res.status)
+
+    def pcg_begin(self, r0, limit: int, tol: float, stop: str = "checon", precision: str = "fp64",
+                  refine_every: int = 0, on_device: bool = False):
+        o = self._opts(limit, tol, stop, precision, 0, refine_every)
+        if on_device:
+            _check(lib().pf_pcg_begin(self.h, C.c_void_p(int(r0)), 1, C.byref(o)), "pf_pcg_begin")
+        else:
+            r = _f64(r0)
+            _check(lib().pf_pcg_begin(self.h, r.ctypes.data, 0, C.byref(o)), "pf_pcg_begin")
+
+    def pcg_run(self, n: int):
+        _check(lib().pf_pcg_run(self.h, int(n)), "pf_pcg_run")
+
+    def pcg_poll(self) -> SolveResult:
+        res = PcgResult()
+        _check(lib().pf_pcg_poll(self.h, C.byref(res)), "pf_pcg_poll")
+        return self._res(res)
+
+    def pcg_x(self) -> np.ndarray:
+        x = np.empty(self.neq)
+        _check(lib().pf_pcg_x(self.h, x.ctypes.data, 0), "pf_pcg_x")
+        return x
+
+    def timing_enable(self, on: bool = True):
+        """Per-kernel HIP-event timing of pcg_run (resets the accumulators)."""
+        _check(lib().pf_pcg_timing_enable(self.h, int(bool(on))), "pf_pcg_timing_enable")
+
+    def timing_get(self):
+        """{kernel: (total ms, launches)} for ebe, scatter, update, pupdate."""
+        ms = np.zeros(4)
+        cnt = np.zeros(4, dtype=np.int32)
+        _check(lib().pf_pcg_timing_get(self.h, _pd(ms), _pi(cnt)), "pf_pcg_timing_get")
+        return {k: (float(ms[i]), int(cnt[i]))
+                for i, k in enumerate(("ebe", "scatter", "update", "pupdate"))}
+
+    def apply(self, x) -> np.ndarray:
+        """y = K x (FP64, with p123 fixed rows)."""
+        xx = _f64(x)
+        y = np.empty(self.neq)
+        _check(lib().pf_apply(self.h, _pd(xx), _pd(y)), "pf_apply")
+        return y
+
+
+def reduce_grid(n: int) -> int:
+    return lib().pf_reduce_grid(int(n))

@@ -1,0 +1,293 @@
+// Fused element assembly kernels (p121 loop A, p123 element loop).
+//
+// One 64-lane wavefront builds one element: SAMPLE's 2x2x2 Gauss points,
+// SHAPE_DER at each point, the jacobian, its determinant and inverse, global
+// derivatives, then the element matrix, written once to HBM in the
+// reference's storkm_pp(ntot,ntot,nels) column-major layout.  Nothing but the
+// element coordinates is read and nothing but the matrix is written.
+//
+// Reference: src/programs/5th_ed/p121/p121.f90:54-64 (elastic),
+// src/programs/5th_ed/p123/p123.f90:70-84 (Laplace); element routines in
+// src/modules/shared/new_library.f90 (SAMPLE :1406-1433, SHAPE_DER hex8
+// :769-794, hex20 :865-896, BEEMAT :976-993, DEEMAT :1671-1686) and
+// src/modules/mpi/maths.f90 (INVERT :525-547, DETERMINANT :615-619).
+//
+// Elastic element: with D from DEEMAT (lambda = dee(1,2), mu = dee(4,4)) the
+// 3x3 block of B^T D B coupling node a, component i to node b, component j
+// is  lambda*da_i*db_j + mu*da_j*db_i + mu*delta_ij*(da . db)  (d = global
+// shape-function derivatives), which is the product BEEMAT/DEEMAT/MATMUL
+// form without the zero entries of B.  Entries are summed over the Gauss
+// points in the reference's order (point 1..8).
+#include "pf_internal.h"
+
+namespace {
+
+constexpr int WAVES = 4;   // elements per block
+
+// SHAPE_DER for hex8 (nod = 8) and hex20 (nod = 20) at (xi, eta, zeta):
+// derivative k (0..2) of shape function l (0-based).
+template <int NOD>
+__device__ __forceinline__ void shape_der3(double xi, double eta, double zeta, int l, double out[3])
+{
+    if constexpr (NOD == 8) {
+        const double etam = 1.0 - eta, xim = 1.0 - xi, zetam = 1.0 - zeta;
+        const double etap = eta + 1.0, xip = xi + 1.0, zetap = zeta + 1.0;
+        // Table of SHAPE_DER case(8): der(k,l) = s * f1 * f2.
+        switch (l) {
+        case 0: out[0] = -0.125 * etam * zetam; out[1] = -0.125 * xim * zetam; out[2] = -0.125 * xim * etam; break;
+        case 1: out[0] = -0.125 * etam * zetap; out[1] = -0.125 * xim * zetap; out[2] = 0.125 * xim * etam; break;
+        case 2: out[0] = 0.125 * etam * zetap;  out[1] = -0.125 * xip * zetap; out[2] = 0.125 * xip * etam; break;
+        case 3: out[0] = 0.125 * etam * zetam;  out[1] = -0.125 * xip * zetam; out[2] = -0.125 * xip * etam; break;
+        case 4: out[0] = -0.125 * etap * zetam; out[1] = 0.125 * xim * zetam;  out[2] = -0.125 * xim * etap; break;
+        case 5: out[0] = -0.125 * etap * zetap; out[1] = 0.125 * xim * zetap;  out[2] = 0.125 * xim * etap; break;
+        case 6: out[0] = 0.125 * etap * zetap;  out[1] = 0.125 * xip * zetap;  out[2] = 0.125 * xip * etap; break;
+        default: out[0] = 0.125 * etap * zetam; out[1] = 0.125 * xip * zetam; out[2] = -0.125 * xip * etap; break;
+        }
+    } else {
+        // Node sign tables of SHAPE_DER case(20).
+        constexpr int xii[20] = {-1, -1, -1, 0, 1, 1, 1, 0, -1, -1, 1, 1, -1, -1, -1, 0, 1, 1, 1, 0};
+        constexpr int etai[20] = {-1, -1, -1, -1, -1, -1, -1, -1, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1};
+        constexpr int zetai[20] = {-1, 0, 1, 1, 1, 0, -1, -1, -1, 1, 1, -1, -1, 0, 1, 1, 1, 0, -1, -1};
+        const double sx = xii[l], sy = etai[l], sz = zetai[l];
+        const double xi0 = xi * sx, eta0 = eta * sy, zeta0 = zeta * sz;
+        const int n = l + 1;
+        if (n == 4 || n == 8 || n == 16 || n == 20) {
+            out[0] = -.5 * xi * (1. + eta0) * (1. + zeta0);
+            out[1] = .25 * sy * (1. - xi * xi) * (1. + zeta0);
+            out[2] = .25 * sz * (1. - xi * xi) * (1. + eta0);
+        } else if (n >= 9 && n <= 12) {
+            out[0] = .25 * sx * (1. - eta * eta) * (1. + zeta0);
+            out[1] = -.5 * eta * (1. + xi0) * (1. + zeta0);
+            out[2] = .25 * sz * (1. + xi0) * (1. - eta * eta);
+        } else if (n == 2 || n == 6 || n == 14 || n == 18) {
+            out[0] = .25 * sx * (1. + eta0) * (1. - zeta * zeta);
+            out[1] = .25 * sy * (1. + xi0) * (1. - zeta * zeta);
+            out[2] = -.5 * zeta * (1. + xi0) * (1. + eta0);
+        } else {
+            out[0] = .125 * sx * (1. + eta0) * (1. + zeta0) * (2. * xi0 + eta0 + zeta0 - 1.);
+            out[1] = .125 * sy * (1. + xi0) * (1. + zeta0) * (xi0 + 2. * eta0 + zeta0 - 1.);
+            out[2] = .125 * sz * (1. + xi0) * (1. + eta0) * (xi0 + eta0 + 2. * zeta0 - 1.);
+        }
+    }
+}
+
+// Per-wave LDS scratch for one element.
+template <int NOD>
+struct ElemLds {
+    double coord[NOD * 3];      // (nod,3) column-major, as g_coord_pp(:,:,iel)
+    double der[8][3][NOD];      // local derivatives per Gauss point
+    double jac[8][9];           // jacobian (3x3 column-major), then its inverse
+    double dw[8];               // det * weight
+    double deriv[8][3][NOD];    // global derivatives
+};
+
+// Stages shared by both element types: Gauss points, local derivatives,
+// jacobian, DETERMINANT/INVERT, global derivatives.  All four waves of the
+// block call it (block-uniform barriers); `valid` masks elements past nels.
+template <int NOD>
+__device__ __forceinline__ void element_geometry(ElemLds<NOD> &L, const double *__restrict__ gc, int lane, bool valid)
+{
+    const double r3 = 1.0 / sqrt(3.0);
+    if (valid)
+        for (int k = lane; k < NOD * 3; k += 64) L.coord[k] = gc[k];
+    // SAMPLE hexahedron nip=8: point g has (xi,eta,zeta) signs from the bits
+    // of g: xi = + for g < 4, eta = + for g%4 < 2, zeta = + for g even.
+    for (int k = lane; k < 8 * NOD; k += 64) {
+        const int g = k / NOD, l = k % NOD;
+        const double xi = (g < 4) ? r3 : -r3;
+        const double eta = ((g & 3) < 2) ? r3 : -r3;
+        const double zeta = ((g & 1) == 0) ? r3 : -r3;
+        double d3[3];
+        shape_der3<NOD>(xi, eta, zeta, l, d3);
+        L.der[g][0][l] = d3[0];
+        L.der[g][1][l] = d3[1];
+        L.der[g][2][l] = d3[2];
+    }
+    __syncthreads();
+    // jac = MATMUL(der, coord): 72 entries (8 points x 3 x 3).
+    for (int k = lane; k < 72; k += 64) {
+        const int g = k / 9, ab = k % 9, a = ab % 3, b = ab / 3;
+        double s = 0.0;
+        for (int l = 0; l < NOD; ++l) s += L.der[g][a][l] * L.coord[l + NOD * b];
+        L.jac[g][a + 3 * b] = s;
+    }
+    __syncthreads();
+    if (lane < 8) {
+        double *m = L.jac[lane];
+#define M(a, b) m[((a)-1) + 3 * ((b)-1)]
+        double det = M(1, 1) * (M(2, 2) * M(3, 3) - M(3, 2) * M(2, 3));
+        det = det - M(1, 2) * (M(2, 1) * M(3, 3) - M(3, 1) * M(2, 3));
+        det = det + M(1, 3) * (M(2, 1) * M(3, 2) - M(3, 1) * M(2, 2));
+        const double j11 = M(2, 2) * M(3, 3) - M(3, 2) * M(2, 3);
+        const double j21 = -M(2, 1) * M(3, 3) + M(3, 1) * M(2, 3);
+        const double j31 = M(2, 1) * M(3, 2) - M(3, 1) * M(2, 2);
+        const double j12 = -M(1, 2) * M(3, 3) + M(3, 2) * M(1, 3);
+        const double j22 = M(1, 1) * M(3, 3) - M(3, 1) * M(1, 3);
+        const double j32 = -M(1, 1) * M(3, 2) + M(3, 1) * M(1, 2);
+        const double j13 = M(1, 2) * M(2, 3) - M(2, 2) * M(1, 3);
+        const double j23 = -M(1, 1) * M(2, 3) + M(2, 1) * M(1, 3);
+        const double j33 = M(1, 1) * M(2, 2) - M(2, 1) * M(1, 2);
+        M(1, 1) = j11 / det; M(1, 2) = j12 / det; M(1, 3) = j13 / det;
+        M(2, 1) = j21 / det; M(2, 2) = j22 / det; M(2, 3) = j23 / det;
+        M(3, 1) = j31 / det; M(3, 2) = j32 / det; M(3, 3) = j33 / det;
+#undef M
+        L.dw[lane] = det;   // SAMPLE weights are 1 for nip = 8
+    }
+    __syncthreads();
+    // deriv = MATMUL(jac^-1, der)
+    for (int k = lane; k < 8 * NOD; k += 64) {
+        const int g = k / NOD, l = k % NOD;
+        const double *ji = L.jac[g];
+        const double d0 = L.der[g][0][l], d1 = L.der[g][1][l], d2 = L.der[g][2][l];
+        for (int a = 0; a < 3; ++a) L.deriv[g][a][l] = ji[a] * d0 + ji[a + 3] * d1 + ji[a + 6] * d2;
+    }
+    __syncthreads();
+}
+
+template <int NOD>
+__global__ __launch_bounds__(64 * WAVES) void k_assemble_elastic(
+    const double *__restrict__ g_coord, const double *__restrict__ e_elem, double lam0, double mu0,
+    double e0, double *__restrict__ K, int nels)
+{
+    constexpr int NTOT = 3 * NOD;
+    __shared__ ElemLds<NOD> lds[WAVES];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long e = (long)blockIdx.x * WAVES + wave;
+    const bool valid = e < nels;
+    ElemLds<NOD> &L = lds[wave];
+    element_geometry<NOD>(L, g_coord + (valid ? e : 0) * (NOD * 3), lane, valid);
+    if (!valid || lane >= NTOT) return;
+    const double E = e_elem ? e_elem[e] : e0;
+    const double lam = lam0 * E, mu = mu0 * E;
+    const int a = lane / 3, ci = lane - 3 * (lane / 3);
+    double A[8], B0[8], B1[8], B2[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+        const double w = L.dw[g];
+        const double da0 = L.deriv[g][0][a], da1 = L.deriv[g][1][a], da2 = L.deriv[g][2][a];
+        const double dci = ci == 0 ? da0 : (ci == 1 ? da1 : da2);
+        A[g] = lam * w * dci;
+        B0[g] = mu * w * da0;
+        B1[g] = mu * w * da1;
+        B2[g] = mu * w * da2;
+    }
+    double *Ke = K + e * (long)(NTOT * NTOT) + lane;
+    for (int b = 0; b < NOD; ++b) {
+        double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+#pragma unroll
+        for (int g = 0; g < 8; ++g) {
+            const double db0 = L.deriv[g][0][b], db1 = L.deriv[g][1][b], db2 = L.deriv[g][2][b];
+            const double dbci = ci == 0 ? db0 : (ci == 1 ? db1 : db2);
+            const double S = B0[g] * db0 + B1[g] * db1 + B2[g] * db2;
+            double t0 = db0 * A[g] + dbci * B0[g];
+            double t1 = db1 * A[g] + dbci * B1[g];
+            double t2 = db2 * A[g] + dbci * B2[g];
+            if (ci == 0) t0 += S;
+            else if (ci == 1) t1 += S;
+            else t2 += S;
+            o0 += t0;
+            o1 += t1;
+            o2 += t2;
+        }
+        Ke[(3 * b + 0) * NTOT] = o0;
+        Ke[(3 * b + 1) * NTOT] = o1;
+        Ke[(3 * b + 2) * NTOT] = o2;
+    }
+}
+
+// p123: kc{x,y,z} = sum_g d{x,y,z}_a d{x,y,z}_b det w; storkc = kx*kcx + ky*kcy + kz*kcz.
+__global__ __launch_bounds__(64 * WAVES) void k_assemble_laplace8(
+    const double *__restrict__ g_coord, const double *__restrict__ k_elem, double kx, double ky,
+    double kz, double *__restrict__ K, int nels)
+{
+    __shared__ ElemLds<8> lds[WAVES];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long e = (long)blockIdx.x * WAVES + wave;
+    const bool valid = e < nels;
+    ElemLds<8> &L = lds[wave];
+    element_geometry<8>(L, g_coord + (valid ? e : 0) * 24, lane, valid);
+    if (!valid) return;
+    const int a = lane & 7, b = lane >> 3;   // entry (a,b) at column-major index a + 8b = lane
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+        const double w = L.dw[g];
+        cx += L.deriv[g][0][a] * L.deriv[g][0][b] * w;
+        cy += L.deriv[g][1][a] * L.deriv[g][1][b] * w;
+        cz += L.deriv[g][2][a] * L.deriv[g][2][b] * w;
+    }
+    const double s = k_elem ? k_elem[e] : 1.0;
+    K[e * 64 + lane] = cx * (kx * s) + cy * (ky * s) + cz * (kz * s);
+}
+
+// hi = fl32(K), lo = fl32(K - hi): hi + lo carries ~48 significant bits.
+__global__ void k_split_fp32(const double *__restrict__ K, float *__restrict__ hi, float *__restrict__ lo, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double k = K[i];
+        const float h = (float)k;
+        hi[i] = h;
+        lo[i] = (float)(k - (double)h);
+    }
+}
+
+// Assembled diagonal: diag(q) = sum over the slots of q, ascending element,
+// of storkm(i,i,e) (p121.f90:65-69 then SCATTER).  diag has a leading 0 slot.
+__global__ void k_diag(const int *__restrict__ ptr, const int *__restrict__ slot, const double *__restrict__ K,
+                       int ntot, double *__restrict__ diag, int neq)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < neq; q += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = ptr[q]; k < ptr[q + 1]; ++k) {
+            const long sl = slot[k];
+            const long e = sl / ntot, i = sl - e * ntot;
+            s = __dadd_rn(s, K[e * ntot * ntot + i * ntot + i]);
+        }
+        diag[q + 1] = s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) diag[0] = 0.0;
+}
+
+}  // namespace
+
+int pf_launch_assemble_elastic(pf_ctx *c, double lam, double mu, double e0, const double *e_elem)
+{
+    const int grid = (c->nels + WAVES - 1) / WAVES;
+    if (c->nod == 20)
+        hipLaunchKernelGGL(k_assemble_elastic<20>, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
+                           e_elem, lam, mu, e0, c->K64, c->nels);
+    else if (c->nod == 8)
+        hipLaunchKernelGGL(k_assemble_elastic<8>, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
+                           e_elem, lam, mu, e0, c->K64, c->nels);
+    else
+        return PF_ERR_ARG;
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem)
+{
+    if (c->nod != 8) return PF_ERR_ARG;
+    const int grid = (c->nels + WAVES - 1) / WAVES;
+    hipLaunchKernelGGL(k_assemble_laplace8, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord, k_elem, kx,
+                       ky, kz, c->K64, c->nels);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+int pf_launch_split_fp32(pf_ctx *c)
+{
+    const long n = (long)c->ntot * c->ntot * c->nels;
+    hipLaunchKernelGGL(k_split_fp32, dim3(4096), dim3(256), 0, c->stream, c->K64, c->K32hi, c->K32lo, n);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+int pf_launch_diag(pf_ctx *c, double *diag)
+{
+    const int grid = (c->neq + 255) / 256 < 8192 ? (c->neq + 255) / 256 : 8192;
+    hipLaunchKernelGGL(k_diag, dim3(grid), dim3(256), 0, c->stream, c->csr_ptr, c->csr_slot, c->K64, c->ntot,
+                       diag, c->neq);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}

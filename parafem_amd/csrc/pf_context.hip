@@ -1,0 +1,478 @@
+// Context, device memory and set-up entry points of libparafem_amd.
+//
+// One pf_ctx drives one GPU from one host thread.  It owns every device
+// buffer of a problem: element coordinates, steering, the transposed
+// steering used by the deterministic scatter, element matrices and the PCG
+// vectors.  Host arrays passed in are only read during the call.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pf_internal.h"
+
+static thread_local char g_last_error[512];
+
+int pf_set_hip_error(hipError_t e, const char *what, const char *file, int line)
+{
+    snprintf(g_last_error, sizeof g_last_error, "%s failed: %s (%s:%d)", what,
+             hipGetErrorString(e), file, line);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return PF_ERR_OOM;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return PF_ERR_NODEVICE;
+    return PF_ERR_HIP;
+}
+
+int pf_alloc(pf_ctx *c, void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return pf_set_hip_error(e, "hipMalloc", __FILE__, __LINE__);
+    }
+    c->bytes += bytes;
+    return PF_OK;
+}
+
+void pf_free(pf_ctx *c, void *p, size_t bytes)
+{
+    if (!p) return;
+    (void)hipFree(p);
+    if (bytes == 0) bytes = 256;
+    c->bytes -= std::min(c->bytes, bytes);
+}
+
+template <class T>
+static void free_arr(pf_ctx *c, T *&p, size_t n)
+{
+    pf_free(c, p, n * sizeof(T));
+    p = nullptr;
+}
+
+bool pf_timing_begin(pf_ctx *c, int k, hipEvent_t *start, hipEvent_t *stop)
+{
+    if (!c->timing) return false;
+    while (c->ev_pool.size() < c->ev_next + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        c->ev_pool.push_back(e);
+    }
+    const int a = (int)c->ev_next, b = (int)c->ev_next + 1;
+    c->ev_next += 2;
+    c->ev_used[k].emplace_back(a, b);
+    *start = c->ev_pool[a];
+    *stop = c->ev_pool[b];
+    return true;
+}
+
+int pf_grid_for(int64_t n)
+{
+    int64_t g = (n + PF_BLOCK - 1) / PF_BLOCK;
+    if (g < 1) g = 1;
+    if (g > PF_REDUCE_MAX_GRID) g = PF_REDUCE_MAX_GRID;
+    return (int)g;
+}
+
+extern "C" {
+
+int pf_version(void) { return 100; }
+
+const char *pf_status_string(int s)
+{
+    switch (s) {
+    case PF_OK: return "ok";
+    case PF_ERR_ARG: return "bad argument or call order";
+    case PF_ERR_HIP: return g_last_error[0] ? g_last_error : "HIP runtime error";
+    case PF_ERR_OOM: return "device out of memory";
+    case PF_ERR_STATE: return "missing set-up call";
+    case PF_NOT_CONVERGED: return "PCG reached its iteration limit";
+    case PF_ERR_BREAKDOWN: return "PCG breakdown (non-finite or zero curvature)";
+    case PF_ERR_NODEVICE: return "no HIP device";
+    default: return "unknown status";
+    }
+}
+
+int pf_reduce_grid(int64_t n) { return pf_grid_for(n); }
+
+int pf_create(int device, pf_ctx **out)
+{
+    if (!out) return PF_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return PF_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) return PF_ERR_NODEVICE;
+    PF_HIP(hipSetDevice(device));
+    pf_ctx *c = new pf_ctx();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return pf_set_hip_error(e, "hipStreamCreate", __FILE__, __LINE__);
+    }
+    c->stream = c->own_stream;
+    int rc = pf_alloc(c, (void **)&c->state, sizeof(pf_state));
+    if (!rc) rc = pf_alloc(c, (void **)&c->partials, sizeof(double) * (PF_NQ * PF_REDUCE_MAX_GRID + 8));
+    if (!rc && hipHostMalloc((void **)&c->state_host, sizeof(pf_state), hipHostMallocDefault) != hipSuccess) rc = PF_ERR_OOM;
+    if (!rc && hipHostMalloc((void **)&c->scalar_host, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = PF_ERR_OOM;
+    if (!rc) {
+        hipError_t z = hipMemsetAsync(c->state, 0, sizeof(pf_state), c->stream);
+        if (z == hipSuccess) z = hipStreamSynchronize(c->stream);
+        if (z != hipSuccess) rc = pf_set_hip_error(z, "init state", __FILE__, __LINE__);
+    }
+    if (rc) {
+        pf_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return PF_OK;
+}
+
+static void free_problem_vectors(pf_ctx *c)
+{
+    size_t nv = (size_t)c->neq + 1;
+    free_arr(c, c->x, nv); free_arr(c, c->r, nv); free_arr(c, c->p, nv);
+    free_arr(c, c->d, nv); free_arr(c, c->u, nv); free_arr(c, c->dinv, nv);
+    free_arr(c, c->diag, nv); free_arr(c, c->fixstore, nv);
+    c->nfix = 0;
+    c->pcg_ready = false;
+}
+
+static void free_matrices(pf_ctx *c)
+{
+    size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+    free_arr(c, c->K64, nk);
+    free_arr(c, c->K32hi, nk);
+    free_arr(c, c->K32lo, nk);
+    c->have_K = false;
+}
+
+int pf_destroy(pf_ctx *c)
+{
+    if (!c) return PF_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_problem_vectors(c);
+    free_matrices(c);
+    free_arr(c, c->g_coord, (size_t)c->nod * 3 * c->nels);
+    free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
+    free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
+    free_arr(c, c->csr_slot, (size_t)c->nnz);
+    free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
+    free_arr(c, c->partials, (size_t)PF_NQ * PF_REDUCE_MAX_GRID + 8);
+    pf_free(c, c->state, sizeof(pf_state));
+    if (c->state_host) (void)hipHostFree(c->state_host);
+    if (c->scalar_host) (void)hipHostFree(c->scalar_host);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return PF_OK;
+}
+
+int pf_set_stream(pf_ctx *c, void *s)
+{
+    if (!c) return PF_ERR_ARG;
+    PF_HIP(hipStreamSynchronize(c->stream));
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return PF_OK;
+}
+
+void *pf_get_stream(pf_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int pf_sync(pf_ctx *c)
+{
+    if (!c) return PF_ERR_ARG;
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_dev_alloc(pf_ctx *c, size_t bytes, void **dptr)
+{
+    if (!c || !dptr) return PF_ERR_ARG;
+    PF_HIP(hipSetDevice(c->device));
+    return pf_alloc(c, dptr, bytes);
+}
+
+int pf_dev_free(pf_ctx *c, void *dptr)
+{
+    if (!c) return PF_ERR_ARG;
+    if (!dptr) return PF_OK;
+    PF_HIP(hipStreamSynchronize(c->stream));
+    size_t sz = 0;
+    if (hipMemPtrGetInfo(dptr, &sz) != hipSuccess) {
+        (void)hipGetLastError();
+        sz = 0;
+    }
+    pf_free(c, dptr, sz);
+    return PF_OK;
+}
+
+int pf_memcpy_h2d(pf_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c || (!dst && bytes) || (!src && bytes)) return PF_ERR_ARG;
+    PF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_memcpy_d2h(pf_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c || (!dst && bytes) || (!src && bytes)) return PF_ERR_ARG;
+    PF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_mem_used(pf_ctx *c, size_t *bytes)
+{
+    if (!c || !bytes) return PF_ERR_ARG;
+    *bytes = c->bytes;
+    return PF_OK;
+}
+
+// Drop every buffer of the current problem (a context can be reused for a
+// new mesh of another size).
+static void reset_problem(pf_ctx *c)
+{
+    (void)hipStreamSynchronize(c->stream);
+    free_problem_vectors(c);
+    free_matrices(c);
+    free_arr(c, c->g_coord, (size_t)c->nod * 3 * c->nels);
+    free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
+    free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
+    free_arr(c, c->csr_slot, (size_t)c->nnz);
+    free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
+    c->nod = c->nels = c->ntot = c->neq = 0;
+    c->nnz = 0;
+}
+
+int pf_mesh_set(pf_ctx *c, int nod, int nels, const double *g_coord)
+{
+    if (!c || !g_coord || (nod != 8 && nod != 20) || nels <= 0) return PF_ERR_ARG;
+    PF_HIP(hipSetDevice(c->device));
+    if (c->nels && (c->nels != nels || (c->ntot && c->ntot % nod != 0))) reset_problem(c);
+    free_arr(c, c->g_coord, (size_t)c->nod * 3 * c->nels);
+    c->nod = nod;
+    c->nels = nels;
+    size_t n = (size_t)nod * 3 * nels;
+    int rc = pf_alloc(c, (void **)&c->g_coord, n * sizeof(double));
+    if (rc) return rc;
+    PF_HIP(hipMemcpyAsync(c->g_coord, g_coord, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_steer_set(pf_ctx *c, int ntot, int nels, int neq, const int *g_g)
+{
+    if (!c || !g_g || ntot <= 0 || ntot > 64 || nels <= 0 || neq <= 0) return PF_ERR_ARG;
+    if (c->nels && c->nels != nels) return PF_ERR_ARG;   // pf_mesh_set first for a new mesh
+    const size_t nslot = (size_t)ntot * nels;
+    if (nslot >= (size_t)INT32_MAX) return PF_ERR_ARG;
+    // Transposed steering: for each equation the element slots that touch it
+    // in ascending slot (= ascending element) order, by a counting sort.
+    std::vector<int> cnt((size_t)neq + 1, 0);
+    for (size_t k = 0; k < nslot; ++k) {
+        int g = g_g[k];
+        if (g < 0 || g > neq) return PF_ERR_ARG;
+        if (g) cnt[g]++;
+    }
+    std::vector<int> ptr((size_t)neq + 1, 0);
+    for (int q = 0; q < neq; ++q) {
+        if (cnt[q + 1] == 0) return PF_ERR_ARG;       // equation touched by no element
+        ptr[q + 1] = ptr[q] + cnt[q + 1];
+    }
+    const int64_t nnz = ptr[neq];
+    std::vector<int> slot((size_t)std::max<int64_t>(nnz, 1));
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t k = 0; k < nslot; ++k) {
+        int g = g_g[k];
+        if (g) slot[fill[g - 1]++] = (int)k;
+    }
+    PF_HIP(hipSetDevice(c->device));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    if (c->ntot && c->ntot != ntot) free_matrices(c);
+    free_problem_vectors(c);
+    free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
+    free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
+    free_arr(c, c->csr_slot, (size_t)c->nnz);
+    free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
+    c->ntot = ntot;
+    c->nels = nels;
+    c->neq = neq;
+    c->nnz = nnz;
+    int rc = pf_alloc(c, (void **)&c->g_g, nslot * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&c->csr_ptr, ((size_t)neq + 1) * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&c->csr_slot, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&c->utemp, nslot * sizeof(double));
+    const size_t nv = (size_t)neq + 1;
+    double **vecs[] = {&c->x, &c->r, &c->p, &c->d, &c->u, &c->dinv, &c->diag};
+    for (double **v : vecs)
+        if (!rc) rc = pf_alloc(c, (void **)v, nv * sizeof(double));
+    if (rc) return rc;
+    PF_HIP(hipMemcpyAsync(c->g_g, g_g, nslot * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    PF_HIP(hipMemcpyAsync(c->csr_ptr, ptr.data(), nv * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    PF_HIP(hipMemcpyAsync(c->csr_slot, slot.data(), (size_t)nnz * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    for (double **v : vecs) PF_HIP(hipMemsetAsync(*v, 0, nv * sizeof(double), c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+static int alloc_matrices(pf_ctx *c, int store)
+{
+    if (store != PF_STORE_FP64 && store != PF_STORE_FP32) return PF_ERR_ARG;
+    free_matrices(c);
+    size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+    int rc = pf_alloc(c, (void **)&c->K64, nk * sizeof(double));
+    c->store = store;
+    return rc;
+}
+
+// After a FP64 assembly or upload: build the FP32 split for FP32 storage and
+// the assembled diagonal (ascending element order, as SCATTER adds it).
+static int finish_matrices(pf_ctx *c, int store)
+{
+    int rc;
+    if (store == PF_STORE_FP32) {
+        size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+        rc = pf_alloc(c, (void **)&c->K32hi, nk * sizeof(float));
+        if (!rc) rc = pf_alloc(c, (void **)&c->K32lo, nk * sizeof(float));
+        if (!rc) rc = pf_launch_split_fp32(c);
+        if (rc) return rc;
+    }
+    rc = pf_launch_diag(c, c->diag);
+    if (rc) return rc;
+    c->have_K = true;
+    c->pcg_ready = false;
+    free_arr(c, c->fixstore, (size_t)c->neq + 1);
+    c->nfix = 0;
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_assemble_elastic(pf_ctx *c, double e, double v, const double *e_elem, int store)
+{
+    if (!c) return PF_ERR_ARG;
+    if (!c->g_coord || !c->g_g) return PF_ERR_STATE;
+    if (c->ntot != 3 * c->nod) return PF_ERR_ARG;
+    if (!(v > -1.0 && v < 0.5)) return PF_ERR_ARG;
+    PF_HIP(hipSetDevice(c->device));
+    int rc = alloc_matrices(c, store);
+    if (rc) return rc;
+    double *ee = nullptr;
+    if (e_elem) {
+        rc = pf_alloc(c, (void **)&ee, sizeof(double) * c->nels);
+        if (rc) return rc;
+        PF_HIP(hipMemcpyAsync(ee, e_elem, sizeof(double) * c->nels, hipMemcpyHostToDevice, c->stream));
+    }
+    // DEEMAT (new_library.f90:1671-1686): dee(1,2) = lambda, dee(4,4) = mu,
+    // formed with the same expressions so a uniform e reproduces dee exactly.
+    double v2 = v / (1.0 - v);
+    double vv = (1.0 - 2.0 * v) / (1.0 - v) * 0.5;
+    double f = 1.0 / (2.0 * (1.0 + v) * vv);   // dee = dee0 * e * f
+    rc = pf_launch_assemble_elastic(c, v2 * f, vv * f, e, ee);
+    if (ee) {
+        (void)hipStreamSynchronize(c->stream);
+        pf_free(c, ee, sizeof(double) * c->nels);
+    }
+    if (rc) return rc;
+    return finish_matrices(c, store);
+}
+
+int pf_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem, int store)
+{
+    if (!c) return PF_ERR_ARG;
+    if (!c->g_coord || !c->g_g) return PF_ERR_STATE;
+    if (c->ntot != c->nod) return PF_ERR_ARG;
+    PF_HIP(hipSetDevice(c->device));
+    int rc = alloc_matrices(c, store);
+    if (rc) return rc;
+    double *kk = nullptr;
+    if (k_elem) {
+        rc = pf_alloc(c, (void **)&kk, sizeof(double) * c->nels);
+        if (rc) return rc;
+        PF_HIP(hipMemcpyAsync(kk, k_elem, sizeof(double) * c->nels, hipMemcpyHostToDevice, c->stream));
+    }
+    rc = pf_launch_assemble_laplace(c, kx, ky, kz, kk);
+    if (kk) {
+        (void)hipStreamSynchronize(c->stream);
+        pf_free(c, kk, sizeof(double) * c->nels);
+    }
+    if (rc) return rc;
+    return finish_matrices(c, store);
+}
+
+int pf_storkm_set(pf_ctx *c, const double *storkm, int store)
+{
+    if (!c || !storkm) return PF_ERR_ARG;
+    if (!c->g_g) return PF_ERR_STATE;
+    PF_HIP(hipSetDevice(c->device));
+    int rc = alloc_matrices(c, store);
+    if (rc) return rc;
+    size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+    PF_HIP(hipMemcpyAsync(c->K64, storkm, nk * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return finish_matrices(c, store);
+}
+
+int pf_storkm_get(pf_ctx *c, double *storkm)
+{
+    if (!c || !storkm) return PF_ERR_ARG;
+    if (!c->have_K) return PF_ERR_STATE;
+    size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+    PF_HIP(hipMemcpyAsync(storkm, c->K64, nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_diag_get(pf_ctx *c, double *diag)
+{
+    if (!c || !diag) return PF_ERR_ARG;
+    if (!c->have_K) return PF_ERR_STATE;
+    PF_HIP(hipMemcpyAsync(diag, c->diag + 1, sizeof(double) * c->neq, hipMemcpyDeviceToHost, c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_fixed_set(pf_ctx *c, int nfix, const int *no_f, double penalty, double *store_out)
+{
+    if (!c || nfix < 0 || (nfix && !no_f)) return PF_ERR_ARG;
+    if (!c->have_K) return PF_ERR_STATE;
+    for (int i = 0; i < nfix; ++i)
+        if (no_f[i] < 1 || no_f[i] > c->neq) return PF_ERR_ARG;
+    PF_HIP(hipSetDevice(c->device));
+    size_t nv = (size_t)c->neq + 1;
+    // Start from the element diagonal again so repeated calls do not stack.
+    int rc = pf_launch_diag(c, c->diag);
+    if (rc) return rc;
+    free_arr(c, c->fixstore, nv);
+    c->nfix = 0;
+    c->pcg_ready = false;
+    if (nfix == 0) return PF_OK;
+    std::vector<double> dg(nv);
+    PF_HIP(hipMemcpyAsync(dg.data(), c->diag, nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    std::vector<double> fs(nv, 0.0);
+    // p123.f90:120-125: diag(j) = diag(j) + penalty; store(i) = diag(j).
+    for (int i = 0; i < nfix; ++i) {
+        int j = no_f[i];
+        dg[j] = dg[j] + penalty;
+        fs[j] = dg[j];
+        if (store_out) store_out[i] = dg[j];
+    }
+    rc = pf_alloc(c, (void **)&c->fixstore, nv * sizeof(double));
+    if (rc) return rc;
+    PF_HIP(hipMemcpyAsync(c->diag, dg.data(), nv * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PF_HIP(hipMemcpyAsync(c->fixstore, fs.data(), nv * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    c->nfix = nfix;
+    return PF_OK;
+}
+
+}  // extern "C"

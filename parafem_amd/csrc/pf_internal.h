@@ -1,0 +1,105 @@
+// Internal declarations shared by the HIP translation units of libparafem_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/parafem_amd.h"
+
+#define PF_BLOCK 256                 // threads per block of the vector kernels
+#define PF_REDUCE_MAX_GRID 2048      // upper bound on reduction grid size G
+
+// Kernels of one PCG iteration, in launch order (pf_pcg_timing_get indices).
+enum { PF_K_EBE = 0, PF_K_SCATTER = 1, PF_K_UPDATE = 2, PF_K_PUPDATE = 3, PF_NKERNELS = 4 };
+
+// Number of per-iteration reduction quantities kept in the partials buffer.
+enum { PF_Q_PU = 0, PF_Q_RD = 1, PF_Q_RR = 2, PF_Q_MAXX = 3, PF_Q_MAXDX = 4, PF_NQ = 5 };
+
+// Device-resident PCG scalars.  Written only by the last block of the
+// reduction kernels; read at the start of the following kernels.
+struct pf_state {
+    double up;        // (r,d) of the current iterate (the reference's "up")
+    double alpha;
+    double beta;
+    double rr0;       // ||r0||^2
+    double rr;        // ||r||^2 (relres mode)
+    double crit;      // last stopping criterion value
+    double tol;
+    int    iters;
+    int    limit;
+    int    stop_mode;
+    int    converged;
+    int    frozen;    // 1 once converged, at the limit, or broken down
+    int    breakdown;
+    unsigned int ticket[4];   // last-block counters (returned to 0 by the last block)
+};
+
+struct pf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;      // stream in use
+    hipStream_t own_stream = nullptr;  // created by the context
+    size_t bytes = 0;                  // device bytes held
+
+    // mesh
+    int nod = 0, nels = 0;
+    double *g_coord = nullptr;         // (nod,3,nels)
+    // steering
+    int ntot = 0, neq = 0;
+    int *g_g = nullptr;                // (ntot,nels), 1-based, 0 = restrained
+    int *csr_ptr = nullptr;            // (neq+1) offsets into csr_slot
+    int *csr_slot = nullptr;           // element slots e*ntot+i, ascending
+    int64_t nnz = 0;
+    // element matrices
+    int store = PF_STORE_FP64;
+    double *K64 = nullptr;             // (ntot,ntot,nels) FP64
+    float *K32hi = nullptr;            // FP32 hi part
+    float *K32lo = nullptr;            // FP32 lo part
+    bool have_K = false;
+    // p123 fixed freedoms: fixstore[eq] = store value, 0 for free equations
+    double *fixstore = nullptr;
+    int nfix = 0;
+
+    // PCG vectors: length neq+1, entry 0 is a permanent zero so that a
+    // steering value of 0 gathers 0 without a branch.
+    double *x = nullptr, *r = nullptr, *p = nullptr, *d = nullptr, *u = nullptr;
+    double *dinv = nullptr, *diag = nullptr;
+    double *utemp = nullptr;           // (ntot,nels) element products
+    double *partials = nullptr;        // PF_NQ * PF_REDUCE_MAX_GRID
+    pf_state *state = nullptr;         // device
+    pf_state *state_host = nullptr;    // pinned mirror for polling
+    double *scalar_host = nullptr;     // pinned scratch for primitive results
+    pf_pcg_opts opts{};
+    bool pcg_ready = false;
+
+    // Optional per-kernel timing of pf_pcg_run: one event pair per launch.
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_used[PF_NKERNELS];   // (start, stop) indices
+    size_t ev_next = 0;
+};
+
+// Event pair handles for timing kernel `k` (returns false when timing is off).
+bool pf_timing_begin(pf_ctx *c, int k, hipEvent_t *start, hipEvent_t *stop);
+
+// error helpers -------------------------------------------------------------
+#define PF_HIP(call)                                        \
+    do {                                                    \
+        hipError_t _e = (call);                             \
+        if (_e != hipSuccess) return pf_set_hip_error(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+int pf_set_hip_error(hipError_t e, const char *what, const char *file, int line);
+int pf_alloc(pf_ctx *c, void **p, size_t bytes);
+void pf_free(pf_ctx *c, void *p, size_t bytes);
+int pf_grid_for(int64_t n);
+
+// kernels launched across translation units ---------------------------------
+int pf_launch_assemble_elastic(pf_ctx *c, double lam, double mu, double e0, const double *e_elem_dev);
+int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem_dev);
+int pf_launch_split_fp32(pf_ctx *c);
+int pf_launch_diag(pf_ctx *c, double *diag_dev);

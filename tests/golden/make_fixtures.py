@@ -1,0 +1,73 @@
+"""Derive small golden fixtures from the reference's committed example files.
+
+Run in a container that has the ParaFEM sources at /root/reference:
+    python tests/golden/make_fixtures.py
+Writes (data only, no reference source):
+  p121_demo.json        -- control data, neq, total load, published results of
+                           examples/5th_ed/p121/demo/p121_demo.res and a
+                           SHA-256 of the parsed deck arrays (g_num after
+                           ABAQUS2SG, coordinates, rest, loaded nodes) and
+                           the loads as printed;
+  p121_demo_displ.npz   -- the nodal displacement field of
+                           p121_demo.ensi.DISPL-000001 (x, y, z per node, as
+                           printed with 4-5 significant digits).
+The xx3_tiny/ directory is a verbatim copy of examples/dev/xx3/demo/xx3-tiny.*
+(the deck and its .res/.dis results).
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from parafem_amd import mesh as M  # noqa: E402
+
+REF = "/root/reference/parafem/examples/5th_ed/p121/demo/p121_demo"
+
+
+def deck_digest(d):
+    h = hashlib.sha256()
+    for k in ("g_num", "coords", "rest", "load_node"):
+        a = np.ascontiguousarray(d[k])
+        if a.dtype.kind == "f":
+            a = a + 0.0          # -0.0 and 0.0 hash alike
+        h.update(k.encode())
+        h.update(str(a.dtype).encode())
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def main():
+    d = M.read_deck(REF)
+    res = open(REF + ".res").read().splitlines()
+    lines = open(REF + ".ensi.DISPL-000001").read().split("\n")[4:]
+    vals = np.array([float(s) for s in lines if s.strip()])
+    disp = vals.reshape(3, -1).T.astype(np.float32)
+    sig = [float(t) for t in res[10].split()]
+    out = {
+        "source": "examples/5th_ed/p121/demo/p121_demo.{dat,d,bnd,lds,res,ensi.DISPL-000001}",
+        "nels": d["nels"], "nn": d["nn"], "nr": d["nr"], "nod": d["nod"],
+        "loaded_nodes": d["loaded_nodes"], "e": d["e"], "v": d["v"], "tol": d["tol"],
+        "limit": d["limit"],
+        "mg": {"nxe": 20, "nye": 20, "nze": 20, "aa": 0.5, "bb": 0.5, "cc": 0.5},
+        "res_processes": int(res[0].split()[4]),
+        "res_neq": int(res[1].split()[7]),
+        "res_total_load": float(res[4].split()[-1]),
+        "res_iters": int(res[5].split()[-1]),
+        "res_x1": float(res[7].split()[-1]),
+        "res_sigma_elem1": sig,
+        "deck_sha256": deck_digest(d),
+        "load_node": d["load_node"].tolist(),
+        "load_val": d["load_val"].tolist(),
+    }
+    with open(os.path.join(HERE, "p121_demo.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    np.savez_compressed(os.path.join(HERE, "p121_demo_displ.npz"), disp=disp)
+    print(json.dumps(out, indent=1))
+
+
+if __name__ == "__main__":
+    main()

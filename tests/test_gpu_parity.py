@@ -1,0 +1,246 @@
+"""GPU parity: the HIP library through its C ABI against the CPU oracle.
+
+Bars (stated per test):
+  * integer maps, gather/scatter, element products, the assembled diagonal
+    and the PCG trajectory on identical element matrices: bit-exact
+    (the oracle restates the device's fixed reduction order for this);
+  * device assembly vs the oracle's literal B^T D B: 1e-13 of max|K_e|;
+  * end-to-end (device assembly + PCG) vs the oracle's sequential
+    reference order at relres 1e-10: 1e-9 relative displacement.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from parafem_amd import DeviceArray, mesh as M, problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _tiny(golden):
+    d = M.read_deck(os.path.join(golden, "xx3_tiny", "xx3-tiny"))
+    nf, neq = M.steering(d["nn"], 3, d["rest"])
+    g_g = M.g_g_from(nf, d["g_num"])
+    gc = M.element_coords(d["g_num"], d["coords"])
+    r = M.nodal_loads_to_rhs(nf, d["load_node"], d["load_val"], neq)
+    return d, nf, neq, g_g, gc, r
+
+
+def _distorted(n=3, seed=1, lognormal=True):
+    pr = P.elastic_cube(n, lognormal_e=lognormal)
+    rng = np.random.default_rng(seed)
+    pr.mesh.coords = pr.mesh.coords + rng.uniform(-0.15, 0.15, pr.mesh.coords.shape)
+    return pr
+
+
+def test_assembly_hex20_matches_oracle(ctx):
+    pr = _distorted(3)
+    say(f"assembly hex20 distorted nels={pr.nels}")
+    P.setup(ctx, pr)
+    Kg = ctx.storkm_get()
+    Ko = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem)
+    scale = np.abs(Ko).max(axis=(1, 2))[:, None, None]
+    assert np.abs(Kg - Ko).max() == 0.0 or (np.abs(Kg - Ko) / scale).max() <= 1e-13
+
+
+def test_assembly_laplace_hex8_matches_oracle(ctx):
+    pr = P.laplace_cube(4)
+    rng = np.random.default_rng(4)
+    pr.mesh.coords = pr.mesh.coords + rng.uniform(-0.15, 0.15, pr.mesh.coords.shape)
+    say(f"assembly hex8 laplace nels={pr.nels}")
+    ctx.mesh_set(pr.g_coord())
+    ctx.steer_set(pr.g_g, pr.neq)
+    ctx.assemble_laplace(1.0, 1.0, 1.0, pr.k_elem)
+    Kg = ctx.storkm_get()
+    Ko = O.assemble_laplace(pr.g_coord(), 1.0, 1.0, 1.0, pr.k_elem)
+    scale = np.abs(Ko).max(axis=(1, 2))[:, None, None]
+    assert (np.abs(Kg - Ko) / scale).max() <= 1e-13
+
+
+def test_diag_and_primitives_bitexact(ctx):
+    pr = _distorted(3, seed=2)
+    say(f"primitives nels={pr.nels} neq={pr.neq}")
+    Ko = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem)
+    ctx.mesh_set(pr.g_coord())
+    ctx.steer_set(pr.g_g, pr.neq)
+    ctx.storkm_set(Ko)
+    assert np.array_equal(ctx.diag_get(), O.diag(pr.g_g, Ko, pr.neq))
+    rng = np.random.default_rng(3)
+    p = rng.normal(size=pr.neq)
+    dp = DeviceArray.from_host(ctx, p)
+    pm = DeviceArray(ctx, pr.nels * pr.ntot)
+    ut = DeviceArray(ctx, pr.nels * pr.ntot)
+    ctx.gather(dp, pm)
+    pm_o = O.gather(pr.g_g, p)
+    assert np.array_equal(pm.download(), pm_o.reshape(-1))
+    ctx.elem_matvec(pm, ut)
+    ut_o = O.elem_matvec(Ko, pm_o)
+    assert np.array_equal(ut.download(), ut_o.reshape(-1))
+    u0 = rng.normal(size=pr.neq)
+    du = DeviceArray.from_host(ctx, u0)
+    ctx.scatter(du, ut)
+    ref = u0 + O.scatter(pr.g_g, ut_o, pr.neq)   # SCATTER: u = u + ul
+    assert np.array_equal(du.download(), ref)
+    ctx.scatter_noadd(ut, du)
+    last = np.zeros(pr.neq)
+    for e in range(pr.nels):            # SCATTER_NOADD: the last element wins
+        g = pr.g_g[e]
+        m = g > 0
+        last[g[m] - 1] = ut_o[e][m]
+    assert np.array_equal(du.download(), last)
+    b = rng.normal(size=pr.neq)
+    db = DeviceArray.from_host(ctx, b)
+    assert ctx.dot_product_p(dp, db) == O.dot_device(p, b)
+    assert ctx.sum_p(dp) == O.dot_device(p)
+    x_old = p + 1e-3 * rng.normal(size=pr.neq)
+    dx = DeviceArray.from_host(ctx, x_old)
+    conv = ctx.checon_par(dp, 1e-2, dx)
+    ref_conv, x_new = O.checon(p, 1e-2, x_old)
+    assert conv == ref_conv
+    assert np.array_equal(dx.download(), x_new)
+
+
+def test_pcg_bitexact_tiny_deck(ctx, golden):
+    """xx3-tiny with the oracle's element matrices: identical iteration count
+    (79) and a bit-identical displacement vector."""
+    d, nf, neq, g_g, gc, r = _tiny(golden)
+    say("pcg bit-exact xx3-tiny")
+    Ko = O.assemble_elastic(gc, d["e"], d["v"])
+    ctx.mesh_set(gc)
+    ctx.steer_set(g_g, neq)
+    ctx.storkm_set(Ko)
+    x, res = ctx.pcg_solve(r, d["limit"], d["tol"], stop="checon")
+    xo, it, conv = O.pcg(g_g, Ko, r, O.diag(g_g, Ko, neq), d["tol"], d["limit"], dot_order=O.DOT_DEVICE)
+    assert res.converged and conv
+    assert res.iters == it == 79
+    assert np.array_equal(x, xo)
+
+
+def test_pcg_tiny_deck_device_assembly(ctx, golden):
+    """Full device flow on xx3-tiny vs the reference's fixtures (79
+    iterations, displacements to print precision) and vs the oracle's
+    sequential order."""
+    d, nf, neq, g_g, gc, r = _tiny(golden)
+    say("pcg device assembly xx3-tiny")
+    ctx.mesh_set(gc)
+    ctx.steer_set(g_g, neq)
+    ctx.assemble_elastic(d["e"], d["v"])
+    x, res = ctx.pcg_solve(r, d["limit"], d["tol"])
+    assert res.converged and res.iters == 79
+    Ko = O.assemble_elastic(gc, d["e"], d["v"])
+    xo, it, conv = O.pcg(g_g, Ko, r, O.diag(g_g, Ko, neq), d["tol"], d["limit"])
+    assert np.abs(x - xo).max() <= 1e-9 * np.abs(xo).max()
+
+
+def test_pcg_demo_deck(ctx, golden):
+    """p121 demo (8000 hex20, 98360 equations): same iteration count as the
+    oracle in the device order, bit-identical x; x(1) = -0.8571."""
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_oracle import _demo_problem
+    meta, m, nf, neq, g_g, r = _demo_problem(golden)
+    say(f"pcg demo deck neq={neq}")
+    Ko = O.assemble_elastic(m.g_coord(), meta["e"], meta["v"])
+    ctx.mesh_set(m.g_coord())
+    ctx.steer_set(g_g, neq)
+    ctx.storkm_set(Ko)
+    x, res = ctx.pcg_solve(r, meta["limit"], meta["tol"])
+    xo, it, conv = O.pcg(g_g, Ko, r, O.diag(g_g, Ko, neq), meta["tol"], meta["limit"],
+                         dot_order=O.DOT_DEVICE, nthreads=8)
+    assert res.iters == it and abs(it - meta["res_iters"]) <= 3
+    assert np.array_equal(x, xo)
+    assert round(x[0], 4) == meta["res_x1"]
+
+
+def test_patch_test_exact_on_device(ctx):
+    pr = P.uniaxial_cube(4)
+    say(f"uniaxial patch test neq={pr.neq}")
+    x, res = P.solve(ctx, pr, 1e-13, 5000, stop="relres")
+    ex = pr.info["exact"]
+    assert res.converged
+    assert np.abs(x - ex).max() <= 1e-10 * np.abs(ex).max()
+
+
+def test_laplace_penalty_flow_on_device(ctx):
+    pr = P.laplace_cube(5, lognormal_k=False)
+    say(f"laplace penalty neq={pr.neq}")
+    x, res = P.solve(ctx, pr, 1e-13, 2000, stop="relres")
+    T = P.nodal_field(pr, x)[:, 0]
+    assert res.converged
+    assert np.abs(T - pr.mesh.coords[:, 0]).max() < 1e-11
+
+
+def test_laplace_lognormal_bitexact(ctx):
+    pr = P.laplace_cube(6, lognormal_k=True)
+    say(f"laplace lognormal bit-exact neq={pr.neq}")
+    Ko = O.assemble_laplace(pr.g_coord(), 1.0, 1.0, 1.0, pr.k_elem)
+    ctx.mesh_set(pr.g_coord())
+    ctx.steer_set(pr.g_g, pr.neq)
+    ctx.storkm_set(Ko)
+    r = P.start_r(ctx, pr)
+    x, res = ctx.pcg_solve(r, 3000, 1e-10, stop="relres")
+    dg = O.diag(pr.g_g, Ko, pr.neq)
+    store = dg[pr.fix_eq - 1] + P.PENALTY
+    dg[pr.fix_eq - 1] = store
+    xo, it, conv = O.pcg(pr.g_g, Ko, r, dg, 1e-10, 3000, stop=O.STOP_RELRES, no_f=pr.fix_eq,
+                         store=store, dot_order=O.DOT_DEVICE)
+    assert res.iters == it and res.converged == conv
+    assert np.array_equal(x, xo)
+
+
+def test_config1_elastic_relres(ctx):
+    """The 20^3 hex20 workload (base fixed, top pressure) at relres 1e-10:
+    device FP64 vs the oracle's sequential (reference) order."""
+    pr = P.elastic_cube(20)
+    say(f"config 20^3 relres neq={pr.neq}")
+    assert pr.neq == 103320
+    x, res = P.solve(ctx, pr, 1e-10, 5000, stop="relres")
+    assert res.converged
+    Ko = O.assemble_elastic(pr.g_coord(), pr.e, pr.v)
+    xo, it, conv = O.pcg(pr.g_g, Ko, pr.r0, O.diag(pr.g_g, Ko, pr.neq), 1e-10, 5000,
+                         stop=O.STOP_RELRES, nthreads=8)
+    assert conv and abs(res.iters - it) <= 2
+    assert np.abs(x - xo).max() <= 1e-9 * np.abs(xo).max()
+    y = ctx.apply(x)
+    assert np.linalg.norm(y - pr.r0) <= 2e-10 * np.linalg.norm(pr.r0)
+
+
+def test_run_to_run_bitwise(ctx):
+    pr = _distorted(4, seed=5)
+    say("determinism")
+    r = P.setup(ctx, pr)
+    x1, r1 = ctx.pcg_solve(r, 500, 1e-8, stop="relres")
+    x2, r2 = ctx.pcg_solve(r, 500, 1e-8, stop="relres")
+    assert r1.iters == r2.iters and np.array_equal(x1, x2)
+
+
+def test_limit_and_zero_rhs(ctx):
+    pr = P.elastic_cube(3)
+    say("limit / zero rhs")
+    r = P.setup(ctx, pr)
+    x, res = ctx.pcg_solve(r, 5, 1e-12)
+    assert res.iters == 5 and not res.converged and res.status == 5
+    x, res = ctx.pcg_solve(np.zeros(pr.neq), 100, 1e-6)
+    assert res.iters == 0 and res.converged and not x.any()
+
+
+def test_split_run_equals_solve(ctx):
+    """pcg_begin/run/poll with overshooting run() calls freezes at the same
+    iterate as the reference loop would stop."""
+    pr = P.elastic_cube(4)
+    say("split run")
+    r = P.setup(ctx, pr)
+    x, res = ctx.pcg_solve(r, 1000, 1e-6)
+    ctx.pcg_begin(r, 1000, 1e-6)
+    ctx.pcg_run(res.iters + 37)
+    res2 = ctx.pcg_poll()
+    assert res2.iters == res.iters and res2.converged
+    assert np.array_equal(ctx.pcg_x(), x)
