@@ -24,6 +24,9 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions stay off (v_pk_*_f32 are never emitted)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP64 vector arithmetic must round every operation as written (the PCG
+    # trajectory is compared bit for bit); fused multiply-adds are explicit.
+    "-ffp-contract=off",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 

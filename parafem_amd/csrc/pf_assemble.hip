@@ -12,12 +12,12 @@
 // :769-794, hex20 :865-896, BEEMAT :976-993, DEEMAT :1671-1686) and
 // src/modules/mpi/maths.f90 (INVERT :525-547, DETERMINANT :615-619).
 //
-// Elastic element: with D from DEEMAT (lambda = dee(1,2), mu = dee(4,4)) the
-// 3x3 block of B^T D B coupling node a, component i to node b, component j
-// is  lambda*da_i*db_j + mu*da_j*db_i + mu*delta_ij*(da . db)  (d = global
-// shape-function derivatives), which is the product BEEMAT/DEEMAT/MATMUL
-// form without the zero entries of B.  Entries are summed over the Gauss
-// points in the reference's order (point 1..8).
+// Every quantity is formed with the reference's operation order (sums in
+// ascending index order, each product and sum rounded, no fused
+// multiply-add), skipping only products with the structural zeros of B and
+// dee, which add exact zeros.  The element matrices therefore equal the ones
+// a sequential evaluation of p121's MATMUL(MATMUL(TRANSPOSE(bee),dee),bee)
+// *det*weights(i) gives, bit for bit.
 #include "pf_internal.h"
 
 namespace {
@@ -144,10 +144,19 @@ __device__ __forceinline__ void element_geometry(ElemLds<NOD> &L, const double *
     __syncthreads();
 }
 
+// Elastic element.  Lane i is row i of K_e (node a = i/3, component ci).
+// The entry (i,j) at a Gauss point is sum_k btd(i,k) B(k,j) with
+// btd(i,k) = sum_l B(l,i) dee(l,k), evaluated in the reference's order
+// (l then k ascending, every product rounded, no fused multiply-add); the
+// zero entries of B and dee contribute exact zeros, so only the non-zero
+// products are formed:
+//   x dof: B(1,i)=dx B(4,i)=dy B(6,i)=dz   y dof: B(2,i)=dy B(4,i)=dx B(5,i)=dz
+//   z dof: B(3,i)=dz B(5,i)=dy B(6,i)=dx   (BEEMAT ih=6)
+// and km(i,j) += (s*det)*w with SAMPLE's w = 1.
 template <int NOD>
 __global__ __launch_bounds__(64 * WAVES) void k_assemble_elastic(
-    const double *__restrict__ g_coord, const double *__restrict__ e_elem, double lam0, double mu0,
-    double e0, double *__restrict__ K, int nels)
+    const double *__restrict__ g_coord, const double *__restrict__ e_elem, double v2, double vv,
+    double den, double e0, double *__restrict__ K, int nels)
 {
     constexpr int NTOT = 3 * NOD;
     __shared__ ElemLds<NOD> lds[WAVES];
@@ -157,37 +166,40 @@ __global__ __launch_bounds__(64 * WAVES) void k_assemble_elastic(
     ElemLds<NOD> &L = lds[wave];
     element_geometry<NOD>(L, g_coord + (valid ? e : 0) * (NOD * 3), lane, valid);
     if (!valid || lane >= NTOT) return;
-    const double E = e_elem ? e_elem[e] : e0;
-    const double lam = lam0 * E, mu = mu0 * E;
+    // DEEMAT: dee = dee0 * (e / (2(1+v)vv)) with dee0 = 1 on the normal
+    // diagonal, v2 off it, vv on the shear diagonal.
+    const double f = (e_elem ? e_elem[e] : e0) / den;
+    const double D11 = 1.0 * f, D12 = v2 * f, D44 = vv * f;
     const int a = lane / 3, ci = lane - 3 * (lane / 3);
-    double A[8], B0[8], B1[8], B2[8];
+    double bt[8][6];
 #pragma unroll
     for (int g = 0; g < 8; ++g) {
-        const double w = L.dw[g];
-        const double da0 = L.deriv[g][0][a], da1 = L.deriv[g][1][a], da2 = L.deriv[g][2][a];
-        const double dci = ci == 0 ? da0 : (ci == 1 ? da1 : da2);
-        A[g] = lam * w * dci;
-        B0[g] = mu * w * da0;
-        B1[g] = mu * w * da1;
-        B2[g] = mu * w * da2;
+        const double dx = L.deriv[g][0][a], dy = L.deriv[g][1][a], dz = L.deriv[g][2][a];
+        if (ci == 0) {
+            bt[g][0] = dx * D11; bt[g][1] = dx * D12; bt[g][2] = dx * D12;
+            bt[g][3] = dy * D44; bt[g][4] = 0.0;      bt[g][5] = dz * D44;
+        } else if (ci == 1) {
+            bt[g][0] = dy * D12; bt[g][1] = dy * D11; bt[g][2] = dy * D12;
+            bt[g][3] = dx * D44; bt[g][4] = dz * D44; bt[g][5] = 0.0;
+        } else {
+            bt[g][0] = dz * D12; bt[g][1] = dz * D12; bt[g][2] = dz * D11;
+            bt[g][3] = 0.0;      bt[g][4] = dy * D44; bt[g][5] = dx * D44;
+        }
     }
     double *Ke = K + e * (long)(NTOT * NTOT) + lane;
     for (int b = 0; b < NOD; ++b) {
         double o0 = 0.0, o1 = 0.0, o2 = 0.0;
 #pragma unroll
         for (int g = 0; g < 8; ++g) {
-            const double db0 = L.deriv[g][0][b], db1 = L.deriv[g][1][b], db2 = L.deriv[g][2][b];
-            const double dbci = ci == 0 ? db0 : (ci == 1 ? db1 : db2);
-            const double S = B0[g] * db0 + B1[g] * db1 + B2[g] * db2;
-            double t0 = db0 * A[g] + dbci * B0[g];
-            double t1 = db1 * A[g] + dbci * B1[g];
-            double t2 = db2 * A[g] + dbci * B2[g];
-            if (ci == 0) t0 += S;
-            else if (ci == 1) t1 += S;
-            else t2 += S;
-            o0 += t0;
-            o1 += t1;
-            o2 += t2;
+            const double dx = L.deriv[g][0][b], dy = L.deriv[g][1][b], dz = L.deriv[g][2][b];
+            const double det = L.dw[g];
+            const double *t = bt[g];
+            const double sx = (t[0] * dx + t[3] * dy) + t[5] * dz;   // column x dof of node b
+            const double sy = (t[1] * dy + t[3] * dx) + t[4] * dz;   // column y dof
+            const double sz = (t[2] * dz + t[4] * dy) + t[5] * dx;   // column z dof
+            o0 = o0 + sx * det;
+            o1 = o1 + sy * det;
+            o2 = o2 + sz * det;
         }
         Ke[(3 * b + 0) * NTOT] = o0;
         Ke[(3 * b + 1) * NTOT] = o1;
@@ -250,15 +262,15 @@ __global__ void k_diag(const int *__restrict__ ptr, const int *__restrict__ slot
 
 }  // namespace
 
-int pf_launch_assemble_elastic(pf_ctx *c, double lam, double mu, double e0, const double *e_elem)
+int pf_launch_assemble_elastic(pf_ctx *c, double v2, double vv, double den, double e0, const double *e_elem)
 {
     const int grid = (c->nels + WAVES - 1) / WAVES;
     if (c->nod == 20)
         hipLaunchKernelGGL(k_assemble_elastic<20>, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
-                           e_elem, lam, mu, e0, c->K64, c->nels);
+                           e_elem, v2, vv, den, e0, c->K64, c->nels);
     else if (c->nod == 8)
         hipLaunchKernelGGL(k_assemble_elastic<8>, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
-                           e_elem, lam, mu, e0, c->K64, c->nels);
+                           e_elem, v2, vv, den, e0, c->K64, c->nels);
     else
         return PF_ERR_ARG;
     PF_HIP(hipGetLastError());

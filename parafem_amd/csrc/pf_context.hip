@@ -372,12 +372,12 @@ int pf_assemble_elastic(pf_ctx *c, double e, double v, const double *e_elem, int
         if (rc) return rc;
         PF_HIP(hipMemcpyAsync(ee, e_elem, sizeof(double) * c->nels, hipMemcpyHostToDevice, c->stream));
     }
-    // DEEMAT (new_library.f90:1671-1686): dee(1,2) = lambda, dee(4,4) = mu,
-    // formed with the same expressions so a uniform e reproduces dee exactly.
-    double v2 = v / (1.0 - v);
-    double vv = (1.0 - 2.0 * v) / (1.0 - v) * 0.5;
-    double f = 1.0 / (2.0 * (1.0 + v) * vv);   // dee = dee0 * e * f
-    rc = pf_launch_assemble_elastic(c, v2 * f, vv * f, e, ee);
+    // DEEMAT (new_library.f90:1671-1686), evaluated with the same
+    // expressions: dee = dee0 * e / (2(1+v)vv), dee0 = 1 | v2 | vv.
+    const double v2 = v / (1.0 - v);
+    const double vv = (1.0 - 2.0 * v) / (1.0 - v) * 0.5;
+    const double den = 2.0 * (1.0 + v) * vv;
+    rc = pf_launch_assemble_elastic(c, v2, vv, den, e, ee);
     if (ee) {
         (void)hipStreamSynchronize(c->stream);
         pf_free(c, ee, sizeof(double) * c->nels);

@@ -99,7 +99,7 @@ void pf_free(pf_ctx *c, void *p, size_t bytes);
 int pf_grid_for(int64_t n);
 
 // kernels launched across translation units ---------------------------------
-int pf_launch_assemble_elastic(pf_ctx *c, double lam, double mu, double e0, const double *e_elem_dev);
+int pf_launch_assemble_elastic(pf_ctx *c, double v2, double vv, double den, double e0, const double *e_elem_dev);
 int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem_dev);
 int pf_launch_split_fp32(pf_ctx *c);
 int pf_launch_diag(pf_ctx *c, double *diag_dev);

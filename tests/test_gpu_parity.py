@@ -1,12 +1,12 @@
 """GPU parity: the HIP library through its C ABI against the CPU oracle.
 
 Bars (stated per test):
-  * integer maps, gather/scatter, element products, the assembled diagonal
-    and the PCG trajectory on identical element matrices: bit-exact
-    (the oracle restates the device's fixed reduction order for this);
-  * device assembly vs the oracle's literal B^T D B: 1e-13 of max|K_e|;
-  * end-to-end (device assembly + PCG) vs the oracle's sequential
-    reference order at relres 1e-10: 1e-9 relative displacement.
+  * element matrices, integer maps, gather/scatter, element products, the
+    assembled diagonal and the whole PCG trajectory: bit-exact against the
+    oracle evaluated in the device's fixed reduction order;
+  * against the oracle's sequential (reference) dot order, whose rounding
+    differs: the same iteration count, 1e-6 relative displacement when both
+    stop by checon at tol 1e-5, 1e-9 when both run to relres 1e-10.
 """
 import json
 import os
@@ -48,8 +48,7 @@ def test_assembly_hex20_matches_oracle(ctx):
     P.setup(ctx, pr)
     Kg = ctx.storkm_get()
     Ko = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem)
-    scale = np.abs(Ko).max(axis=(1, 2))[:, None, None]
-    assert np.abs(Kg - Ko).max() == 0.0 or (np.abs(Kg - Ko) / scale).max() <= 1e-13
+    assert np.array_equal(Kg, Ko)
 
 
 def test_assembly_laplace_hex8_matches_oracle(ctx):
@@ -62,8 +61,7 @@ def test_assembly_laplace_hex8_matches_oracle(ctx):
     ctx.assemble_laplace(1.0, 1.0, 1.0, pr.k_elem)
     Kg = ctx.storkm_get()
     Ko = O.assemble_laplace(pr.g_coord(), 1.0, 1.0, 1.0, pr.k_elem)
-    scale = np.abs(Ko).max(axis=(1, 2))[:, None, None]
-    assert (np.abs(Kg - Ko) / scale).max() <= 1e-13
+    assert np.array_equal(Kg, Ko)
 
 
 def test_diag_and_primitives_bitexact(ctx):
@@ -126,9 +124,9 @@ def test_pcg_bitexact_tiny_deck(ctx, golden):
 
 
 def test_pcg_tiny_deck_device_assembly(ctx, golden):
-    """Full device flow on xx3-tiny vs the reference's fixtures (79
-    iterations, displacements to print precision) and vs the oracle's
-    sequential order."""
+    """Full device flow on xx3-tiny: bit-identical to the oracle in the
+    device's summation order; 79 iterations and displacements within the
+    checon tolerance of the oracle's sequential (reference) order."""
     d, nf, neq, g_g, gc, r = _tiny(golden)
     say("pcg device assembly xx3-tiny")
     ctx.mesh_set(gc)
@@ -137,8 +135,13 @@ def test_pcg_tiny_deck_device_assembly(ctx, golden):
     x, res = ctx.pcg_solve(r, d["limit"], d["tol"])
     assert res.converged and res.iters == 79
     Ko = O.assemble_elastic(gc, d["e"], d["v"])
-    xo, it, conv = O.pcg(g_g, Ko, r, O.diag(g_g, Ko, neq), d["tol"], d["limit"])
-    assert np.abs(x - xo).max() <= 1e-9 * np.abs(xo).max()
+    dg = O.diag(g_g, Ko, neq)
+    xd, itd, _ = O.pcg(g_g, Ko, r, dg, d["tol"], d["limit"], dot_order=O.DOT_DEVICE)
+    assert itd == res.iters and np.array_equal(x, xd)
+    xo, it, conv = O.pcg(g_g, Ko, r, dg, d["tol"], d["limit"])
+    assert it == res.iters
+    # two FP64 summation orders stopped by checon at tol = 1e-5
+    assert np.abs(x - xo).max() <= 1e-6 * np.abs(xo).max()
 
 
 def test_pcg_demo_deck(ctx, golden):
