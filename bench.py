@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="elements per cube edge")
     ap.add_argument("--store", default="fp64", choices=["fp64"])
     ap.add_argument("--cpu-sample", type=int, default=24, help="edge of the CPU-baseline cube (0 = skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the streaming copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -88,7 +90,17 @@ def main():
                           "sharding is a design only (see DESIGN.md); use --gpus 1"}))
         sys.exit(2)
 
-    from parafem_amd import Context, problems as P
+    from parafem_amd import Context, DeviceArray, problems as P
+
+    if args.copy_calibration:
+        ctx = Context(0)
+        n = args.copy_calibration * 2**20 // 8
+        a, b = DeviceArray(ctx, n), DeviceArray(ctx, n)
+        for _ in range(5):
+            ctx.dev_copy(b, a)
+        ctx.sync()
+        print(json.dumps({"copy_bytes": n * 8, "launches": 5}), flush=True)
+        return
 
     t0 = time.perf_counter()
     pr = P.elastic_cube(args.n, lognormal_e=True)

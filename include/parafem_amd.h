@@ -52,8 +52,11 @@ extern "C" {
 
 /* PCG arithmetic. */
 #define PF_PREC_FP64       0  /* FP64 everywhere (the reference's arithmetic) */
-#define PF_PREC_MIXED      1  /* FP32 element products, FP64 vectors, dots,
-                                 recurrences and true-residual refinement     */
+#define PF_PREC_MIXED      1  /* element products from the FP32 hi matrices in
+                                 FP32; vectors, dots and recurrences in FP64;
+                                 reliable updates replace r by the true
+                                 residual f - (hi+lo) x computed in FP64.
+                                 Needs PF_STORE_FP32 and PF_STOP_RELRES.       */
 
 typedef struct pf_ctx pf_ctx;
 
@@ -63,7 +66,11 @@ typedef struct {
     int    stop_mode;    /* PF_STOP_*                                          */
     int    precision;    /* PF_PREC_*                                          */
     int    check_every;  /* host polls convergence every n iterations (0=16)  */
-    int    refine_every; /* mixed: true-residual replacement period (0=auto)  */
+    int    refine_every; /* mixed: also replace r at least every n iterations
+                            (0 = only on the residual-drop trigger)           */
+    double refine_delta; /* mixed: replace r when ||r|| falls below delta
+                            times its value at the last replacement
+                            (0 = 1e-2)                                         */
 } pf_pcg_opts;
 
 typedef struct {
@@ -100,6 +107,10 @@ int pf_memcpy_h2d(pf_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes
 int pf_memcpy_d2h(pf_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
 /* Bytes of device memory the context holds. */
 int pf_mem_used(pf_ctx *ctx, size_t *bytes);
+/* Device-to-device copy by a plain streaming kernel (k_copy): moves
+ * `bytes` (a multiple of 16) once through HBM in each direction.  Also the
+ * calibration kernel for the HBM byte counters.  No reference counterpart. */
+int pf_dev_copy(pf_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
 
 /* ------------------------------------------------------------- set-up */
 /* Element coordinates g_coord_pp(nod,3,nels) (host), as read by
@@ -185,13 +196,14 @@ int pf_pcg_timing_enable(pf_ctx *ctx, int on);
 int pf_pcg_timing_get(pf_ctx *ctx, double ms[4], int counts[4]);
 
 /* ------------------------------------------------- layout introspection */
-/* Grid size G (blocks of 256 threads) the FP64 reductions use for length n.
- * It fixes the summation order of every dot product: thread g = 256*b + t
- * adds a_i*b_i for i = g, g + 256*G, ... in turn (s = s + a*b, no fused
- * multiply-add); each 64-lane wavefront folds its lanes with offsets
- * 32,16,8,4,2,1 (v[l] = v[l] + v[l+off]); a block adds its 4 wavefront sums
- * left to right; the last block to finish folds the G block sums the same
- * way (thread t sums entries t, t+256, ..., then wavefronts, then blocks). */
+/* Grid size G = clamp(ceil(n/256), 1, 16384) (blocks of 256 threads) the
+ * FP64 reductions use for length n.  It fixes the summation order of every
+ * dot product: thread g = 256*b + t adds a_i*b_i for i = g, g + 256*G, ...
+ * in turn (s = s + a*b, no fused multiply-add); each 64-lane wavefront
+ * folds its lanes with offsets 32,16,8,4,2,1 (v[l] = v[l] + v[l+off]); a
+ * block adds its 4 wavefront sums left to right; a one-block kernel folds
+ * the G block sums the same way (thread t sums entries t, t+256, ..., then
+ * wavefronts, then the 4 wavefront sums left to right). */
 int pf_reduce_grid(int64_t n);
 
 #ifdef __cplusplus

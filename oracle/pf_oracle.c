@@ -394,7 +394,7 @@ enum { OR_DOT_SEQ = 0, OR_DOT_THREADS = 1, OR_DOT_DEVICE = 2 };
 /* Summation order of the device library's FP64 reductions, restated on the
  * CPU so that a device solve can be compared bit for bit (it is not the
  * reference's order; the reference sums sequentially at one rank and by
- * contiguous per-rank chunks at several).  G = clamp(ceil(n/256), 1, 2048)
+ * contiguous per-rank chunks at several).  G = clamp(ceil(n/256), 1, 16384)
  * blocks of 256 threads; thread g sums terms g, g+256G, ...; each 64-lane
  * group folds with offsets 32..1; a block adds its 4 group sums left to
  * right; the G block sums are folded the same way by one block. */
@@ -402,7 +402,7 @@ static int dev_grid(long n)
 {
     long g = (n + 255) / 256;
     if (g < 1) g = 1;
-    if (g > 2048) g = 2048;
+    if (g > 16384) g = 16384;
     return (int)g;
 }
 

@@ -53,7 +53,8 @@ class PFError(RuntimeError):
 
 class PcgOpts(C.Structure):
     _fields_ = [("limit", C.c_int), ("tol", C.c_double), ("stop_mode", C.c_int),
-                ("precision", C.c_int), ("check_every", C.c_int), ("refine_every", C.c_int)]
+                ("precision", C.c_int), ("check_every", C.c_int), ("refine_every", C.c_int),
+                ("refine_delta", C.c_double)]
 
 
 class PcgResult(C.Structure):
@@ -112,6 +113,7 @@ def lib():
         "pf_reduce_grid": (i32, [i64]),
         "pf_pcg_timing_enable": (i32, [vp, i32]),
         "pf_pcg_timing_get": (i32, [vp, pd, pi]),
+        "pf_dev_copy": (i32, [vp, vp, vp, sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -245,6 +247,12 @@ class Context:
     def sync(self):
         _check(lib().pf_sync(self.h), "pf_sync")
 
+    def dev_copy(self, dst: "DeviceArray", src: "DeviceArray"):
+        """dst = src on the device with the streaming copy kernel."""
+        if dst.n != src.n:
+            raise ValueError("size mismatch")
+        _check(lib().pf_dev_copy(self.h, dst.ptr, src.ptr, src.n * 8), "pf_dev_copy")
+
     def mem_used(self) -> int:
         n = C.c_size_t()
         _check(lib().pf_mem_used(self.h, C.byref(n)), "pf_mem_used")
@@ -347,17 +355,18 @@ class Context:
 
     # ----------------------------------------------------------- the solve
     @staticmethod
-    def _opts(limit, tol, stop, precision, check_every=0, refine_every=0):
+    def _opts(limit, tol, stop, precision, check_every=0, refine_every=0, refine_delta=0.0):
         return PcgOpts(int(limit), float(tol), STOP[stop], PREC[precision], int(check_every),
-                       int(refine_every))
+                       int(refine_every), float(refine_delta))
 
     def pcg_solve(self, r0, limit: int, tol: float, stop: str = "checon", precision: str = "fp64",
-                  check_every: int = 0, refine_every: int = 0):
-        """The p121/p123 PCG loop; returns (xnew, SolveResult)."""
+                  check_every: int = 0, refine_every: int = 0, refine_delta: float = 0.0):
+        """The p121/p123 PCG loop; returns (xnew, SolveResult).  precision
+        "mixed" needs FP32 storage and stop "relres"."""
         r = _f64(r0)
         assert r.size == self.neq
         x = np.zeros(self.neq)
-        o = self._opts(limit, tol, stop, precision, check_every, refine_every)
+        o = self._opts(limit, tol, stop, precision, check_every, refine_every, refine_delta)
         res = PcgResult()
         rc = lib().pf_pcg_solve(self.h, _pd(r), _pd(x), C.byref(o), C.byref(res))
         _check(rc, "pf_pcg_solve", ok=(PF_OK, PF_NOT_CONVERGED, PF_ERR_BREAKDOWN))
@@ -369,8 +378,8 @@ class Context:
                            res.refinements, res.status)
 
     def pcg_begin(self, r0, limit: int, tol: float, stop: str = "checon", precision: str = "fp64",
-                  refine_every: int = 0, on_device: bool = False):
-        o = self._opts(limit, tol, stop, precision, 0, refine_every)
+                  refine_every: int = 0, refine_delta: float = 0.0, on_device: bool = False):
+        o = self._opts(limit, tol, stop, precision, 0, refine_every, refine_delta)
         if on_device:
             _check(lib().pf_pcg_begin(self.h, C.c_void_p(int(r0)), 1, C.byref(o)), "pf_pcg_begin")
         else:

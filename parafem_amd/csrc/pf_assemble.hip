@@ -3,8 +3,8 @@
 // One 64-lane wavefront builds one element: SAMPLE's 2x2x2 Gauss points,
 // SHAPE_DER at each point, the jacobian, its determinant and inverse, global
 // derivatives, then the element matrix, written once to HBM in the
-// reference's storkm_pp(ntot,ntot,nels) column-major layout.  Nothing but the
-// element coordinates is read and nothing but the matrix is written.
+// library's FP64 column-pair layout (pf_internal.h).  Nothing but the element
+// coordinates is read and nothing but the matrix is written.
 //
 // Reference: src/programs/5th_ed/p121/p121.f90:54-64 (elastic),
 // src/programs/5th_ed/p123/p123.f90:70-84 (Laplace); element routines in
@@ -186,24 +186,33 @@ __global__ __launch_bounds__(64 * WAVES) void k_assemble_elastic(
             bt[g][3] = 0.0;      bt[g][4] = dy * D44; bt[g][5] = dx * D44;
         }
     }
-    double *Ke = K + e * (long)(NTOT * NTOT) + lane;
-    for (int b = 0; b < NOD; ++b) {
-        double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+    // Two nodes (6 columns = 3 column pairs) per step so that every store
+    // is one 16-byte column pair of the FP64 pair layout.
+    double2 *Kp = reinterpret_cast<double2 *>(K + e * (long)(NTOT * NTOT)) + lane;
+    for (int bb = 0; bb < NOD / 2; ++bb) {
+        double o[6];
 #pragma unroll
-        for (int g = 0; g < 8; ++g) {
-            const double dx = L.deriv[g][0][b], dy = L.deriv[g][1][b], dz = L.deriv[g][2][b];
-            const double det = L.dw[g];
-            const double *t = bt[g];
-            const double sx = (t[0] * dx + t[3] * dy) + t[5] * dz;   // column x dof of node b
-            const double sy = (t[1] * dy + t[3] * dx) + t[4] * dz;   // column y dof
-            const double sz = (t[2] * dz + t[4] * dy) + t[5] * dx;   // column z dof
-            o0 = o0 + sx * det;
-            o1 = o1 + sy * det;
-            o2 = o2 + sz * det;
+        for (int h = 0; h < 2; ++h) {
+            const int b = 2 * bb + h;
+            double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) {
+                const double dx = L.deriv[g][0][b], dy = L.deriv[g][1][b], dz = L.deriv[g][2][b];
+                const double det = L.dw[g];
+                const double *t = bt[g];
+                const double sx = (t[0] * dx + t[3] * dy) + t[5] * dz;   // column x dof of node b
+                const double sy = (t[1] * dy + t[3] * dx) + t[4] * dz;   // column y dof
+                const double sz = (t[2] * dz + t[4] * dy) + t[5] * dx;   // column z dof
+                o0 = o0 + sx * det;
+                o1 = o1 + sy * det;
+                o2 = o2 + sz * det;
+            }
+            o[3 * h + 0] = o0;
+            o[3 * h + 1] = o1;
+            o[3 * h + 2] = o2;
         }
-        Ke[(3 * b + 0) * NTOT] = o0;
-        Ke[(3 * b + 1) * NTOT] = o1;
-        Ke[(3 * b + 2) * NTOT] = o2;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) Kp[(3 * bb + q) * NTOT] = make_double2(o[2 * q], o[2 * q + 1]);
     }
 }
 
@@ -219,7 +228,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_assemble_laplace8(
     ElemLds<8> &L = lds[wave];
     element_geometry<8>(L, g_coord + (valid ? e : 0) * 24, lane, valid);
     if (!valid) return;
-    const int a = lane & 7, b = lane >> 3;   // entry (a,b) at column-major index a + 8b = lane
+    const int a = lane & 7, b = lane >> 3;   // row a, column b
     double cx = 0.0, cy = 0.0, cz = 0.0;
 #pragma unroll
     for (int g = 0; g < 8; ++g) {
@@ -229,17 +238,51 @@ __global__ __launch_bounds__(64 * WAVES) void k_assemble_laplace8(
         cz += L.deriv[g][2][a] * L.deriv[g][2][b] * w;
     }
     const double s = k_elem ? k_elem[e] : 1.0;
-    K[e * 64 + lane] = cx * (kx * s) + cy * (ky * s) + cz * (kz * s);
+    K[pf_idx_pair(e, 8, a, b)] = cx * (kx * s) + cy * (ky * s) + cz * (kz * s);
 }
 
-// hi = fl32(K), lo = fl32(K - hi): hi + lo carries ~48 significant bits.
-__global__ void k_split_fp32(const double *__restrict__ K, float *__restrict__ hi, float *__restrict__ lo, long n)
+// hi = fl32(K), lo = fl32(K - hi) in the column-quad layout: hi + lo carries
+// ~48 significant bits and is exact in FP64 arithmetic.
+__global__ void k_split_fp32(const double *__restrict__ K, float *__restrict__ hi, float *__restrict__ lo,
+                             int nt, long n)
 {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double k = K[i];
+    const long nn = (long)nt * nt;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const long e = q / nn, w = q - e * nn;
+        const int jq = (int)(w / (4 * nt)), rem = (int)(w - (long)jq * 4 * nt);
+        const int i = rem >> 2, j = 4 * jq + (rem & 3);
+        const double k = K[pf_idx_pair(e, nt, i, j)];
         const float h = (float)k;
-        hi[i] = h;
-        lo[i] = (float)(k - (double)h);
+        hi[q] = h;
+        lo[q] = (float)(k - (double)h);
+    }
+}
+
+// storkm_pp chunk (column-major, elements e0..e0+nel-1) -> FP64 pair layout.
+__global__ void k_from_colmajor(const double *__restrict__ src, double *__restrict__ K, int nt, long e0, long nel)
+{
+    const long nn = (long)nt * nt, n = nel * nn;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const long el = q / nn, w = q - el * nn;
+        const int j = (int)(w / nt), i = (int)(w - (long)j * nt);
+        K[pf_idx_pair(e0 + el, nt, i, j)] = src[q];
+    }
+}
+
+// internal layout -> storkm_pp chunk (FP64 pairs, or hi + lo quads).
+__global__ void k_to_colmajor(const double *__restrict__ K, const float *__restrict__ hi,
+                              const float *__restrict__ lo, double *__restrict__ dst, int nt, long e0, long nel)
+{
+    const long nn = (long)nt * nt, n = nel * nn;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const long el = q / nn, w = q - el * nn;
+        const int j = (int)(w / nt), i = (int)(w - (long)j * nt);
+        if (K) {
+            dst[q] = K[pf_idx_pair(e0 + el, nt, i, j)];
+        } else {
+            const long t = pf_idx_quad(e0 + el, nt, i, j);
+            dst[q] = (double)hi[t] + (double)lo[t];
+        }
     }
 }
 
@@ -252,8 +295,9 @@ __global__ void k_diag(const int *__restrict__ ptr, const int *__restrict__ slot
         double s = 0.0;
         for (int k = ptr[q]; k < ptr[q + 1]; ++k) {
             const long sl = slot[k];
-            const long e = sl / ntot, i = sl - e * ntot;
-            s = __dadd_rn(s, K[e * ntot * ntot + i * ntot + i]);
+            const long e = sl / ntot;
+            const int i = (int)(sl - e * ntot);
+            s = __dadd_rn(s, K[pf_idx_pair(e, ntot, i, i)]);
         }
         diag[q + 1] = s;
     }
@@ -290,7 +334,22 @@ int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const
 int pf_launch_split_fp32(pf_ctx *c)
 {
     const long n = (long)c->ntot * c->ntot * c->nels;
-    hipLaunchKernelGGL(k_split_fp32, dim3(4096), dim3(256), 0, c->stream, c->K64, c->K32hi, c->K32lo, n);
+    hipLaunchKernelGGL(k_split_fp32, dim3(8192), dim3(256), 0, c->stream, c->K64, c->K32hi, c->K32lo, c->ntot, n);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+int pf_launch_from_colmajor(pf_ctx *c, const double *src, long e0, long nel)
+{
+    hipLaunchKernelGGL(k_from_colmajor, dim3(8192), dim3(256), 0, c->stream, src, c->K64, c->ntot, e0, nel);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+int pf_launch_to_colmajor(pf_ctx *c, double *dst, long e0, long nel)
+{
+    hipLaunchKernelGGL(k_to_colmajor, dim3(8192), dim3(256), 0, c->stream, c->K64, c->K32hi, c->K32lo, dst,
+                       c->ntot, e0, nel);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }

@@ -78,9 +78,48 @@ int pf_grid_for(int64_t n)
     return (int)g;
 }
 
+static const size_t kPartials = (size_t)PF_NQ * PF_REDUCE_MAX_GRID + 8;
+
+static void free_problem_vectors(pf_ctx *c)
+{
+    size_t nv = (size_t)c->neq + 1;
+    free_arr(c, c->x, nv); free_arr(c, c->r, nv); free_arr(c, c->p, nv);
+    free_arr(c, c->d, nv); free_arr(c, c->u, nv); free_arr(c, c->dinv, nv);
+    free_arr(c, c->diag, nv); free_arr(c, c->diag0, nv); free_arr(c, c->f, nv);
+    free_arr(c, c->fixstore, nv);
+    c->nfix = 0;
+    c->pcg_ready = false;
+}
+
+static void free_matrices(pf_ctx *c)
+{
+    size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+    free_arr(c, c->K64, nk);
+    free_arr(c, c->K32hi, nk);
+    free_arr(c, c->K32lo, nk);
+    c->have_K = false;
+}
+
+// Drop every buffer of the current problem (a context can be reused for a
+// new mesh of another size).
+static void reset_problem(pf_ctx *c)
+{
+    (void)hipStreamSynchronize(c->stream);
+    free_problem_vectors(c);
+    free_matrices(c);
+    free_arr(c, c->g_coord, (size_t)c->nod * 3 * c->nels);
+    free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
+    free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
+    free_arr(c, c->csr_slot, (size_t)c->nnz);
+    free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
+    free_arr(c, c->utemp32, (size_t)c->ntot * c->nels);
+    c->nod = c->nels = c->ntot = c->neq = 0;
+    c->nnz = 0;
+}
+
 extern "C" {
 
-int pf_version(void) { return 100; }
+int pf_version(void) { return 200; }
 
 const char *pf_status_string(int s)
 {
@@ -120,7 +159,7 @@ int pf_create(int device, pf_ctx **out)
     }
     c->stream = c->own_stream;
     int rc = pf_alloc(c, (void **)&c->state, sizeof(pf_state));
-    if (!rc) rc = pf_alloc(c, (void **)&c->partials, sizeof(double) * (PF_NQ * PF_REDUCE_MAX_GRID + 8));
+    if (!rc) rc = pf_alloc(c, (void **)&c->partials, sizeof(double) * kPartials);
     if (!rc && hipHostMalloc((void **)&c->state_host, sizeof(pf_state), hipHostMallocDefault) != hipSuccess) rc = PF_ERR_OOM;
     if (!rc && hipHostMalloc((void **)&c->scalar_host, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = PF_ERR_OOM;
     if (!rc) {
@@ -136,38 +175,13 @@ int pf_create(int device, pf_ctx **out)
     return PF_OK;
 }
 
-static void free_problem_vectors(pf_ctx *c)
-{
-    size_t nv = (size_t)c->neq + 1;
-    free_arr(c, c->x, nv); free_arr(c, c->r, nv); free_arr(c, c->p, nv);
-    free_arr(c, c->d, nv); free_arr(c, c->u, nv); free_arr(c, c->dinv, nv);
-    free_arr(c, c->diag, nv); free_arr(c, c->fixstore, nv);
-    c->nfix = 0;
-    c->pcg_ready = false;
-}
-
-static void free_matrices(pf_ctx *c)
-{
-    size_t nk = (size_t)c->ntot * c->ntot * c->nels;
-    free_arr(c, c->K64, nk);
-    free_arr(c, c->K32hi, nk);
-    free_arr(c, c->K32lo, nk);
-    c->have_K = false;
-}
-
 int pf_destroy(pf_ctx *c)
 {
     if (!c) return PF_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    free_problem_vectors(c);
-    free_matrices(c);
-    free_arr(c, c->g_coord, (size_t)c->nod * 3 * c->nels);
-    free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
-    free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
-    free_arr(c, c->csr_slot, (size_t)c->nnz);
-    free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
-    free_arr(c, c->partials, (size_t)PF_NQ * PF_REDUCE_MAX_GRID + 8);
+    reset_problem(c);
+    free_arr(c, c->partials, kPartials);
     pf_free(c, c->state, sizeof(pf_state));
     if (c->state_host) (void)hipHostFree(c->state_host);
     if (c->scalar_host) (void)hipHostFree(c->scalar_host);
@@ -238,22 +252,6 @@ int pf_mem_used(pf_ctx *c, size_t *bytes)
     return PF_OK;
 }
 
-// Drop every buffer of the current problem (a context can be reused for a
-// new mesh of another size).
-static void reset_problem(pf_ctx *c)
-{
-    (void)hipStreamSynchronize(c->stream);
-    free_problem_vectors(c);
-    free_matrices(c);
-    free_arr(c, c->g_coord, (size_t)c->nod * 3 * c->nels);
-    free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
-    free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
-    free_arr(c, c->csr_slot, (size_t)c->nnz);
-    free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
-    c->nod = c->nels = c->ntot = c->neq = 0;
-    c->nnz = 0;
-}
-
 int pf_mesh_set(pf_ctx *c, int nod, int nels, const double *g_coord)
 {
     if (!c || !g_coord || (nod != 8 && nod != 20) || nels <= 0) return PF_ERR_ARG;
@@ -272,8 +270,9 @@ int pf_mesh_set(pf_ctx *c, int nod, int nels, const double *g_coord)
 
 int pf_steer_set(pf_ctx *c, int ntot, int nels, int neq, const int *g_g)
 {
-    if (!c || !g_g || ntot <= 0 || ntot > 64 || nels <= 0 || neq <= 0) return PF_ERR_ARG;
-    if (c->nels && c->nels != nels) return PF_ERR_ARG;   // pf_mesh_set first for a new mesh
+    if (!c || !g_g || nels <= 0 || neq <= 0) return PF_ERR_ARG;
+    if (ntot != 60 && ntot != 24 && ntot != 8) return PF_ERR_ARG;   // hex20/hex8 elastic, hex8 Laplace
+    if (c->nels && c->nels != nels) return PF_ERR_ARG;             // pf_mesh_set first for a new mesh
     const size_t nslot = (size_t)ntot * nels;
     if (nslot >= (size_t)INT32_MAX) return PF_ERR_ARG;
     // Transposed steering: for each equation the element slots that touch it
@@ -304,6 +303,7 @@ int pf_steer_set(pf_ctx *c, int ntot, int nels, int neq, const int *g_g)
     free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
     free_arr(c, c->csr_slot, (size_t)c->nnz);
     free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
+    free_arr(c, c->utemp32, (size_t)c->ntot * c->nels);
     c->ntot = ntot;
     c->nels = nels;
     c->neq = neq;
@@ -313,7 +313,7 @@ int pf_steer_set(pf_ctx *c, int ntot, int nels, int neq, const int *g_g)
     if (!rc) rc = pf_alloc(c, (void **)&c->csr_slot, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int));
     if (!rc) rc = pf_alloc(c, (void **)&c->utemp, nslot * sizeof(double));
     const size_t nv = (size_t)neq + 1;
-    double **vecs[] = {&c->x, &c->r, &c->p, &c->d, &c->u, &c->dinv, &c->diag};
+    double **vecs[] = {&c->x, &c->r, &c->p, &c->d, &c->u, &c->dinv, &c->diag, &c->diag0, &c->f};
     for (double **v : vecs)
         if (!rc) rc = pf_alloc(c, (void **)v, nv * sizeof(double));
     if (rc) return rc;
@@ -325,33 +325,37 @@ int pf_steer_set(pf_ctx *c, int ntot, int nels, int neq, const int *g_g)
     return PF_OK;
 }
 
-static int alloc_matrices(pf_ctx *c, int store)
+static int alloc_k64(pf_ctx *c, int store)
 {
     if (store != PF_STORE_FP64 && store != PF_STORE_FP32) return PF_ERR_ARG;
     free_matrices(c);
-    size_t nk = (size_t)c->ntot * c->ntot * c->nels;
-    int rc = pf_alloc(c, (void **)&c->K64, nk * sizeof(double));
+    const size_t nk = (size_t)c->ntot * c->ntot * c->nels;
     c->store = store;
-    return rc;
+    return pf_alloc(c, (void **)&c->K64, nk * sizeof(double));
 }
 
-// After a FP64 assembly or upload: build the FP32 split for FP32 storage and
-// the assembled diagonal (ascending element order, as SCATTER adds it).
-static int finish_matrices(pf_ctx *c, int store)
+// After a FP64 assembly or upload into K64: the assembled diagonal
+// (ascending element order, as SCATTER adds it) and, for FP32 storage, the
+// hi/lo split, after which the FP64 copy is released.
+static int finish_matrices(pf_ctx *c)
 {
-    int rc;
-    if (store == PF_STORE_FP32) {
-        size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+    const size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+    const size_t nv = (size_t)c->neq + 1;
+    int rc = pf_launch_diag(c, c->diag0);
+    if (rc) return rc;
+    PF_HIP(hipMemcpyAsync(c->diag, c->diag0, nv * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (c->store == PF_STORE_FP32) {
         rc = pf_alloc(c, (void **)&c->K32hi, nk * sizeof(float));
         if (!rc) rc = pf_alloc(c, (void **)&c->K32lo, nk * sizeof(float));
+        if (!rc && !c->utemp32) rc = pf_alloc(c, (void **)&c->utemp32, (size_t)c->ntot * c->nels * sizeof(float));
         if (!rc) rc = pf_launch_split_fp32(c);
         if (rc) return rc;
+        PF_HIP(hipStreamSynchronize(c->stream));
+        free_arr(c, c->K64, nk);
     }
-    rc = pf_launch_diag(c, c->diag);
-    if (rc) return rc;
     c->have_K = true;
     c->pcg_ready = false;
-    free_arr(c, c->fixstore, (size_t)c->neq + 1);
+    free_arr(c, c->fixstore, nv);
     c->nfix = 0;
     PF_HIP(hipStreamSynchronize(c->stream));
     return PF_OK;
@@ -364,7 +368,7 @@ int pf_assemble_elastic(pf_ctx *c, double e, double v, const double *e_elem, int
     if (c->ntot != 3 * c->nod) return PF_ERR_ARG;
     if (!(v > -1.0 && v < 0.5)) return PF_ERR_ARG;
     PF_HIP(hipSetDevice(c->device));
-    int rc = alloc_matrices(c, store);
+    int rc = alloc_k64(c, store);
     if (rc) return rc;
     double *ee = nullptr;
     if (e_elem) {
@@ -383,16 +387,16 @@ int pf_assemble_elastic(pf_ctx *c, double e, double v, const double *e_elem, int
         pf_free(c, ee, sizeof(double) * c->nels);
     }
     if (rc) return rc;
-    return finish_matrices(c, store);
+    return finish_matrices(c);
 }
 
 int pf_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem, int store)
 {
     if (!c) return PF_ERR_ARG;
     if (!c->g_coord || !c->g_g) return PF_ERR_STATE;
-    if (c->ntot != c->nod) return PF_ERR_ARG;
+    if (c->ntot != c->nod || c->nod != 8) return PF_ERR_ARG;
     PF_HIP(hipSetDevice(c->device));
-    int rc = alloc_matrices(c, store);
+    int rc = alloc_k64(c, store);
     if (rc) return rc;
     double *kk = nullptr;
     if (k_elem) {
@@ -406,7 +410,16 @@ int pf_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double
         pf_free(c, kk, sizeof(double) * c->nels);
     }
     if (rc) return rc;
-    return finish_matrices(c, store);
+    return finish_matrices(c);
+}
+
+// storkm_pp is moved through a bounded staging buffer, chunk by chunk.
+static long staging_elems(const pf_ctx *c)
+{
+    const long per = (long)c->ntot * c->ntot * (long)sizeof(double);
+    long n = (256L << 20) / per;
+    if (n < 1) n = 1;
+    return n < c->nels ? n : c->nels;
 }
 
 int pf_storkm_set(pf_ctx *c, const double *storkm, int store)
@@ -414,21 +427,44 @@ int pf_storkm_set(pf_ctx *c, const double *storkm, int store)
     if (!c || !storkm) return PF_ERR_ARG;
     if (!c->g_g) return PF_ERR_STATE;
     PF_HIP(hipSetDevice(c->device));
-    int rc = alloc_matrices(c, store);
+    int rc = alloc_k64(c, store);
     if (rc) return rc;
-    size_t nk = (size_t)c->ntot * c->ntot * c->nels;
-    PF_HIP(hipMemcpyAsync(c->K64, storkm, nk * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    return finish_matrices(c, store);
+    const long chunk = staging_elems(c), nn = (long)c->ntot * c->ntot;
+    double *stage = nullptr;
+    rc = pf_alloc(c, (void **)&stage, (size_t)chunk * nn * sizeof(double));
+    for (long e0 = 0; !rc && e0 < c->nels; e0 += chunk) {
+        const long nel = std::min(chunk, (long)c->nels - e0);
+        hipError_t he = hipMemcpyAsync(stage, storkm + e0 * nn, (size_t)nel * nn * sizeof(double),
+                                       hipMemcpyHostToDevice, c->stream);
+        if (he != hipSuccess) rc = pf_set_hip_error(he, "storkm upload", __FILE__, __LINE__);
+        if (!rc) rc = pf_launch_from_colmajor(c, stage, e0, nel);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    pf_free(c, stage, (size_t)chunk * nn * sizeof(double));
+    if (rc) return rc;
+    return finish_matrices(c);
 }
 
 int pf_storkm_get(pf_ctx *c, double *storkm)
 {
     if (!c || !storkm) return PF_ERR_ARG;
     if (!c->have_K) return PF_ERR_STATE;
-    size_t nk = (size_t)c->ntot * c->ntot * c->nels;
-    PF_HIP(hipMemcpyAsync(storkm, c->K64, nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    PF_HIP(hipStreamSynchronize(c->stream));
-    return PF_OK;
+    PF_HIP(hipSetDevice(c->device));
+    const long chunk = staging_elems(c), nn = (long)c->ntot * c->ntot;
+    double *stage = nullptr;
+    int rc = pf_alloc(c, (void **)&stage, (size_t)chunk * nn * sizeof(double));
+    for (long e0 = 0; !rc && e0 < c->nels; e0 += chunk) {
+        const long nel = std::min(chunk, (long)c->nels - e0);
+        rc = pf_launch_to_colmajor(c, stage, e0, nel);
+        if (rc) break;
+        hipError_t he = hipMemcpyAsync(storkm + e0 * nn, stage, (size_t)nel * nn * sizeof(double),
+                                       hipMemcpyDeviceToHost, c->stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+        if (he != hipSuccess) rc = pf_set_hip_error(he, "storkm download", __FILE__, __LINE__);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    pf_free(c, stage, (size_t)chunk * nn * sizeof(double));
+    return rc;
 }
 
 int pf_diag_get(pf_ctx *c, double *diag)
@@ -448,15 +484,17 @@ int pf_fixed_set(pf_ctx *c, int nfix, const int *no_f, double penalty, double *s
         if (no_f[i] < 1 || no_f[i] > c->neq) return PF_ERR_ARG;
     PF_HIP(hipSetDevice(c->device));
     size_t nv = (size_t)c->neq + 1;
-    // Start from the element diagonal again so repeated calls do not stack.
-    int rc = pf_launch_diag(c, c->diag);
-    if (rc) return rc;
     free_arr(c, c->fixstore, nv);
     c->nfix = 0;
     c->pcg_ready = false;
-    if (nfix == 0) return PF_OK;
+    // Start from the element diagonal again so repeated calls do not stack.
+    PF_HIP(hipMemcpyAsync(c->diag, c->diag0, nv * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (nfix == 0) {
+        PF_HIP(hipStreamSynchronize(c->stream));
+        return PF_OK;
+    }
     std::vector<double> dg(nv);
-    PF_HIP(hipMemcpyAsync(dg.data(), c->diag, nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PF_HIP(hipMemcpyAsync(dg.data(), c->diag0, nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PF_HIP(hipStreamSynchronize(c->stream));
     std::vector<double> fs(nv, 0.0);
     // p123.f90:120-125: diag(j) = diag(j) + penalty; store(i) = diag(j).
@@ -466,7 +504,7 @@ int pf_fixed_set(pf_ctx *c, int nfix, const int *no_f, double penalty, double *s
         fs[j] = dg[j];
         if (store_out) store_out[i] = dg[j];
     }
-    rc = pf_alloc(c, (void **)&c->fixstore, nv * sizeof(double));
+    int rc = pf_alloc(c, (void **)&c->fixstore, nv * sizeof(double));
     if (rc) return rc;
     PF_HIP(hipMemcpyAsync(c->diag, dg.data(), nv * sizeof(double), hipMemcpyHostToDevice, c->stream));
     PF_HIP(hipMemcpyAsync(c->fixstore, fs.data(), nv * sizeof(double), hipMemcpyHostToDevice, c->stream));

@@ -6,27 +6,32 @@
 // src/modules/mpi/gather_scatter.f90:547-993; DOT_PRODUCT_P/SUM_P
 // src/modules/mpi/maths.f90:168-216,271-315; checon_par maths.f90:999-1069.
 //
-// Per PCG iteration four kernels run on the context's stream:
-//   k_ebe      gathers p through the steering (p has a leading 0 so a
-//              restrained freedom reads 0), forms u_e = K_e p_e with one
-//              64-lane wavefront per hex20 element (lane = row, K read one
-//              column at a time, p_e broadcast from lane j), writes utemp;
-//   k_scatter  sums each equation's element slots in ascending element order
-//              (the order SCATTER adds them at one rank), applies the p123
-//              fixed-freedom rows, accumulates p.u and, in its last block,
-//              alpha = up / (p.u);
-//   k_update   xnew = x + p*alpha, r = r - u*alpha, d = dinv*r, partial
-//              (r,d), (r,r), max|xnew|, max|xnew-x|; last block: beta,
-//              stopping rule, iteration count, freeze flag;
-//   k_pupdate  p = d + p*beta.
+// One PCG iteration on the context's stream:
+//   k_ebe         gathers p through the steering (p has a leading 0 so a
+//                 restrained freedom reads 0), stages the element vector in
+//                 LDS and forms u_e = K_e p_e, one 64-lane wavefront per
+//                 hex20 element (lane = row, 16-byte column-pair or
+//                 column-quad loads), writing utemp;
+//   k_scatter_pu  sums each equation's element slots in ascending element
+//                 order (the order SCATTER adds them at one rank), applies
+//                 the p123 fixed-freedom rows, block sums of p.u;
+//   k_fold_alpha  (one block) alpha = up / (p.u);
+//   k_update      xnew = x + p*alpha, r = r - u*alpha, d = dinv*r, block
+//                 sums of (r,d), (r,r), block maxima of |xnew|, |xnew-x|;
+//   k_fold_update (one block) beta, the stopping rule, the iteration count
+//                 and the freeze flag;
+//   k_pupdate     p = d + p*beta.
+// The mixed solver adds a true-residual replacement (reliable update) on
+// the iterations its fold kernel flags: k_ebe over hi+lo in FP64 with x,
+// k_scatter_res (r = f - K x, d = dinv*r) and k_fold_refine.
 // Every kernel returns at once when the freeze flag is set, so iterations
 // enqueued after convergence are no-ops and the count equals the host loop's.
 //
-// All FP64 vector arithmetic is written with explicit roundings (__dadd_rn,
-// __dmul_rn) so the compiler cannot fuse it; the element product is a chain
-// of fused multiply-adds in ascending column order.  Reductions use one fixed
-// order (pf_reduce_grid in the public header), so results are reproducible
-// bit for bit from run to run.
+// FP64 vector arithmetic is compiled without contraction: every product and
+// sum rounds as written; the element product is an explicit chain of fused
+// multiply-adds in ascending column order.  Reductions use one fixed order
+// (pf_reduce_grid in the public header) and no atomics, so results are
+// reproducible bit for bit.
 #include <cmath>
 #include <vector>
 
@@ -36,14 +41,14 @@ namespace {
 
 constexpr int MAXG = PF_REDUCE_MAX_GRID;
 
-__device__ __forceinline__ double bcast64(double v, int src)
+enum { KIND_F64 = 0, KIND_F32 = 1, KIND_ACC = 2 };   // K_e storage / arithmetic
+enum { RUN_ALWAYS = 0, RUN_LIVE = 1, RUN_REFINE = 2 };  // state gating
+
+__device__ __forceinline__ bool skip(const pf_state *st, int gate)
 {
-    // v_readlane on both halves: src is a compile-time constant in the
-    // unrolled loops below, so this is two scalar reads and no LDS traffic.
-    const unsigned long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffu), src);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-    return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    if (!st || gate == RUN_ALWAYS) return false;
+    if (st->frozen) return true;
+    return gate == RUN_REFINE && !st->refine;
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -61,14 +66,11 @@ __device__ __forceinline__ double wave_max(double v)
 }
 
 // Folds NQ per-thread values of this block (sum or max per quantity) and
-// stores the block results in partials[q*MAXG + blockIdx.x]; returns true in
-// every thread of the block that finished last.
+// stores the block results in partials[qid*MAXG + blockIdx.x].
 template <int NQ>
-__device__ bool block_to_partials(double (&v)[NQ], const int (&qid)[NQ], const bool (&ismax)[NQ],
-                                  double *partials, unsigned int *ticket)
+__device__ void block_to_partials(double (&v)[NQ], const int (&qid)[NQ], const bool (&ismax)[NQ], double *partials)
 {
     __shared__ double ws[NQ][PF_BLOCK / 64];
-    __shared__ int last;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) v[k] = ismax[k] ? wave_max(v[k]) : wave_sum(v[k]);
@@ -83,24 +85,17 @@ __device__ bool block_to_partials(double (&v)[NQ], const int (&qid)[NQ], const b
             for (int w = 1; w < PF_BLOCK / 64; ++w) s = ismax[k] ? fmax(s, ws[k][w]) : __dadd_rn(s, ws[k][w]);
             partials[qid[k] * MAXG + blockIdx.x] = s;
         }
-        __threadfence();
-        const unsigned int t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    return last != 0;
 }
 
-// Run by the last block only: folds the G block results of each quantity in
-// the fixed order and returns them in thread 0 (out[] valid there).
+// In a one-block kernel: folds the G block results of each quantity in the
+// fixed order; out[] is valid in thread 0.
 template <int NQ>
 __device__ void fold_partials(double (&out)[NQ], const int (&qid)[NQ], const bool (&ismax)[NQ],
-                              const double *partials, unsigned int *ticket)
+                              const double *partials, int G)
 {
     __shared__ double ws[NQ][PF_BLOCK / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __threadfence();
-    const int G = gridDim.x;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
         double s = 0.0;
@@ -119,7 +114,6 @@ __device__ void fold_partials(double (&out)[NQ], const int (&qid)[NQ], const boo
             for (int w = 1; w < PF_BLOCK / 64; ++w) s = ismax[k] ? fmax(s, ws[k][w]) : __dadd_rn(s, ws[k][w]);
             out[k] = s;
         }
-        *ticket = 0u;
     }
 }
 
@@ -130,51 +124,98 @@ template <int NT> struct Lanes {
     static constexpr int EPB = EPW * (PF_BLOCK / 64);   // elements per block
 };
 
-// Element products in FP64.  GATHER: p is the neq+1 vector with p[0] = 0 and
-// the element vector is p[g_g]; otherwise src is pmul (ntot,nels).
-template <int NT, bool GATHER>
-__global__ __launch_bounds__(PF_BLOCK) void k_ebe_f64(const double *__restrict__ K, const int *__restrict__ gg,
-                                                     const double *__restrict__ src, double *__restrict__ utemp,
-                                                     int nels, const pf_state *__restrict__ st)
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef double d2_t __attribute__((ext_vector_type(2)));
+
+// Element products.  GATHER: src is a neq+1 vector with src[0] = 0 and the
+// element vector is src[g_g]; otherwise src is pmul (ntot,nels).
+//   KIND_F64: FP64 column pairs, FP64 chain -> out (double)
+//   KIND_F32: FP32 hi column quads, FP32 chain on fl32(p) -> out (float)
+//   KIND_ACC: (double)hi + (double)lo, FP64 chain -> out (double)
+template <int NT, int KIND, bool GATHER, typename OUT>
+__global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
+                                                 const float *__restrict__ Klo, const int *__restrict__ gg,
+                                                 const double *__restrict__ src, OUT *__restrict__ out, int nels,
+                                                 const pf_state *__restrict__ st, int gate)
 {
-    if (st && st->frozen) return;
+    if (skip(st, gate)) return;
     using L = Lanes<NT>;
+    __shared__ double ps[PF_BLOCK / 64][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sub = lane / L::W, li = lane % L::W;
     const long e = ((long)blockIdx.x * (PF_BLOCK / 64) + wave) * L::EPW + sub;
     const long ec = e < nels ? e : (long)nels - 1;
     const int i = li < NT ? li : NT - 1;
-    const double *__restrict__ Ke = K + ec * (long)(NT * NT) + i;
-    // Issue every column load of K_e before the dependent steering -> p
-    // gather so the element's whole matrix is in flight at once.
-    double kc[NT];
+    const int g = GATHER ? gg[ec * NT + i] : 0;
+    const double *pw = &ps[wave][sub * L::W];
+    if constexpr (KIND == KIND_F64) {
+        // Every column pair of K_e is in flight before the dependent gather.
+        const d2_t *Kp = reinterpret_cast<const d2_t *>(K64) + ec * (long)(NT * NT / 2) + i;
+        d2_t kc[NT / 2];
 #pragma unroll
-    for (int j = 0; j < NT; ++j) kc[j] = __builtin_nontemporal_load(&Ke[j * NT]);
-    __builtin_amdgcn_sched_barrier(0);
-    const double pe = GATHER ? src[gg[ec * NT + i]] : src[ec * NT + i];
-    double acc = 0.0;
+        for (int j = 0; j < NT / 2; ++j) kc[j] = __builtin_nontemporal_load(&Kp[j * NT]);
+        ps[wave][lane] = GATHER ? src[g] : src[ec * NT + i];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double acc = 0.0;
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
-        const double pj = (L::W == 64) ? bcast64(pe, j) : __shfl(pe, sub * L::W + j, 64);
-        acc = __fma_rn(kc[j], pj, acc);
+        for (int j = 0; j < NT / 2; ++j) {
+            acc = __fma_rn(kc[j].x, pw[2 * j], acc);
+            acc = __fma_rn(kc[j].y, pw[2 * j + 1], acc);
+        }
+        if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
+    } else if constexpr (KIND == KIND_F32) {
+        const f4_t *Kq = reinterpret_cast<const f4_t *>(Khi) + ec * (long)(NT * NT / 4) + i;
+        f4_t kc[NT / 4];
+#pragma unroll
+        for (int j = 0; j < NT / 4; ++j) kc[j] = __builtin_nontemporal_load(&Kq[j * NT]);
+        ps[wave][lane] = GATHER ? src[g] : src[ec * NT + i];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < NT / 4; ++j) {
+            acc = __fmaf_rn(kc[j].x, (float)pw[4 * j], acc);
+            acc = __fmaf_rn(kc[j].y, (float)pw[4 * j + 1], acc);
+            acc = __fmaf_rn(kc[j].z, (float)pw[4 * j + 2], acc);
+            acc = __fmaf_rn(kc[j].w, (float)pw[4 * j + 3], acc);
+        }
+        if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
+    } else {
+        const f4_t *Kh = reinterpret_cast<const f4_t *>(Khi) + ec * (long)(NT * NT / 4) + i;
+        const f4_t *Kl = reinterpret_cast<const f4_t *>(Klo) + ec * (long)(NT * NT / 4) + i;
+        ps[wave][lane] = GATHER ? src[g] : src[ec * NT + i];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < NT / 4; ++j) {
+            const f4_t h = __builtin_nontemporal_load(&Kh[j * NT]);
+            const f4_t l = __builtin_nontemporal_load(&Kl[j * NT]);
+            acc = __fma_rn((double)h.x + (double)l.x, pw[4 * j], acc);
+            acc = __fma_rn((double)h.y + (double)l.y, pw[4 * j + 1], acc);
+            acc = __fma_rn((double)h.z + (double)l.z, pw[4 * j + 2], acc);
+            acc = __fma_rn((double)h.w + (double)l.w, pw[4 * j + 3], acc);
+        }
+        if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
     }
-    if (e < nels && li < NT) utemp[e * NT + li] = acc;
 }
 
 // SCATTER fused with p.u.  u and p have the leading 0 slot.
+template <typename UT>
 __global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__ ptr, const int *__restrict__ slot,
-                                                        const double *__restrict__ utemp,
+                                                        const UT *__restrict__ utemp,
                                                         const double *__restrict__ fixstore,
                                                         const double *__restrict__ p, double *__restrict__ u,
-                                                        int neq, double *partials, pf_state *st)
+                                                        int neq, double *partials, const pf_state *st)
 {
-    if (st->frozen) return;
+    if (skip(st, RUN_LIVE)) return;
     double acc[1] = {0.0};
     const int stride = gridDim.x * PF_BLOCK;
     for (int q = blockIdx.x * PF_BLOCK + threadIdx.x; q < neq; q += stride) {
         const int a = ptr[q], b = ptr[q + 1];
         double s = 0.0;
-        for (int k = a; k < b; ++k) s = __dadd_rn(s, utemp[slot[k]]);
+        for (int k = a; k < b; ++k) s = __dadd_rn(s, (double)utemp[slot[k]]);
         const double pq = p[q + 1];
         if (fixstore) {
             const double fs = fixstore[q + 1];
@@ -185,29 +226,35 @@ __global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__
     }
     constexpr int qid[1] = {PF_Q_PU};
     constexpr bool mx[1] = {false};
-    if (block_to_partials<1>(acc, qid, mx, partials, &st->ticket[0])) {
-        double out[1];
-        fold_partials<1>(out, qid, mx, partials, &st->ticket[0]);
-        if (threadIdx.x == 0) {
-            const double alpha = st->up / out[0];
-            st->alpha = alpha;
-            if (!isfinite(alpha)) {
-                st->breakdown = 1;
-                st->frozen = 1;
-            }
+    block_to_partials<1>(acc, qid, mx, partials);
+}
+
+__global__ __launch_bounds__(PF_BLOCK) void k_fold_alpha(const double *partials, int G, pf_state *st)
+{
+    if (skip(st, RUN_LIVE)) return;
+    constexpr int qid[1] = {PF_Q_PU};
+    constexpr bool mx[1] = {false};
+    double out[1];
+    fold_partials<1>(out, qid, mx, partials, G);
+    if (threadIdx.x == 0) {
+        const double alpha = st->up / out[0];
+        st->alpha = alpha;
+        if (!isfinite(alpha)) {
+            st->breakdown = 1;
+            st->frozen = 1;
         }
     }
 }
 
-// x/r/d updates, (r,d), (r,r), the checon maxima, then beta and the rule.
+// x/r/d updates and the partial (r,d), (r,r), max|xnew|, max|xnew-x|.
 __global__ __launch_bounds__(PF_BLOCK) void k_update(double *__restrict__ x, double *__restrict__ r,
                                                     double *__restrict__ d, const double *__restrict__ p,
                                                     const double *__restrict__ u, const double *__restrict__ dinv,
-                                                    int neq, double *partials, pf_state *st)
+                                                    int neq, double *partials, const pf_state *st)
 {
-    if (st->frozen) return;
+    if (skip(st, RUN_LIVE)) return;
     const double alpha = st->alpha;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};   // rd, rr, max|xnew|, max|xnew-x|
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
     const int stride = gridDim.x * PF_BLOCK;
     for (int q = blockIdx.x * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
         const double xo = x[q];
@@ -224,40 +271,112 @@ __global__ __launch_bounds__(PF_BLOCK) void k_update(double *__restrict__ x, dou
     }
     constexpr int qid[4] = {PF_Q_RD, PF_Q_RR, PF_Q_MAXX, PF_Q_MAXDX};
     constexpr bool mx[4] = {false, false, true, true};
-    if (block_to_partials<4>(v, qid, mx, partials, &st->ticket[1])) {
-        double out[4];
-        fold_partials<4>(out, qid, mx, partials, &st->ticket[1]);
-        if (threadIdx.x == 0) {
-            const double rd = out[0], rr = out[1];
-            const double beta = rd / st->up;
-            const double crit = st->stop_mode == PF_STOP_CHECON ? out[3] / out[2] : sqrt(rr / st->rr0);
-            st->beta = beta;
-            st->up = rd;
-            st->rr = rr;
-            st->crit = crit;
-            st->iters += 1;
-            st->converged = crit <= st->tol;
-            if (!isfinite(beta) || !isfinite(rd)) st->breakdown = 1;
-            if (st->converged || st->iters >= st->limit || st->breakdown) st->frozen = 1;
+    block_to_partials<4>(v, qid, mx, partials);
+}
+
+// End of an iteration: beta = rd/up, the stopping rule, the count.
+__device__ void finish_iteration(pf_state *st, double rd, double rr, double maxx, double maxdx)
+{
+    const double beta = rd / st->up;
+    const double crit = st->stop_mode == PF_STOP_CHECON ? maxdx / maxx : sqrt(rr / st->rr0);
+    st->beta = beta;
+    st->up = rd;
+    st->rr = rr;
+    st->crit = crit;
+    st->iters += 1;
+    st->converged = crit <= st->tol;
+    if (!isfinite(beta) || !isfinite(rd)) st->breakdown = 1;
+    if (st->converged || st->iters >= st->limit || st->breakdown) st->frozen = 1;
+}
+
+__global__ __launch_bounds__(PF_BLOCK) void k_fold_update(const double *partials, int G, pf_state *st)
+{
+    if (skip(st, RUN_LIVE)) return;
+    constexpr int qid[4] = {PF_Q_RD, PF_Q_RR, PF_Q_MAXX, PF_Q_MAXDX};
+    constexpr bool mx[4] = {false, false, true, true};
+    double out[4];
+    fold_partials<4>(out, qid, mx, partials, G);
+    if (threadIdx.x == 0) {
+        if (st->precision == PF_PREC_MIXED) {
+            // Reliable update: replace the recurrence residual by the true
+            // one when it has dropped by delta since the last replacement,
+            // periodically if asked, and before accepting convergence.
+            st->since_refine += 1;
+            const double rr = out[1];
+            const bool trig = rr <= st->delta2 * st->rr_ref || sqrt(rr / st->rr0) <= st->tol ||
+                              (st->refine_every > 0 && st->since_refine >= st->refine_every);
+            if (trig && isfinite(rr)) {
+                st->refine = 1;
+                return;   // finish_iteration runs after the replacement
+            }
         }
+        finish_iteration(st, out[0], out[1], out[2], out[3]);
+    }
+}
+
+// True residual r = f - K x (K from hi + lo, or the fixed-row override
+// x*store), d = dinv*r, partial (r,d), (r,r).
+__global__ __launch_bounds__(PF_BLOCK) void k_scatter_res(const int *__restrict__ ptr, const int *__restrict__ slot,
+                                                         const double *__restrict__ utemp,
+                                                         const double *__restrict__ fixstore,
+                                                         const double *__restrict__ x, const double *__restrict__ f,
+                                                         double *__restrict__ r, double *__restrict__ d,
+                                                         const double *__restrict__ dinv, int neq, double *partials,
+                                                         const pf_state *st)
+{
+    if (skip(st, RUN_REFINE)) return;
+    double v[2] = {0.0, 0.0};
+    const int stride = gridDim.x * PF_BLOCK;
+    for (int q = blockIdx.x * PF_BLOCK + threadIdx.x; q < neq; q += stride) {
+        double s = 0.0;
+        for (int k = ptr[q]; k < ptr[q + 1]; ++k) s = __dadd_rn(s, utemp[slot[k]]);
+        if (fixstore) {
+            const double fs = fixstore[q + 1];
+            if (fs != 0.0) s = __dmul_rn(x[q + 1], fs);
+        }
+        const double rq = __dsub_rn(f[q + 1], s);
+        const double dq = __dmul_rn(dinv[q + 1], rq);
+        r[q + 1] = rq;
+        d[q + 1] = dq;
+        v[0] = __dadd_rn(v[0], __dmul_rn(rq, dq));
+        v[1] = __dadd_rn(v[1], __dmul_rn(rq, rq));
+    }
+    constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
+    constexpr bool mx[2] = {false, false};
+    block_to_partials<2>(v, qid, mx, partials);
+}
+
+__global__ __launch_bounds__(PF_BLOCK) void k_fold_refine(const double *partials, int G, pf_state *st)
+{
+    if (skip(st, RUN_REFINE)) return;
+    constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
+    constexpr bool mx[2] = {false, false};
+    double out[2];
+    fold_partials<2>(out, qid, mx, partials, G);
+    if (threadIdx.x == 0) {
+        st->refine = 0;
+        st->refinements += 1;
+        st->since_refine = 0;
+        st->rr_ref = out[1];
+        finish_iteration(st, out[0], out[1], 0.0, 0.0);
     }
 }
 
 __global__ __launch_bounds__(PF_BLOCK) void k_pupdate(double *__restrict__ p, const double *__restrict__ d, int neq,
                                                      const pf_state *__restrict__ st)
 {
-    if (st->frozen) return;
+    if (skip(st, RUN_LIVE)) return;
     const double beta = st->beta;
     const int stride = gridDim.x * PF_BLOCK;
     for (int q = blockIdx.x * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride)
         p[q] = __dadd_rn(d[q], __dmul_rn(p[q], beta));
 }
 
-// PCG start: dinv = 1/diag, d = dinv*r, p = d, x = 0, up = (r,d), rr0 = (r,r).
+// PCG start: dinv = 1/diag, d = dinv*r, p = d, x = 0, f = r; partial (r,d), (r,r).
 __global__ __launch_bounds__(PF_BLOCK) void k_pcg_init(double *__restrict__ x, const double *__restrict__ r,
                                                       double *__restrict__ d, double *__restrict__ p,
                                                       double *__restrict__ dinv, const double *__restrict__ diag,
-                                                      int neq, double *partials, pf_state *st)
+                                                      double *__restrict__ f, int neq, double *partials)
 {
     double v[2] = {0.0, 0.0};
     const int stride = gridDim.x * PF_BLOCK;
@@ -269,33 +388,47 @@ __global__ __launch_bounds__(PF_BLOCK) void k_pcg_init(double *__restrict__ x, c
         d[q] = dq;
         p[q] = dq;
         x[q] = 0.0;
+        f[q] = rq;
         v[0] = __dadd_rn(v[0], __dmul_rn(rq, dq));
         v[1] = __dadd_rn(v[1], __dmul_rn(rq, rq));
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        x[0] = 0.0; d[0] = 0.0; p[0] = 0.0; dinv[0] = 0.0;
+        x[0] = 0.0; d[0] = 0.0; p[0] = 0.0; dinv[0] = 0.0; f[0] = 0.0;
     }
     constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
     constexpr bool mx[2] = {false, false};
-    if (block_to_partials<2>(v, qid, mx, partials, &st->ticket[2])) {
-        double out[2];
-        fold_partials<2>(out, qid, mx, partials, &st->ticket[2]);
-        if (threadIdx.x == 0) {
-            st->up = out[0];
-            st->rr0 = out[1];
-            st->rr = out[1];
-            st->iters = 0;
-            st->converged = 0;
-            st->breakdown = !isfinite(out[0]) || !isfinite(out[1]);
-            // A zero right-hand side is already solved by x = 0.
-            st->frozen = (out[1] == 0.0) || st->breakdown;
-            st->converged = out[1] == 0.0;
-            st->crit = 0.0;
-        }
+    block_to_partials<2>(v, qid, mx, partials);
+}
+
+__global__ __launch_bounds__(PF_BLOCK) void k_fold_init(const double *partials, int G, pf_state *st)
+{
+    constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
+    constexpr bool mx[2] = {false, false};
+    double out[2];
+    fold_partials<2>(out, qid, mx, partials, G);
+    if (threadIdx.x == 0) {
+        st->up = out[0];
+        st->rr0 = out[1];
+        st->rr = out[1];
+        st->rr_ref = out[1];
+        st->iters = 0;
+        st->breakdown = !isfinite(out[0]) || !isfinite(out[1]);
+        // A zero right-hand side is already solved by x = 0.
+        st->converged = out[1] == 0.0;
+        st->frozen = st->converged || st->breakdown;
+        st->crit = 0.0;
     }
 }
 
 // ---------------------------------------------------------- primitives
+typedef unsigned u4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_copy(const u4_t *__restrict__ src, u4_t *__restrict__ dst, long n16)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_gather(const int *__restrict__ gg, const double *__restrict__ p, double *__restrict__ pmul, long n)
 {
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
@@ -319,11 +452,10 @@ __global__ void k_scatter_prim(const int *__restrict__ ptr, const int *__restric
     }
 }
 
-// Generic reduction for the primitives: mode 0 = sum a*b, 1 = sum a,
-// 2 = max|a| and max|a-b| then b = a (checon_par).  Result in out[0..1].
+// Reductions for the primitives: mode 0 = sum a*b, 1 = sum a,
+// 2 = max|a| and max|a-b| then b = a (checon_par).
 __global__ __launch_bounds__(PF_BLOCK) void k_reduce_prim(const double *__restrict__ a, double *__restrict__ b,
-                                                         long n, int mode, double *partials, unsigned int *ticket,
-                                                         double *out)
+                                                         long n, int mode, double *partials)
 {
     double v[2] = {0.0, 0.0};
     const long stride = (long)gridDim.x * PF_BLOCK;
@@ -339,27 +471,34 @@ __global__ __launch_bounds__(PF_BLOCK) void k_reduce_prim(const double *__restri
             b[i] = ai;
         }
     }
+    constexpr int qid[2] = {0, 1};
     if (mode == 2) {
-        constexpr int qid[2] = {0, 1};
         constexpr bool mx[2] = {true, true};
-        if (block_to_partials<2>(v, qid, mx, partials, ticket)) {
-            double o[2];
-            fold_partials<2>(o, qid, mx, partials, ticket);
-            if (threadIdx.x == 0) { out[0] = o[0]; out[1] = o[1]; }
-        }
+        block_to_partials<2>(v, qid, mx, partials);
     } else {
-        double w[1] = {v[0]};
-        constexpr int qid[1] = {0};
-        constexpr bool mx[1] = {false};
-        if (block_to_partials<1>(w, qid, mx, partials, ticket)) {
-            double o[1];
-            fold_partials<1>(o, qid, mx, partials, ticket);
-            if (threadIdx.x == 0) out[0] = o[0];
-        }
+        constexpr bool mx[2] = {false, false};
+        block_to_partials<2>(v, qid, mx, partials);
     }
 }
 
-// y = K x in FP64 with the fixed rows (pf_apply): scatter into y (no ext).
+__global__ __launch_bounds__(PF_BLOCK) void k_fold_prim(const double *partials, int G, int mode, double *out)
+{
+    constexpr int qid[2] = {0, 1};
+    double o[2];
+    if (mode == 2) {
+        constexpr bool mx[2] = {true, true};
+        fold_partials<2>(o, qid, mx, partials, G);
+    } else {
+        constexpr bool mx[2] = {false, false};
+        fold_partials<2>(o, qid, mx, partials, G);
+    }
+    if (threadIdx.x == 0) {
+        out[0] = o[0];
+        out[1] = o[1];
+    }
+}
+
+// y = K x with the fixed rows (pf_apply): scatter into y (no leading slot).
 __global__ void k_scatter_fix(const int *__restrict__ ptr, const int *__restrict__ slot,
                               const double *__restrict__ utemp, const double *__restrict__ fixstore,
                               const double *__restrict__ xext, double *__restrict__ y, int neq)
@@ -372,31 +511,34 @@ __global__ void k_scatter_fix(const int *__restrict__ ptr, const int *__restrict
     }
 }
 
-template <bool GATHER>
-int launch_ebe_f64(pf_ctx *c, const double *src, double *utemp, const pf_state *st)
+template <int NT, int KIND, bool GATHER, typename OUT>
+int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate)
 {
-    int grid;
-    switch (c->ntot) {
-    case 60:
-        grid = (c->nels + Lanes<60>::EPB - 1) / Lanes<60>::EPB;
-        hipLaunchKernelGGL((k_ebe_f64<60, GATHER>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64, c->g_g, src,
-                           utemp, c->nels, st);
-        break;
-    case 24:
-        grid = (c->nels + Lanes<24>::EPB - 1) / Lanes<24>::EPB;
-        hipLaunchKernelGGL((k_ebe_f64<24, GATHER>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64, c->g_g, src,
-                           utemp, c->nels, st);
-        break;
-    case 8:
-        grid = (c->nels + Lanes<8>::EPB - 1) / Lanes<8>::EPB;
-        hipLaunchKernelGGL((k_ebe_f64<8, GATHER>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64, c->g_g, src,
-                           utemp, c->nels, st);
-        break;
-    default:
-        return PF_ERR_ARG;
-    }
+    const int grid = (c->nels + Lanes<NT>::EPB - 1) / Lanes<NT>::EPB;
+    hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64, c->K32hi,
+                       c->K32lo, c->g_g, src, out, c->nels, st, gate);
     PF_HIP(hipGetLastError());
     return PF_OK;
+}
+
+template <int KIND, bool GATHER, typename OUT>
+int launch_ebe(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate)
+{
+    switch (c->ntot) {
+    case 60: return launch_ebe_nt<60, KIND, GATHER, OUT>(c, src, out, st, gate);
+    case 24: return launch_ebe_nt<24, KIND, GATHER, OUT>(c, src, out, st, gate);
+    case 8: return launch_ebe_nt<8, KIND, GATHER, OUT>(c, src, out, st, gate);
+    default: return PF_ERR_ARG;
+    }
+}
+
+// FP64 element products of the stored matrices (K64, or hi+lo for FP32
+// storage), without state gating: the drop-in primitive and pf_apply.
+template <bool GATHER>
+int launch_ebe_exact(pf_ctx *c, const double *src, double *out)
+{
+    if (c->K64) return launch_ebe<KIND_F64, GATHER, double>(c, src, out, nullptr, RUN_ALWAYS);
+    return launch_ebe<KIND_ACC, GATHER, double>(c, src, out, nullptr, RUN_ALWAYS);
 }
 
 int reduce_prim(pf_ctx *c, const double *a, double *b, int64_t n, int mode, double *out2)
@@ -406,9 +548,9 @@ int reduce_prim(pf_ctx *c, const double *a, double *b, int64_t n, int mode, doub
         return PF_OK;
     }
     const int G = pf_grid_for(n);
-    double *dout = c->partials + (size_t)PF_NQ * MAXG;   // 2 spare slots past the PCG partials
-    hipLaunchKernelGGL(k_reduce_prim, dim3(G), dim3(PF_BLOCK), 0, c->stream, a, b, (long)n, mode, c->partials,
-                       &c->state->ticket[3], dout);
+    double *dout = c->partials + (size_t)PF_NQ * MAXG;   // spare slots past the PCG partials
+    hipLaunchKernelGGL(k_reduce_prim, dim3(G), dim3(PF_BLOCK), 0, c->stream, a, b, (long)n, mode, c->partials);
+    hipLaunchKernelGGL(k_fold_prim, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, mode, dout);
     PF_HIP(hipGetLastError());
     PF_HIP(hipMemcpyAsync(c->scalar_host, dout, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PF_HIP(hipStreamSynchronize(c->stream));
@@ -420,6 +562,16 @@ int reduce_prim(pf_ctx *c, const double *a, double *b, int64_t n, int mode, doub
 }  // namespace
 
 extern "C" {
+
+int pf_dev_copy(pf_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c || (bytes && (!dst || !src)) || bytes % 16) return PF_ERR_ARG;
+    if (!bytes) return PF_OK;
+    const long n16 = (long)(bytes / 16);
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, c->stream, (const u4_t *)src, (u4_t *)dst, n16);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
 
 int pf_gather(pf_ctx *c, const double *p, double *pmul)
 {
@@ -435,7 +587,7 @@ int pf_elem_matvec(pf_ctx *c, const double *pmul, double *utemp)
 {
     if (!c || !pmul || !utemp) return PF_ERR_ARG;
     if (!c->have_K) return PF_ERR_STATE;
-    return launch_ebe_f64<false>(c, pmul, utemp, nullptr);
+    return launch_ebe_exact<false>(c, pmul, utemp);
 }
 
 int pf_scatter(pf_ctx *c, double *u, const double *utemp)
@@ -491,7 +643,14 @@ int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opt
     if (!c->have_K) return PF_ERR_STATE;
     if (opts->limit < 1 || !(opts->tol >= 0.0)) return PF_ERR_ARG;
     if (opts->stop_mode != PF_STOP_CHECON && opts->stop_mode != PF_STOP_RELRES) return PF_ERR_ARG;
-    if (opts->precision != PF_PREC_FP64) return PF_ERR_ARG;
+    if (opts->precision == PF_PREC_FP64) {
+        if (!c->K64) return PF_ERR_ARG;   // FP64 iterations need FP64 storage
+    } else if (opts->precision == PF_PREC_MIXED) {
+        if (!c->K32hi || opts->stop_mode != PF_STOP_RELRES) return PF_ERR_ARG;
+        if (opts->refine_delta < 0.0 || opts->refine_delta >= 1.0 || opts->refine_every < 0) return PF_ERR_ARG;
+    } else {
+        return PF_ERR_ARG;
+    }
     PF_HIP(hipSetDevice(c->device));
     c->opts = *opts;
     PF_HIP(hipMemcpyAsync(c->r + 1, r0, sizeof(double) * c->neq,
@@ -500,10 +659,16 @@ int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opt
     s.tol = opts->tol;
     s.limit = opts->limit;
     s.stop_mode = opts->stop_mode;
+    s.precision = opts->precision;
+    const double delta = opts->refine_delta > 0.0 ? opts->refine_delta : 1e-2;
+    s.delta2 = delta * delta;
+    s.refine_every = opts->refine_every;
     *c->state_host = s;
     PF_HIP(hipMemcpyAsync(c->state, c->state_host, sizeof(pf_state), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_pcg_init, dim3(pf_grid_for(c->neq)), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p,
-                       c->dinv, c->diag, c->neq, c->partials, c->state);
+    const int G = pf_grid_for(c->neq);
+    hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, c->dinv, c->diag,
+                       c->f, c->neq, c->partials);
+    hipLaunchKernelGGL(k_fold_init, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
     PF_HIP(hipGetLastError());
     // The host mirror is read back by pf_pcg_poll; wait so it is not reused
     // while the copy above is still in flight.
@@ -517,23 +682,50 @@ int pf_pcg_run(pf_ctx *c, int n)
     if (!c || n < 0) return PF_ERR_ARG;
     if (!c->pcg_ready) return PF_ERR_STATE;
     const int G = pf_grid_for(c->neq);
+    const bool mixed = c->opts.precision == PF_PREC_MIXED;
     hipEvent_t t0, t1;
     for (int it = 0; it < n; ++it) {
+        int rc;
         bool tm = pf_timing_begin(c, PF_K_EBE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
-        int rc = launch_ebe_f64<true>(c, c->p, c->utemp, c->state);
+        if (mixed)
+            rc = launch_ebe<KIND_F32, true, float>(c, c->p, c->utemp32, c->state, RUN_LIVE);
+        else
+            rc = launch_ebe<KIND_F64, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE);
         if (rc) return rc;
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         tm = pf_timing_begin(c, PF_K_SCATTER, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
-        hipLaunchKernelGGL(k_scatter_pu, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, c->csr_slot, c->utemp,
-                           c->fixstore, c->p, c->u, c->neq, c->partials, c->state);
+        if (mixed)
+            hipLaunchKernelGGL(k_scatter_pu<float>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, c->csr_slot,
+                               c->utemp32, c->fixstore, c->p, c->u, c->neq, c->partials, c->state);
+        else
+            hipLaunchKernelGGL(k_scatter_pu<double>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr,
+                               c->csr_slot, c->utemp, c->fixstore, c->p, c->u, c->neq, c->partials, c->state);
+        hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         tm = pf_timing_begin(c, PF_K_UPDATE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
         hipLaunchKernelGGL(k_update, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, c->u, c->dinv,
                            c->neq, c->partials, c->state);
+        hipLaunchKernelGGL(k_fold_update, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
+        if (mixed) {
+            // True-residual replacement; the kernels do nothing unless the
+            // fold above flagged this iteration.
+            tm = pf_timing_begin(c, PF_K_EBE, &t0, &t1);
+            if (tm) PF_HIP(hipEventRecord(t0, c->stream));
+            rc = launch_ebe<KIND_ACC, true, double>(c, c->x, c->utemp, c->state, RUN_REFINE);
+            if (rc) return rc;
+            if (tm) PF_HIP(hipEventRecord(t1, c->stream));
+            tm = pf_timing_begin(c, PF_K_SCATTER, &t0, &t1);
+            if (tm) PF_HIP(hipEventRecord(t0, c->stream));
+            hipLaunchKernelGGL(k_scatter_res, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, c->csr_slot,
+                               c->utemp, c->fixstore, c->x, c->f, c->r, c->d, c->dinv, c->neq, c->partials,
+                               c->state);
+            hipLaunchKernelGGL(k_fold_refine, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
+            if (tm) PF_HIP(hipEventRecord(t1, c->stream));
+        }
         tm = pf_timing_begin(c, PF_K_PUPDATE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
         hipLaunchKernelGGL(k_pupdate, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->p, c->d, c->neq, c->state);
@@ -582,7 +774,7 @@ int pf_pcg_poll(pf_ctx *c, pf_pcg_result *res)
     res->crit = s.crit;
     res->rr0 = s.rr0;
     res->rr = s.rr;
-    res->refinements = 0;
+    res->refinements = s.refinements;
     res->status = s.breakdown ? PF_ERR_BREAKDOWN : (s.converged ? PF_OK : PF_NOT_CONVERGED);
     return PF_OK;
 }
@@ -636,7 +828,7 @@ int pf_apply(pf_ctx *c, const double *x_host, double *y_host)
         if (e == hipSuccess) e = hipMemcpyAsync(xe + 1, x_host, c->neq * sizeof(double), hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) rc = pf_set_hip_error(e, "pf_apply upload", __FILE__, __LINE__);
     }
-    if (!rc) rc = launch_ebe_f64<true>(c, xe, ut, nullptr);
+    if (!rc) rc = launch_ebe_exact<true>(c, xe, ut);
     if (!rc) {
         hipLaunchKernelGGL(k_scatter_fix, dim3(pf_grid_for(c->neq)), dim3(256), 0, c->stream, c->csr_ptr,
                            c->csr_slot, ut, c->fixstore, xe, y, c->neq);

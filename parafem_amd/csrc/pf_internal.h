@@ -12,31 +12,56 @@
 #include "../../include/parafem_amd.h"
 
 #define PF_BLOCK 256                 // threads per block of the vector kernels
-#define PF_REDUCE_MAX_GRID 2048      // upper bound on reduction grid size G
+#define PF_REDUCE_MAX_GRID 16384     // upper bound on reduction grid size G
 
-// Kernels of one PCG iteration, in launch order (pf_pcg_timing_get indices).
+// Kernels of one PCG iteration, in launch order (pf_pcg_timing_get indices):
+// element products, scatter + p.u (and its fold), x/r/d update (and its
+// fold), p update.  True-residual replacements of the mixed solver are
+// counted with the element products and the scatter.
 enum { PF_K_EBE = 0, PF_K_SCATTER = 1, PF_K_UPDATE = 2, PF_K_PUPDATE = 3, PF_NKERNELS = 4 };
 
-// Number of per-iteration reduction quantities kept in the partials buffer.
+// Per-block partial quantities kept in the partials buffer.
 enum { PF_Q_PU = 0, PF_Q_RD = 1, PF_Q_RR = 2, PF_Q_MAXX = 3, PF_Q_MAXDX = 4, PF_NQ = 5 };
 
-// Device-resident PCG scalars.  Written only by the last block of the
-// reduction kernels; read at the start of the following kernels.
+// Internal element-matrix layouts (the C ABI speaks storkm_pp's column-major
+// order; pf_storkm_set/get convert).  For element e, row i, column j of an
+// ntot x ntot matrix:
+//   FP64 "column pairs": e*ntot^2 + (j/2)*2*ntot + 2*i + (j%2)
+//     -> lane i reads columns (j, j+1) as one 16-byte load
+//   FP32 "column quads": e*ntot^2 + (j/4)*4*ntot + 4*i + (j%4)
+//     -> lane i reads columns j..j+3 as one 16-byte load
+__host__ __device__ inline long pf_idx_pair(long e, int nt, int i, int j)
+{
+    return e * (long)nt * nt + (long)(j >> 1) * 2 * nt + 2 * i + (j & 1);
+}
+__host__ __device__ inline long pf_idx_quad(long e, int nt, int i, int j)
+{
+    return e * (long)nt * nt + (long)(j >> 2) * 4 * nt + 4 * i + (j & 3);
+}
+
+// Device-resident PCG scalars.  Written only by the one-block fold kernels;
+// read at the start of the following kernels.
 struct pf_state {
     double up;        // (r,d) of the current iterate (the reference's "up")
     double alpha;
     double beta;
     double rr0;       // ||r0||^2
-    double rr;        // ||r||^2 (relres mode)
+    double rr;        // ||r||^2 (recurrence residual, or true residual after a replacement)
+    double rr_ref;    // ||r||^2 at the last true-residual replacement (mixed)
     double crit;      // last stopping criterion value
     double tol;
+    double delta2;    // mixed: replacement trigger (delta^2)
     int    iters;
     int    limit;
     int    stop_mode;
+    int    precision;
     int    converged;
     int    frozen;    // 1 once converged, at the limit, or broken down
     int    breakdown;
-    unsigned int ticket[4];   // last-block counters (returned to 0 by the last block)
+    int    refine;    // mixed: a true-residual replacement is due this iteration
+    int    refinements;
+    int    refine_every;
+    int    since_refine;
 };
 
 struct pf_ctx {
@@ -54,11 +79,11 @@ struct pf_ctx {
     int *csr_ptr = nullptr;            // (neq+1) offsets into csr_slot
     int *csr_slot = nullptr;           // element slots e*ntot+i, ascending
     int64_t nnz = 0;
-    // element matrices
+    // element matrices (internal layouts above)
     int store = PF_STORE_FP64;
-    double *K64 = nullptr;             // (ntot,ntot,nels) FP64
-    float *K32hi = nullptr;            // FP32 hi part
-    float *K32lo = nullptr;            // FP32 lo part
+    double *K64 = nullptr;             // FP64 column pairs (PF_STORE_FP64)
+    float *K32hi = nullptr;            // FP32 column quads, hi = fl32(K)
+    float *K32lo = nullptr;            // FP32 column quads, lo = fl32(K - hi)
     bool have_K = false;
     // p123 fixed freedoms: fixstore[eq] = store value, 0 for free equations
     double *fixstore = nullptr;
@@ -67,9 +92,11 @@ struct pf_ctx {
     // PCG vectors: length neq+1, entry 0 is a permanent zero so that a
     // steering value of 0 gathers 0 without a branch.
     double *x = nullptr, *r = nullptr, *p = nullptr, *d = nullptr, *u = nullptr;
-    double *dinv = nullptr, *diag = nullptr;
-    double *utemp = nullptr;           // (ntot,nels) element products
-    double *partials = nullptr;        // PF_NQ * PF_REDUCE_MAX_GRID
+    double *dinv = nullptr, *diag = nullptr, *f = nullptr;
+    double *diag0 = nullptr;           // element diagonal before penalty rows
+    double *utemp = nullptr;           // (ntot,nels) element products, FP64
+    float *utemp32 = nullptr;          // (ntot,nels) element products, FP32
+    double *partials = nullptr;        // PF_NQ * PF_REDUCE_MAX_GRID (+ spare)
     pf_state *state = nullptr;         // device
     pf_state *state_host = nullptr;    // pinned mirror for polling
     double *scalar_host = nullptr;     // pinned scratch for primitive results
@@ -103,3 +130,6 @@ int pf_launch_assemble_elastic(pf_ctx *c, double v2, double vv, double den, doub
 int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem_dev);
 int pf_launch_split_fp32(pf_ctx *c);
 int pf_launch_diag(pf_ctx *c, double *diag_dev);
+// storkm_pp (column-major, nels_chunk elements starting at e0) <-> internal
+int pf_launch_from_colmajor(pf_ctx *c, const double *src, long e0, long nel);
+int pf_launch_to_colmajor(pf_ctx *c, double *dst, long e0, long nel);

@@ -142,10 +142,13 @@ def start_r(ctx, prob: Problem):
 
 
 def solve(ctx, prob: Problem, tol: float, limit: int, stop: str = "checon",
-          precision: str = "fp64", store: str = "fp64"):
-    """Full flow: set-up, assembly, PCG.  Returns (x, SolveResult)."""
+          precision: str = "fp64", store: str | None = None, **kw):
+    """Full flow: set-up, assembly, PCG.  Returns (x, SolveResult).  The
+    mixed solver stores the element matrices as FP32 hi/lo."""
+    if store is None:
+        store = "fp32" if precision == "mixed" else "fp64"
     r = setup(ctx, prob, store=store)
-    return ctx.pcg_solve(r, limit, tol, stop=stop, precision=precision)
+    return ctx.pcg_solve(r, limit, tol, stop=stop, precision=precision, **kw)
 
 
 def nodal_field(prob: Problem, x: np.ndarray) -> np.ndarray:

@@ -27,8 +27,9 @@ def test_status_strings_and_version():
 
 
 def test_reduce_grid_contract():
-    # G = clamp(ceil(n/256), 1, 2048): fixes the dot-product summation order
-    for n, g in ((0, 1), (1, 1), (256, 1), (257, 2), (524288, 2048), (10 ** 8, 2048)):
+    # G = clamp(ceil(n/256), 1, 16384): fixes the dot-product summation order
+    for n, g in ((0, 1), (1, 1), (256, 1), (257, 2), (524288, 2048), (4194304, 16384),
+                 (10 ** 8, 16384)):
         assert api.reduce_grid(n) == g
 
 

@@ -247,3 +247,45 @@ def test_split_run_equals_solve(ctx):
     res2 = ctx.pcg_poll()
     assert res2.iters == res.iters and res2.converged
     assert np.array_equal(ctx.pcg_x(), x)
+
+
+def test_storkm_roundtrip_layouts(ctx):
+    """storkm_pp in, storkm_pp out: exact for FP64 storage; hi+lo within
+    2^-46 of K for FP32 storage."""
+    pr = _distorted(3, seed=6)
+    say("storkm round trip")
+    Ko = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem)
+    ctx.mesh_set(pr.g_coord())
+    ctx.steer_set(pr.g_g, pr.neq)
+    ctx.storkm_set(Ko, store="fp64")
+    assert np.array_equal(ctx.storkm_get(), Ko)
+    ctx.storkm_set(Ko, store="fp32")
+    K2 = ctx.storkm_get()
+    assert np.all(np.abs(K2 - Ko) <= 2.0 ** -46 * np.abs(Ko))
+    assert np.array_equal(ctx.diag_get(), O.diag(pr.g_g, Ko, pr.neq))
+
+
+@pytest.mark.parametrize("n", [6, 12])
+def test_mixed_precision_reaches_fp64_solution(ctx, n):
+    """FP32-stored element matrices, FP32 element products, FP64 vectors and
+    reliable updates: relres 1e-10 on the true residual, displacements within
+    1e-8 of the FP64 solve of the same system at relres 1e-12."""
+    pr = P.elastic_cube(n, lognormal_e=True)
+    say(f"mixed precision n={n} neq={pr.neq}")
+    x64, r64 = P.solve(ctx, pr, 1e-12, 20000, stop="relres")
+    xm, rm = P.solve(ctx, pr, 1e-10, 20000, stop="relres", precision="mixed")
+    assert r64.converged and rm.converged and rm.refinements >= 1
+    y = ctx.apply(xm)
+    assert np.linalg.norm(y - pr.r0) <= 1.5e-10 * np.linalg.norm(pr.r0)
+    assert np.abs(xm - x64).max() <= 1e-8 * np.abs(x64).max()
+    x64b, r64b = P.solve(ctx, pr, 1e-10, 20000, stop="relres")
+    assert rm.iters <= 1.5 * r64b.iters + 10
+
+
+def test_mixed_precision_laplace_penalty(ctx):
+    pr = P.laplace_cube(8, lognormal_k=True)
+    say(f"mixed laplace neq={pr.neq}")
+    x64, r64 = P.solve(ctx, pr, 1e-12, 20000, stop="relres")
+    xm, rm = P.solve(ctx, pr, 1e-10, 20000, stop="relres", precision="mixed")
+    assert rm.converged
+    assert np.abs(xm - x64).max() <= 1e-8 * np.abs(x64).max()
