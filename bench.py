@@ -5,13 +5,19 @@ Workload: the p121 hex20 linear-elastic cube of 100 x 100 x 100 = 1,000,000
 20-node bricks (12,180,600 equations), unit edges, base fixed, uniform unit
 pressure on the top face, per-element Young's modulus log-normal
 (mu = ln 100, sigma = 0.5, SplitMix64/Box-Muller seed 1), nu = 0.3, element
-matrices assembled on the device and stored in FP64, Jacobi-preconditioned
-PCG in FP64.  A step is one PCG iteration (gather -> element products ->
-scatter -> dots -> recurrences -> stopping test) with every input resident
-in HBM.  The stopping tolerance is 0 so every timed step does full work.
+matrices assembled on the device.
 
-Prints one JSON line (see the "roofline" and "cpu_baseline" keys for the
-dominant kernel's achieved HBM bandwidth and the CPU oracle's rate).
+Headline (value): iterations/s of the mixed-precision Jacobi EBE-PCG --
+element matrices stored as FP32 hi/lo, element products from hi in FP32,
+vectors, dots and recurrences in FP64, reliable updates with the FP64
+(hi+lo) residual.  A step is one PCG iteration (gather -> element products
+-> scatter -> dots -> recurrences -> stopping test) with every input
+resident in HBM; the stopping tolerance is 0 during the timed steps so every
+step does full work.  The same measurement of the plain FP64 path, both
+times-to-solution at relative residual 1e-10 (the solves/s half of the
+metric) and the CPU oracle's rate are reported beside it.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -34,11 +40,54 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(nels, ntot, neq, s_k=8):
-    """Per-iteration algorithmic HBM bytes: every element matrix once, the
-    int32 steering once, and 15 FP64 vector streams of length neq."""
-    per_elem = ntot * ntot * s_k + ntot * 4
-    return nels * per_elem, nels * per_elem + 15 * neq * 8, per_elem
+def ebe_bytes(ntot, store):
+    """Algorithmic HBM bytes of one element product: the element matrix once
+    (FP64, or the FP32 hi part) and the int32 steering once."""
+    return ntot * ntot * (8 if store == "fp64" else 4) + ntot * 4
+
+
+def iteration_bytes(nels, ntot, neq, store):
+    """Per-iteration algorithmic bytes: element matrices + steering + 15 FP64
+    vector streams of length neq."""
+    return nels * ebe_bytes(ntot, store) + 15 * neq * 8
+
+
+def time_iterations(ctx, pr, precision, steps, warmup):
+    """Iterations/s with tol = 0 (no early exit), then a per-kernel pass with
+    HIP events on the context's stream."""
+    limit = warmup + steps + 200
+    ctx.pcg_begin(pr.r0, limit, 0.0, stop="relres", precision=precision)
+    ctx.pcg_run(warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.pcg_run(steps)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    res = ctx.pcg_poll()
+    if res.status == 6 or res.iters != warmup + steps:
+        raise RuntimeError(f"{precision}: unexpected state {res}")
+    nprof = min(steps, 40)
+    ctx.timing_enable(True)
+    ctx.pcg_run(nprof)
+    tk = ctx.timing_get()
+    ctx.timing_enable(False)
+    return elapsed, tk, nprof
+
+
+def solve(ctx, pr, precision, store, tol):
+    from parafem_amd import problems as P
+    t0 = time.perf_counter()
+    r = P.setup(ctx, pr, store=store)
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    x, res = ctx.pcg_solve(r, 100000, tol, stop="relres", precision=precision, check_every=64)
+    t_solve = time.perf_counter() - t0
+    y = ctx.apply(x)
+    true_rel = float(np.linalg.norm(y - pr.r0) / np.linalg.norm(pr.r0))
+    return x, {"iters": res.iters, "refinements": res.refinements, "converged": res.converged,
+               "time_s": round(t_solve, 3), "setup_assembly_s": round(t_setup, 3),
+               "solves_per_s": round(1.0 / t_solve, 5), "true_relres": true_rel}
 
 
 def cpu_baseline(n_sample: int, budget_s: float = 12.0):
@@ -61,7 +110,7 @@ def cpu_baseline(n_sample: int, budget_s: float = 12.0):
     per = max(run(4) - t1, 1e-6) / 2
     n = max(4, int(budget_s / per))
     t = run(n + 2) - t1
-    rate = n / t                                   # iterations/s of the sample
+    rate = n / t
     return {
         "value": rate * pr.nels / 1_000_000,
         "unit": "iter/s",
@@ -79,7 +128,10 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=100, help="elements per cube edge")
-    ap.add_argument("--store", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
+                    help="path whose iteration rate is the headline value")
+    ap.add_argument("--solve-tol", type=float, default=1e-10,
+                    help="relative residual of the time-to-solution runs (0 = skip them)")
     ap.add_argument("--cpu-sample", type=int, default=24, help="edge of the CPU-baseline cube (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (HBM counter calibration)")
@@ -106,42 +158,45 @@ def main():
     pr = P.elastic_cube(args.n, lognormal_e=True)
     log(f"problem built: nels={pr.nels} neq={pr.neq} ({time.perf_counter() - t0:.1f} s)")
     ctx = Context(0)
-    t0 = time.perf_counter()
-    ctx.mesh_set(pr.g_coord())
-    ctx.steer_set(pr.g_g, pr.neq)
-    ctx.sync()
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ctx.assemble_elastic(pr.e, pr.v, pr.e_elem, store=args.store)
-    ctx.sync()
-    t_asm = time.perf_counter() - t0
-    log(f"set-up {t_setup:.2f} s, assembly {t_asm * 1e3:.1f} ms, device memory {ctx.mem_used() / 2**30:.1f} GiB")
+    paths = {}
+    order = ["fp64", "mixed"] if args.precision == "mixed" else ["mixed", "fp64"]
+    for prec in order:
+        store = "fp64" if prec == "fp64" else "fp32"
+        t0 = time.perf_counter()
+        P.setup(ctx, pr, store=store)
+        ctx.sync()
+        t_setup = time.perf_counter() - t0
+        elapsed, tk, nprof = time_iterations(ctx, pr, prec, args.steps, args.warmup)
+        ebe_ms = tk["ebe"][0] / max(tk["ebe"][1], 1)
+        per_elem = ebe_bytes(pr.ntot, store)
+        achieved = pr.nels * per_elem / (ebe_ms * 1e-3) / 1e9
+        iter_ms = sum(v[0] for v in tk.values()) / nprof
+        paths[prec] = {
+            "iter_per_s": round(args.steps / elapsed, 3),
+            "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+            "elapsed_s": elapsed,
+            "store": store,
+            "kernel_ms_per_iter": {k: round(v[0] / nprof, 4) for k, v in tk.items()},
+            "ebe_ms": ebe_ms,
+            "ebe_gbs": round(achieved, 1),
+            "bytes_per_element": per_elem,
+            "iteration_gbs": round(iteration_bytes(pr.nels, pr.ntot, pr.neq, store) / (iter_ms * 1e-3) / 1e9, 1),
+            "setup_assembly_s": round(t_setup, 3),
+            "device_gib": round(ctx.mem_used() / 2**30, 2),
+        }
+        log(f"{prec}: {paths[prec]['iter_per_s']} iter/s, element products {ebe_ms:.3f} ms "
+            f"({achieved:.0f} GB/s), kernels " +
+            ", ".join(f"{k}={v}" for k, v in paths[prec]["kernel_ms_per_iter"].items()))
 
-    limit = args.warmup + 2 * args.steps + 100
-    ctx.pcg_begin(pr.r0, limit, 0.0, stop="relres")
-    ctx.pcg_run(args.warmup)
-    ctx.sync()
-    log("warm-up done")
-    t0 = time.perf_counter()
-    ctx.pcg_run(args.steps)
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    res = ctx.pcg_poll()
-    assert res.iters == args.warmup + args.steps, res
-    assert res.status != 6, "breakdown"
-    log(f"timed {args.steps} iterations: {elapsed * 1e3:.1f} ms")
-
-    # Per-kernel HIP-event timing on the context's stream (separate pass).
-    nprof = min(args.steps, 50)
-    ctx.timing_enable(True)
-    ctx.pcg_run(nprof)
-    tk = ctx.timing_get()
-    ctx.timing_enable(False)
-    ebe_ms = tk["ebe"][0] / max(tk["ebe"][1], 1)
-    iter_ms = sum(v[0] for v in tk.values()) / nprof
-    b_ebe, b_iter, per_elem = algorithmic_bytes(pr.nels, pr.ntot, pr.neq)
-    achieved = b_ebe / (ebe_ms * 1e-3) / 1e9
-    log("kernel ms/launch: " + ", ".join(f"{k}={v[0] / max(v[1], 1):.3f}" for k, v in tk.items()))
+    solves = None
+    if args.solve_tol > 0:
+        solves = {}
+        x64, solves["fp64"] = solve(ctx, pr, "fp64", "fp64", args.solve_tol)
+        log(f"fp64 solve: {solves['fp64']}")
+        xm, solves["mixed"] = solve(ctx, pr, "mixed", "fp32", args.solve_tol)
+        solves["mixed"]["rel_err_vs_fp64"] = float(np.abs(xm - x64).max() / np.abs(x64).max())
+        solves["speedup_mixed_vs_fp64"] = round(solves["fp64"]["time_s"] / solves["mixed"]["time_s"], 3)
+        log(f"mixed solve: {solves['mixed']}")
 
     cpu = None
     if args.cpu_sample > 0:
@@ -149,38 +204,39 @@ def main():
         cpu = cpu_baseline(args.cpu_sample)
         log(f"cpu baseline {cpu['value']:.3f} iter/s ({cpu['cores']} threads)")
 
-    value = args.steps / elapsed
+    head = paths[args.precision]
     out = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": head["iter_per_s"],
         "unit": "iter/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"p121 hex20 elastic cube {args.n}^3 ({pr.nels} elements, {pr.neq} equations), "
-                        "log-normal E (seed 1), FP64 storkm, Jacobi EBE-PCG iteration",
-            "nels": pr.nels, "neq": pr.neq, "store": args.store, "precision": "fp64",
-            "iteration_gbs": round(b_iter / (iter_ms * 1e-3) / 1e9, 1),
-            "iteration_bytes": b_iter,
-            "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in tk.items()},
-            "assembly_ms": round(t_asm * 1e3, 1),
+                        "log-normal E (seed 1), Jacobi EBE-PCG iteration, "
+                        + ("FP32-stored element matrices with FP64 vectors and reliable updates"
+                           if args.precision == "mixed" else "FP64 element matrices"),
+            "nels": pr.nels, "neq": pr.neq, "precision": args.precision,
+            "paths": {k: {kk: vv for kk, vv in v.items() if kk != "elapsed_s"} for k, v in paths.items()},
+            "solve_relres": args.solve_tol,
+            "solves": solves,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_ebe_f64<60> (gather + element products)",
-            "achieved": round(achieved, 1),
+            "kernel": "k_ebe element products (" + ("FP32 hi" if args.precision == "mixed" else "FP64") + ")",
+            "achieved": head["ebe_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(head["ebe_gbs"] / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "bytes_per_element": per_elem,
+            "bytes_per_element": head["bytes_per_element"],
         },
         "cpu_baseline": cpu,
     }

@@ -165,6 +165,18 @@ int pf_sum_p(pf_ctx *ctx, const double *a_dev, int64_t n, double *out);
 /* checon_par(loads, tol, converged, oldlds) (maths.f90:999-1069):
  * converged = max|xnew-x|/max|xnew| <= tol, then x = xnew. */
 int pf_checon_par(pf_ctx *ctx, const double *xnew_dev, double tol, int *converged, double *x_dev, int64_t n);
+/* The whole-array statements of the PCG loop on device vectors of length n
+ * (p121.f90:86-87,91,100-102), each rounded as written:
+ *   pf_vec_fill : x = value                    (u_pp = zero)
+ *   pf_vec_waxpy: w = x + y*a                   (xnew = x + p*alpha,
+ *                 r = r - u*alpha as a = -alpha, p = d + p*beta)
+ *   pf_vec_mul  : z = x*y                       (d = diag_precon*r)
+ *   pf_vec_recip: y = 1/x                       (diag_precon = 1/diag_precon)
+ * w, z, y may alias the inputs. */
+int pf_vec_fill(pf_ctx *ctx, double *x_dev, double value, int64_t n);
+int pf_vec_waxpy(pf_ctx *ctx, double *w_dev, const double *x_dev, const double *y_dev, double a, int64_t n);
+int pf_vec_mul(pf_ctx *ctx, double *z_dev, const double *x_dev, const double *y_dev, int64_t n);
+int pf_vec_recip(pf_ctx *ctx, double *y_dev, const double *x_dev, int64_t n);
 
 /* ----------------------------------------------------- the PCG solve */
 /* The PCG loop of p121.f90:86-104 / p123.f90:133-151 on device-resident
@@ -189,11 +201,14 @@ int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);
 /* No counterpart in the reference (its timing is CPU_TIME around phases,
  * src/modules/shared/timing.f90:35-62).  While enabled, pf_pcg_run records a
  * HIP event pair on the context's stream around every kernel it launches.
- * pf_pcg_timing_get synchronises and returns, per kernel of an iteration
- * (0 element products, 1 scatter + p.u, 2 x/r/d update, 3 p update), the
- * summed milliseconds and the launch count since the last enable. */
+ * pf_pcg_timing_get synchronises and returns, per kernel group of an
+ * iteration (0 element products, 1 scatter + p.u, 2 x/r/d update, 3 p
+ * update, 4 the mixed solver's true-residual replacement, whose launches do
+ * nothing on iterations without a replacement), the summed milliseconds and
+ * the launch count since the last enable. */
+#define PF_TIMING_SLOTS 5
 int pf_pcg_timing_enable(pf_ctx *ctx, int on);
-int pf_pcg_timing_get(pf_ctx *ctx, double ms[4], int counts[4]);
+int pf_pcg_timing_get(pf_ctx *ctx, double ms[PF_TIMING_SLOTS], int counts[PF_TIMING_SLOTS]);
 
 /* ------------------------------------------------- layout introspection */
 /* Grid size G = clamp(ceil(n/256), 1, 16384) (blocks of 256 threads) the

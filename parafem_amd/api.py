@@ -104,6 +104,10 @@ def lib():
         "pf_dot_product_p": (i32, [vp, vp, vp, i64, pd]),
         "pf_sum_p": (i32, [vp, vp, i64, pd]),
         "pf_checon_par": (i32, [vp, vp, dbl, pi, vp, i64]),
+        "pf_vec_fill": (i32, [vp, vp, dbl, i64]),
+        "pf_vec_waxpy": (i32, [vp, vp, vp, vp, dbl, i64]),
+        "pf_vec_mul": (i32, [vp, vp, vp, vp, i64]),
+        "pf_vec_recip": (i32, [vp, vp, vp, i64]),
         "pf_pcg_solve": (i32, [vp, pd, pd, C.POINTER(PcgOpts), C.POINTER(PcgResult)]),
         "pf_pcg_begin": (i32, [vp, vp, i32, C.POINTER(PcgOpts)]),
         "pf_pcg_run": (i32, [vp, i32]),
@@ -353,6 +357,25 @@ class Context:
                "pf_checon_par")
         return bool(conv.value)
 
+    def vec_fill(self, x: DeviceArray, value: float):
+        """x = value"""
+        _check(lib().pf_vec_fill(self.h, x.ptr, float(value), x.n), "pf_vec_fill")
+
+    def vec_waxpy(self, w: DeviceArray, x: DeviceArray, y: DeviceArray, a: float):
+        """w = x + y*a (xnew = x + p*alpha; r = r - u*alpha as a = -alpha)."""
+        self._need(x, w.n); self._need(y, w.n)
+        _check(lib().pf_vec_waxpy(self.h, w.ptr, x.ptr, y.ptr, float(a), w.n), "pf_vec_waxpy")
+
+    def vec_mul(self, z: DeviceArray, x: DeviceArray, y: DeviceArray):
+        """z = x*y (d = diag_precon*r)."""
+        self._need(x, z.n); self._need(y, z.n)
+        _check(lib().pf_vec_mul(self.h, z.ptr, x.ptr, y.ptr, z.n), "pf_vec_mul")
+
+    def vec_recip(self, y: DeviceArray, x: DeviceArray):
+        """y = 1/x (diag_precon = 1/diag_precon)."""
+        self._need(x, y.n)
+        _check(lib().pf_vec_recip(self.h, y.ptr, x.ptr, y.n), "pf_vec_recip")
+
     # ----------------------------------------------------------- the solve
     @staticmethod
     def _opts(limit, tol, stop, precision, check_every=0, refine_every=0, refine_delta=0.0):
@@ -404,12 +427,13 @@ class Context:
         _check(lib().pf_pcg_timing_enable(self.h, int(bool(on))), "pf_pcg_timing_enable")
 
     def timing_get(self):
-        """{kernel: (total ms, launches)} for ebe, scatter, update, pupdate."""
-        ms = np.zeros(4)
-        cnt = np.zeros(4, dtype=np.int32)
+        """{kernel group: (total ms, launches)} for ebe, scatter, update,
+        pupdate and refine (the mixed solver's true-residual replacement)."""
+        ms = np.zeros(5)
+        cnt = np.zeros(5, dtype=np.int32)
         _check(lib().pf_pcg_timing_get(self.h, _pd(ms), _pi(cnt)), "pf_pcg_timing_get")
         return {k: (float(ms[i]), int(cnt[i]))
-                for i, k in enumerate(("ebe", "scatter", "update", "pupdate"))}
+                for i, k in enumerate(("ebe", "scatter", "update", "pupdate", "refine"))}
 
     def apply(self, x) -> np.ndarray:
         """y = K x (FP64, with p123 fixed rows)."""

@@ -1,8 +1,10 @@
 """Build libparafem_amd.so (HIP, gfx950) in-tree.
 
 ``python -m parafem_amd.build`` compiles every ``csrc/*.hip`` with hipcc for
-gfx950 and links them into ``parafem_amd/libparafem_amd.so``.  It always
-recompiles (the caller decides when to build).
+gfx950 and links them into ``parafem_amd/libparafem_amd.so``, then builds the
+Fortran binding (``fortran/pf_gpu.f90``) and the p121 driver
+(``bin/pf_p121``) with amdflang.  It always recompiles (the caller decides
+when to build).
 """
 from __future__ import annotations
 
@@ -17,6 +19,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libparafem_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
+FSRC = os.path.join(HERE, "fortran")
+P121 = os.path.join(HERE, "bin", "pf_p121")
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -40,6 +45,28 @@ def _compile(src: str) -> str:
     return obj
 
 
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the Fortran binding module and the p121 driver against the
+    library (rpath to this directory).  Skipped, returning None, when no
+    Fortran compiler is installed."""
+    if not os.path.exists(FLANG):
+        return None
+    moddir = os.path.join(OBJDIR, "fmod")
+    os.makedirs(moddir, exist_ok=True)
+    os.makedirs(os.path.dirname(P121), exist_ok=True)
+    tmp = P121 + ".tmp"
+    cmd = [FLANG, "-O2", "-J", moddir, os.path.join(FSRC, "pf_gpu.f90"),
+           os.path.join(FSRC, "pf_p121.f90"), "-L", HERE, "-lparafem_amd",
+           "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=moddir)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, P121)
+    if verbose:
+        print("built", P121)
+    return P121
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -51,6 +78,7 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     shutil.rmtree(OBJDIR, ignore_errors=True)
     if verbose:
         print("built", LIB)

@@ -98,10 +98,19 @@ __device__ void fold_partials(double (&out)[NQ], const int (&qid)[NQ], const boo
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
+        // Thread t adds entries t, t+256, ... in that order; the loads of a
+        // batch of 16 are issued before the dependent additions.
         double s = 0.0;
-        for (int b = threadIdx.x; b < G; b += PF_BLOCK) {
-            const double pv = partials[qid[k] * MAXG + b];
-            s = ismax[k] ? fmax(s, pv) : __dadd_rn(s, pv);
+        for (int b0 = threadIdx.x; b0 < G; b0 += 16 * PF_BLOCK) {
+            double t[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int b = b0 + u * PF_BLOCK;
+                t[u] = b < G ? partials[qid[k] * MAXG + b] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (b0 + u * PF_BLOCK < G) s = ismax[k] ? fmax(s, t[u]) : __dadd_rn(s, t[u]);
         }
         s = ismax[k] ? wave_max(s) : wave_sum(s);
         if (lane == 0) ws[k][wave] = s;
@@ -140,7 +149,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
 {
     if (skip(st, gate)) return;
     using L = Lanes<NT>;
-    __shared__ double ps[PF_BLOCK / 64][64];
+    __shared__ __attribute__((aligned(16))) double ps[PF_BLOCK / 64][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sub = lane / L::W, li = lane % L::W;
     const long e = ((long)blockIdx.x * (PF_BLOCK / 64) + wave) * L::EPW + sub;
@@ -165,20 +174,24 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
         }
         if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
     } else if constexpr (KIND == KIND_F32) {
+        // fl32(p_e) staged in LDS, read back four columns per 16-byte load.
+        float *psf = reinterpret_cast<float *>(&ps[wave][0]);
         const f4_t *Kq = reinterpret_cast<const f4_t *>(Khi) + ec * (long)(NT * NT / 4) + i;
         f4_t kc[NT / 4];
 #pragma unroll
         for (int j = 0; j < NT / 4; ++j) kc[j] = __builtin_nontemporal_load(&Kq[j * NT]);
-        ps[wave][lane] = GATHER ? src[g] : src[ec * NT + i];
+        psf[lane] = (float)(GATHER ? src[g] : src[ec * NT + i]);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const f4_t *pf4 = reinterpret_cast<const f4_t *>(psf + sub * L::W);
         float acc = 0.0f;
 #pragma unroll
         for (int j = 0; j < NT / 4; ++j) {
-            acc = __fmaf_rn(kc[j].x, (float)pw[4 * j], acc);
-            acc = __fmaf_rn(kc[j].y, (float)pw[4 * j + 1], acc);
-            acc = __fmaf_rn(kc[j].z, (float)pw[4 * j + 2], acc);
-            acc = __fmaf_rn(kc[j].w, (float)pw[4 * j + 3], acc);
+            const f4_t pj = pf4[j];
+            acc = __fmaf_rn(kc[j].x, pj.x, acc);
+            acc = __fmaf_rn(kc[j].y, pj.y, acc);
+            acc = __fmaf_rn(kc[j].z, pj.z, acc);
+            acc = __fmaf_rn(kc[j].w, pj.w, acc);
         }
         if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
     } else {
@@ -246,7 +259,9 @@ __global__ __launch_bounds__(PF_BLOCK) void k_fold_alpha(const double *partials,
     }
 }
 
-// x/r/d updates and the partial (r,d), (r,r), max|xnew|, max|xnew-x|.
+// x/r/d updates and the partial (r,d) with, for checon_par, max|xnew| and
+// max|xnew-x|, or, for the residual rule, (r,r).
+template <bool CHECON>
 __global__ __launch_bounds__(PF_BLOCK) void k_update(double *__restrict__ x, double *__restrict__ r,
                                                     double *__restrict__ d, const double *__restrict__ p,
                                                     const double *__restrict__ u, const double *__restrict__ dinv,
@@ -254,7 +269,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_update(double *__restrict__ x, dou
 {
     if (skip(st, RUN_LIVE)) return;
     const double alpha = st->alpha;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    constexpr int NQ = CHECON ? 3 : 2;
+    double v[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) v[k] = 0.0;
     const int stride = gridDim.x * PF_BLOCK;
     for (int q = blockIdx.x * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
         const double xo = x[q];
@@ -265,13 +283,22 @@ __global__ __launch_bounds__(PF_BLOCK) void k_update(double *__restrict__ x, dou
         r[q] = rq;
         d[q] = dq;
         v[0] = __dadd_rn(v[0], __dmul_rn(rq, dq));
-        v[1] = __dadd_rn(v[1], __dmul_rn(rq, rq));
-        v[2] = fmax(v[2], fabs(xn));
-        v[3] = fmax(v[3], fabs(__dsub_rn(xn, xo)));
+        if constexpr (CHECON) {
+            v[1] = fmax(v[1], fabs(xn));
+            v[2] = fmax(v[2], fabs(__dsub_rn(xn, xo)));
+        } else {
+            v[1] = __dadd_rn(v[1], __dmul_rn(rq, rq));
+        }
     }
-    constexpr int qid[4] = {PF_Q_RD, PF_Q_RR, PF_Q_MAXX, PF_Q_MAXDX};
-    constexpr bool mx[4] = {false, false, true, true};
-    block_to_partials<4>(v, qid, mx, partials);
+    if constexpr (CHECON) {
+        constexpr int qid[3] = {PF_Q_RD, PF_Q_MAXX, PF_Q_MAXDX};
+        constexpr bool mx[3] = {false, true, true};
+        block_to_partials<3>(v, qid, mx, partials);
+    } else {
+        constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
+        constexpr bool mx[2] = {false, false};
+        block_to_partials<2>(v, qid, mx, partials);
+    }
 }
 
 // End of an iteration: beta = rd/up, the stopping rule, the count.
@@ -289,13 +316,27 @@ __device__ void finish_iteration(pf_state *st, double rd, double rr, double maxx
     if (st->converged || st->iters >= st->limit || st->breakdown) st->frozen = 1;
 }
 
+template <bool CHECON>
 __global__ __launch_bounds__(PF_BLOCK) void k_fold_update(const double *partials, int G, pf_state *st)
 {
     if (skip(st, RUN_LIVE)) return;
-    constexpr int qid[4] = {PF_Q_RD, PF_Q_RR, PF_Q_MAXX, PF_Q_MAXDX};
-    constexpr bool mx[4] = {false, false, true, true};
-    double out[4];
-    fold_partials<4>(out, qid, mx, partials, G);
+    double out[4] = {0.0, 0.0, 0.0, 0.0};   // rd, rr, max|xnew|, max|xnew-x|
+    if constexpr (CHECON) {
+        constexpr int qid[3] = {PF_Q_RD, PF_Q_MAXX, PF_Q_MAXDX};
+        constexpr bool mx[3] = {false, true, true};
+        double o[3];
+        fold_partials<3>(o, qid, mx, partials, G);
+        out[0] = o[0];
+        out[2] = o[1];
+        out[3] = o[2];
+    } else {
+        constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
+        constexpr bool mx[2] = {false, false};
+        double o[2];
+        fold_partials<2>(o, qid, mx, partials, G);
+        out[0] = o[0];
+        out[1] = o[1];
+    }
     if (threadIdx.x == 0) {
         if (st->precision == PF_PREC_MIXED) {
             // Reliable update: replace the recurrence residual by the true
@@ -427,6 +468,19 @@ __global__ __launch_bounds__(256) void k_copy(const u4_t *__restrict__ src, u4_t
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+// Whole-array vector statements: 0 fill, 1 w = x + y*a, 2 z = x*y, 3 y = 1/x.
+__global__ __launch_bounds__(256) void k_vec(int op, double *w, const double *x, const double *y, double a, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        switch (op) {
+        case 0: w[i] = a; break;
+        case 1: w[i] = __dadd_rn(x[i], __dmul_rn(y[i], a)); break;
+        case 2: w[i] = __dmul_rn(x[i], y[i]); break;
+        default: w[i] = 1.0 / x[i]; break;
+        }
+    }
 }
 
 __global__ void k_gather(const int *__restrict__ gg, const double *__restrict__ p, double *__restrict__ pmul, long n)
@@ -573,6 +627,26 @@ int pf_dev_copy(pf_ctx *c, void *dst, const void *src, size_t bytes)
     return PF_OK;
 }
 
+static int vec_op(pf_ctx *c, int op, double *w, const double *x, const double *y, double a, int64_t n)
+{
+    if (!c || n < 0 || (n && !w)) return PF_ERR_ARG;
+    if (n && op != 0 && !x) return PF_ERR_ARG;
+    if (n && (op == 1 || op == 2) && !y) return PF_ERR_ARG;
+    if (!n) return PF_OK;
+    int grid = (int)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_vec, dim3(grid), dim3(256), 0, c->stream, op, w, x, y, a, (long)n);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+int pf_vec_fill(pf_ctx *c, double *x, double value, int64_t n) { return vec_op(c, 0, x, nullptr, nullptr, value, n); }
+int pf_vec_waxpy(pf_ctx *c, double *w, const double *x, const double *y, double a, int64_t n)
+{
+    return vec_op(c, 1, w, x, y, a, n);
+}
+int pf_vec_mul(pf_ctx *c, double *z, const double *x, const double *y, int64_t n) { return vec_op(c, 2, z, x, y, 1.0, n); }
+int pf_vec_recip(pf_ctx *c, double *y, const double *x, int64_t n) { return vec_op(c, 3, y, x, nullptr, 1.0, n); }
+
 int pf_gather(pf_ctx *c, const double *p, double *pmul)
 {
     if (!c || !p || !pmul) return PF_ERR_ARG;
@@ -706,20 +780,25 @@ int pf_pcg_run(pf_ctx *c, int n)
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         tm = pf_timing_begin(c, PF_K_UPDATE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
-        hipLaunchKernelGGL(k_update, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, c->u, c->dinv,
-                           c->neq, c->partials, c->state);
-        hipLaunchKernelGGL(k_fold_update, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
+        if (c->opts.stop_mode == PF_STOP_CHECON) {
+            hipLaunchKernelGGL(k_update<true>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, c->u,
+                               c->dinv, c->neq, c->partials, c->state);
+            hipLaunchKernelGGL(k_fold_update<true>, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G,
+                               c->state);
+        } else {
+            hipLaunchKernelGGL(k_update<false>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, c->u,
+                               c->dinv, c->neq, c->partials, c->state);
+            hipLaunchKernelGGL(k_fold_update<false>, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G,
+                               c->state);
+        }
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         if (mixed) {
             // True-residual replacement; the kernels do nothing unless the
             // fold above flagged this iteration.
-            tm = pf_timing_begin(c, PF_K_EBE, &t0, &t1);
+            tm = pf_timing_begin(c, PF_K_REFINE, &t0, &t1);
             if (tm) PF_HIP(hipEventRecord(t0, c->stream));
             rc = launch_ebe<KIND_ACC, true, double>(c, c->x, c->utemp, c->state, RUN_REFINE);
             if (rc) return rc;
-            if (tm) PF_HIP(hipEventRecord(t1, c->stream));
-            tm = pf_timing_begin(c, PF_K_SCATTER, &t0, &t1);
-            if (tm) PF_HIP(hipEventRecord(t0, c->stream));
             hipLaunchKernelGGL(k_scatter_res, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, c->csr_slot,
                                c->utemp, c->fixstore, c->x, c->f, c->r, c->d, c->dinv, c->neq, c->partials,
                                c->state);
@@ -745,7 +824,7 @@ int pf_pcg_timing_enable(pf_ctx *c, int on)
     return PF_OK;
 }
 
-int pf_pcg_timing_get(pf_ctx *c, double ms[4], int counts[4])
+int pf_pcg_timing_get(pf_ctx *c, double ms[PF_TIMING_SLOTS], int counts[PF_TIMING_SLOTS])
 {
     if (!c || !ms || !counts) return PF_ERR_ARG;
     PF_HIP(hipStreamSynchronize(c->stream));

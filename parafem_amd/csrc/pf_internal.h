@@ -16,9 +16,9 @@
 
 // Kernels of one PCG iteration, in launch order (pf_pcg_timing_get indices):
 // element products, scatter + p.u (and its fold), x/r/d update (and its
-// fold), p update.  True-residual replacements of the mixed solver are
-// counted with the element products and the scatter.
-enum { PF_K_EBE = 0, PF_K_SCATTER = 1, PF_K_UPDATE = 2, PF_K_PUPDATE = 3, PF_NKERNELS = 4 };
+// fold), p update, and the mixed solver's true-residual replacement (all of
+// its launches, which are no-ops on iterations that do not replace r).
+enum { PF_K_EBE = 0, PF_K_SCATTER = 1, PF_K_UPDATE = 2, PF_K_PUPDATE = 3, PF_K_REFINE = 4, PF_NKERNELS = 5 };
 
 // Per-block partial quantities kept in the partials buffer.
 enum { PF_Q_PU = 0, PF_Q_RD = 1, PF_Q_RR = 2, PF_Q_MAXX = 3, PF_Q_MAXDX = 4, PF_NQ = 5 };

@@ -1,0 +1,436 @@
+!------------------------------------------------------------------------------
+! pf_gpu: ParaFEM's EBE-PCG call surface on an MI355X, through ISO_C_BINDING
+! over libparafem_amd (include/parafem_amd.h).
+!
+! The routine names and argument meaning follow the reference's library so a
+! p121/p123-style driver keeps its loop:
+!   GATHER(p_pp,pmul_pp), SCATTER(u_pp,utemp_pp), SCATTER_NOADD(utemp_pp,u_pp)
+!     src/modules/mpi/gather_scatter.f90:547-993
+!   DOT_PRODUCT_P(a,b), SUM_P(a), checon_par(loads,tol,converged,oldlds)
+!     src/modules/mpi/maths.f90:168-216, 271-315, 999-1069
+! Vectors and element arrays live in device memory behind TYPE(pf_dvec)
+! handles instead of host arrays; the one device context is module state, as
+! the reference keeps its halo tables as module state.
+!------------------------------------------------------------------------------
+MODULE pf_gpu
+  USE, INTRINSIC :: ISO_C_BINDING
+  IMPLICIT NONE
+  PRIVATE
+
+  INTEGER, PARAMETER, PUBLIC :: iwp = SELECTED_REAL_KIND(15, 300)
+  INTEGER(c_int), PARAMETER, PUBLIC :: PF_OK = 0, PF_NOT_CONVERGED = 5
+  INTEGER(c_int), PARAMETER, PUBLIC :: PF_STORE_FP64 = 0, PF_STORE_FP32 = 1
+  INTEGER(c_int), PARAMETER, PUBLIC :: PF_STOP_CHECON = 0, PF_STOP_RELRES = 1
+  INTEGER(c_int), PARAMETER, PUBLIC :: PF_PREC_FP64 = 0, PF_PREC_MIXED = 1
+
+  TYPE, BIND(C), PUBLIC :: pf_pcg_opts
+    INTEGER(c_int) :: limit
+    REAL(c_double) :: tol
+    INTEGER(c_int) :: stop_mode
+    INTEGER(c_int) :: precision
+    INTEGER(c_int) :: check_every
+    INTEGER(c_int) :: refine_every
+    REAL(c_double) :: refine_delta
+  END TYPE pf_pcg_opts
+
+  TYPE, BIND(C), PUBLIC :: pf_pcg_result
+    INTEGER(c_int) :: iters
+    INTEGER(c_int) :: converged
+    REAL(c_double) :: crit
+    REAL(c_double) :: rr0
+    REAL(c_double) :: rr
+    INTEGER(c_int) :: refinements
+    INTEGER(c_int) :: status
+  END TYPE pf_pcg_result
+
+  ! A device vector of n REAL(iwp).
+  TYPE, PUBLIC :: pf_dvec
+    TYPE(c_ptr) :: ptr = c_null_ptr
+    INTEGER(c_int64_t) :: n = 0
+  END TYPE pf_dvec
+
+  TYPE(c_ptr), SAVE :: ctx = c_null_ptr
+
+  PUBLIC :: pf_init, pf_finalize, pf_alloc, pf_release, pf_put, pf_get
+  PUBLIC :: pf_mesh, pf_steer, pf_assemble_elastic, pf_assemble_laplace
+  PUBLIC :: pf_storkm, pf_diag, pf_fixed, pf_pcg, pf_check
+  PUBLIC :: gather, scatter, scatter_noadd, elem_matvec
+  PUBLIC :: dot_product_p, sum_p, checon_par
+  PUBLIC :: vec_fill, vec_waxpy, vec_mul, vec_recip
+
+  INTERFACE
+    INTEGER(c_int) FUNCTION c_create(device, c) BIND(C, name="pf_create")
+      IMPORT :: c_int, c_ptr
+      INTEGER(c_int), VALUE :: device
+      TYPE(c_ptr) :: c
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_destroy(c) BIND(C, name="pf_destroy")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: c
+    END FUNCTION
+    FUNCTION c_status_string(s) BIND(C, name="pf_status_string")
+      IMPORT :: c_int, c_ptr
+      INTEGER(c_int), VALUE :: s
+      TYPE(c_ptr) :: c_status_string
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_dev_alloc(c, bytes, p) BIND(C, name="pf_dev_alloc")
+      IMPORT :: c_int, c_ptr, c_size_t
+      TYPE(c_ptr), VALUE :: c
+      INTEGER(c_size_t), VALUE :: bytes
+      TYPE(c_ptr) :: p
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_dev_free(c, p) BIND(C, name="pf_dev_free")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: c, p
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_h2d(c, dst, src, bytes) BIND(C, name="pf_memcpy_h2d")
+      IMPORT :: c_int, c_ptr, c_size_t
+      TYPE(c_ptr), VALUE :: c, dst, src
+      INTEGER(c_size_t), VALUE :: bytes
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_d2h(c, dst, src, bytes) BIND(C, name="pf_memcpy_d2h")
+      IMPORT :: c_int, c_ptr, c_size_t
+      TYPE(c_ptr), VALUE :: c, dst, src
+      INTEGER(c_size_t), VALUE :: bytes
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_mesh_set(c, nod, nels, g_coord) BIND(C, name="pf_mesh_set")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      INTEGER(c_int), VALUE :: nod, nels
+      REAL(c_double), INTENT(IN) :: g_coord(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_steer_set(c, ntot, nels, neq, g_g) BIND(C, name="pf_steer_set")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: c
+      INTEGER(c_int), VALUE :: ntot, nels, neq
+      INTEGER(c_int), INTENT(IN) :: g_g(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_assemble_elastic(c, e, v, e_elem, store) BIND(C, name="pf_assemble_elastic")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), VALUE :: e, v
+      TYPE(c_ptr), VALUE :: e_elem
+      INTEGER(c_int), VALUE :: store
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_assemble_laplace(c, kx, ky, kz, k_elem, store) BIND(C, name="pf_assemble_laplace")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), VALUE :: kx, ky, kz
+      TYPE(c_ptr), VALUE :: k_elem
+      INTEGER(c_int), VALUE :: store
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_storkm_set(c, storkm, store) BIND(C, name="pf_storkm_set")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), INTENT(IN) :: storkm(*)
+      INTEGER(c_int), VALUE :: store
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_diag_get(c, diag) BIND(C, name="pf_diag_get")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), INTENT(OUT) :: diag(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_fixed_set(c, nfix, no_f, penalty, store) BIND(C, name="pf_fixed_set")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      INTEGER(c_int), VALUE :: nfix
+      INTEGER(c_int), INTENT(IN) :: no_f(*)
+      REAL(c_double), VALUE :: penalty
+      REAL(c_double), INTENT(OUT) :: store(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_gather(c, p, pmul) BIND(C, name="pf_gather")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: c, p, pmul
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_elem_matvec(c, pmul, utemp) BIND(C, name="pf_elem_matvec")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: c, pmul, utemp
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_scatter(c, u, utemp) BIND(C, name="pf_scatter")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: c, u, utemp
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_scatter_noadd(c, utemp, u) BIND(C, name="pf_scatter_noadd")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: c, utemp, u
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_dot(c, a, b, n, out) BIND(C, name="pf_dot_product_p")
+      IMPORT :: c_int, c_ptr, c_int64_t, c_double
+      TYPE(c_ptr), VALUE :: c, a, b
+      INTEGER(c_int64_t), VALUE :: n
+      REAL(c_double) :: out
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_sum(c, a, n, out) BIND(C, name="pf_sum_p")
+      IMPORT :: c_int, c_ptr, c_int64_t, c_double
+      TYPE(c_ptr), VALUE :: c, a
+      INTEGER(c_int64_t), VALUE :: n
+      REAL(c_double) :: out
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_checon(c, xnew, tol, conv, x, n) BIND(C, name="pf_checon_par")
+      IMPORT :: c_int, c_ptr, c_int64_t, c_double
+      TYPE(c_ptr), VALUE :: c, xnew, x
+      REAL(c_double), VALUE :: tol
+      INTEGER(c_int) :: conv
+      INTEGER(c_int64_t), VALUE :: n
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_vec_fill(c, x, val, n) BIND(C, name="pf_vec_fill")
+      IMPORT :: c_int, c_ptr, c_int64_t, c_double
+      TYPE(c_ptr), VALUE :: c, x
+      REAL(c_double), VALUE :: val
+      INTEGER(c_int64_t), VALUE :: n
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_vec_waxpy(c, w, x, y, a, n) BIND(C, name="pf_vec_waxpy")
+      IMPORT :: c_int, c_ptr, c_int64_t, c_double
+      TYPE(c_ptr), VALUE :: c, w, x, y
+      REAL(c_double), VALUE :: a
+      INTEGER(c_int64_t), VALUE :: n
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_vec_mul(c, z, x, y, n) BIND(C, name="pf_vec_mul")
+      IMPORT :: c_int, c_ptr, c_int64_t
+      TYPE(c_ptr), VALUE :: c, z, x, y
+      INTEGER(c_int64_t), VALUE :: n
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_vec_recip(c, y, x, n) BIND(C, name="pf_vec_recip")
+      IMPORT :: c_int, c_ptr, c_int64_t
+      TYPE(c_ptr), VALUE :: c, y, x
+      INTEGER(c_int64_t), VALUE :: n
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_pcg_solve(c, r0, x, opts, res) BIND(C, name="pf_pcg_solve")
+      IMPORT :: c_int, c_ptr, c_double, pf_pcg_opts, pf_pcg_result
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), INTENT(IN) :: r0(*)
+      REAL(c_double), INTENT(OUT) :: x(*)
+      TYPE(pf_pcg_opts), INTENT(IN) :: opts
+      TYPE(pf_pcg_result), INTENT(OUT) :: res
+    END FUNCTION
+  END INTERFACE
+
+CONTAINS
+
+  !> Stop with the library's message on a failed call (the reference's fatal
+  !> paths STOP as well, e.g. gather_scatter.f90:804-816).
+  SUBROUTINE pf_check(rc, what)
+    INTEGER(c_int), INTENT(IN) :: rc
+    CHARACTER(*), INTENT(IN) :: what
+    CHARACTER(KIND=c_char), POINTER :: msg(:)
+    INTEGER :: i
+    CHARACTER(LEN=256) :: text
+    IF (rc == PF_OK) RETURN
+    CALL c_f_pointer(c_status_string(rc), msg, [256])
+    text = ' '
+    DO i = 1, 256
+      IF (msg(i) == c_null_char) EXIT
+      text(i:i) = msg(i)
+    END DO
+    WRITE(*, '(A,A,A,I0,A,A)') 'pf_gpu: ', what, ' failed, status ', rc, ': ', TRIM(text)
+    STOP 1
+  END SUBROUTINE pf_check
+
+  !> FIND_PE_PROCS counterpart: one context on one GPU.
+  SUBROUTINE pf_init(device)
+    INTEGER, INTENT(IN) :: device
+    CALL pf_check(c_create(INT(device, c_int), ctx), 'pf_create')
+  END SUBROUTINE pf_init
+
+  !> SHUTDOWN counterpart.
+  SUBROUTINE pf_finalize()
+    INTEGER(c_int) :: rc
+    rc = c_destroy(ctx)
+    ctx = c_null_ptr
+  END SUBROUTINE pf_finalize
+
+  SUBROUTINE pf_alloc(v, n)
+    TYPE(pf_dvec), INTENT(INOUT) :: v
+    INTEGER, INTENT(IN) :: n
+    v%n = INT(n, c_int64_t)
+    CALL pf_check(c_dev_alloc(ctx, INT(MAX(n, 1), c_size_t) * 8_c_size_t, v%ptr), 'pf_dev_alloc')
+  END SUBROUTINE pf_alloc
+
+  SUBROUTINE pf_release(v)
+    TYPE(pf_dvec), INTENT(INOUT) :: v
+    INTEGER(c_int) :: rc
+    IF (c_associated(v%ptr)) rc = c_dev_free(ctx, v%ptr)
+    v%ptr = c_null_ptr
+    v%n = 0
+  END SUBROUTINE pf_release
+
+  !> Host array -> device vector (sizes must match).
+  SUBROUTINE pf_put(v, a)
+    TYPE(pf_dvec), INTENT(INOUT) :: v
+    REAL(iwp), INTENT(IN), TARGET, CONTIGUOUS :: a(:)
+    IF (INT(SIZE(a), c_int64_t) /= v%n) CALL pf_check(1_c_int, 'pf_put size')
+    CALL pf_check(c_h2d(ctx, v%ptr, c_loc(a), INT(v%n, c_size_t) * 8_c_size_t), 'pf_memcpy_h2d')
+  END SUBROUTINE pf_put
+
+  !> Device vector -> host array.
+  SUBROUTINE pf_get(v, a)
+    TYPE(pf_dvec), INTENT(IN) :: v
+    REAL(iwp), INTENT(INOUT), TARGET, CONTIGUOUS :: a(:)
+    IF (INT(SIZE(a), c_int64_t) /= v%n) CALL pf_check(1_c_int, 'pf_get size')
+    CALL pf_check(c_d2h(ctx, c_loc(a), v%ptr, INT(v%n, c_size_t) * 8_c_size_t), 'pf_memcpy_d2h')
+  END SUBROUTINE pf_get
+
+  !> g_coord_pp(nod,3,nels) (READ_G_COORD_PP's array) to the device.
+  SUBROUTINE pf_mesh(g_coord_pp)
+    REAL(iwp), INTENT(IN) :: g_coord_pp(:, :, :)
+    CALL pf_check(c_mesh_set(ctx, INT(SIZE(g_coord_pp, 1), c_int), INT(SIZE(g_coord_pp, 3), c_int), &
+                             g_coord_pp), 'pf_mesh_set')
+  END SUBROUTINE pf_mesh
+
+  !> g_g_pp(ntot,nels) (find_g3/find_g4's array) and neq to the device.
+  SUBROUTINE pf_steer(g_g_pp, neq)
+    INTEGER, INTENT(IN) :: g_g_pp(:, :), neq
+    CALL pf_check(c_steer_set(ctx, INT(SIZE(g_g_pp, 1), c_int), INT(SIZE(g_g_pp, 2), c_int), &
+                              INT(neq, c_int), g_g_pp), 'pf_steer_set')
+  END SUBROUTINE pf_steer
+
+  !> p121's element loop (p121.f90:54-64) on the device; e_elem optional.
+  SUBROUTINE pf_assemble_elastic(e, v, store, e_elem)
+    REAL(iwp), INTENT(IN) :: e, v
+    INTEGER, INTENT(IN) :: store
+    REAL(iwp), INTENT(IN), TARGET, OPTIONAL :: e_elem(:)
+    TYPE(c_ptr) :: pe
+    pe = c_null_ptr
+    IF (PRESENT(e_elem)) pe = c_loc(e_elem)
+    CALL pf_check(c_assemble_elastic(ctx, e, v, pe, INT(store, c_int)), 'pf_assemble_elastic')
+  END SUBROUTINE pf_assemble_elastic
+
+  !> p123's element loop (p123.f90:70-84) on the device.
+  SUBROUTINE pf_assemble_laplace(kx, ky, kz, store, k_elem)
+    REAL(iwp), INTENT(IN) :: kx, ky, kz
+    INTEGER, INTENT(IN) :: store
+    REAL(iwp), INTENT(IN), TARGET, OPTIONAL :: k_elem(:)
+    TYPE(c_ptr) :: pk
+    pk = c_null_ptr
+    IF (PRESENT(k_elem)) pk = c_loc(k_elem)
+    CALL pf_check(c_assemble_laplace(ctx, kx, ky, kz, pk, INT(store, c_int)), 'pf_assemble_laplace')
+  END SUBROUTINE pf_assemble_laplace
+
+  !> A driver's own storkm_pp(ntot,ntot,nels) to the device.
+  SUBROUTINE pf_storkm(storkm_pp, store)
+    REAL(iwp), INTENT(IN) :: storkm_pp(:, :, :)
+    INTEGER, INTENT(IN) :: store
+    CALL pf_check(c_storkm_set(ctx, storkm_pp, INT(store, c_int)), 'pf_storkm_set')
+  END SUBROUTINE pf_storkm
+
+  !> Assembled diagonal (diag_precon before inversion, p121.f90:65-69).
+  SUBROUTINE pf_diag(diag)
+    REAL(iwp), INTENT(OUT) :: diag(:)
+    CALL pf_check(c_diag_get(ctx, diag), 'pf_diag_get')
+  END SUBROUTINE pf_diag
+
+  !> p123 penalty rows (p123.f90:120-125); returns store_pp.
+  SUBROUTINE pf_fixed(no_f, penalty, store)
+    INTEGER, INTENT(IN) :: no_f(:)
+    REAL(iwp), INTENT(IN) :: penalty
+    REAL(iwp), INTENT(OUT) :: store(:)
+    CALL pf_check(c_fixed_set(ctx, INT(SIZE(no_f), c_int), no_f, penalty, store), 'pf_fixed_set')
+  END SUBROUTINE pf_fixed
+
+  !> GATHER(p_pp, pmul_pp): pmul(:,e) = p(g_g(:,e)), 0 for restrained.
+  SUBROUTINE gather(p_pp, pmul_pp)
+    TYPE(pf_dvec), INTENT(IN) :: p_pp
+    TYPE(pf_dvec), INTENT(INOUT) :: pmul_pp
+    CALL pf_check(c_gather(ctx, p_pp%ptr, pmul_pp%ptr), 'GATHER')
+  END SUBROUTINE gather
+
+  !> utemp(:,e) = MATMUL(storkm(:,:,e), pmul(:,e)) for every element.
+  SUBROUTINE elem_matvec(pmul_pp, utemp_pp)
+    TYPE(pf_dvec), INTENT(IN) :: pmul_pp
+    TYPE(pf_dvec), INTENT(INOUT) :: utemp_pp
+    CALL pf_check(c_elem_matvec(ctx, pmul_pp%ptr, utemp_pp%ptr), 'element products')
+  END SUBROUTINE elem_matvec
+
+  !> SCATTER(u_pp, utemp_pp): u accumulates, ascending element order.
+  SUBROUTINE scatter(u_pp, utemp_pp)
+    TYPE(pf_dvec), INTENT(INOUT) :: u_pp
+    TYPE(pf_dvec), INTENT(IN) :: utemp_pp
+    CALL pf_check(c_scatter(ctx, u_pp%ptr, utemp_pp%ptr), 'SCATTER')
+  END SUBROUTINE scatter
+
+  !> SCATTER_NOADD(utemp_pp, u_pp): assignment, the last element wins.
+  SUBROUTINE scatter_noadd(utemp_pp, u_pp)
+    TYPE(pf_dvec), INTENT(IN) :: utemp_pp
+    TYPE(pf_dvec), INTENT(INOUT) :: u_pp
+    CALL pf_check(c_scatter_noadd(ctx, utemp_pp%ptr, u_pp%ptr), 'SCATTER_NOADD')
+  END SUBROUTINE scatter_noadd
+
+  FUNCTION dot_product_p(vectora, vectorb) RESULT(s)
+    TYPE(pf_dvec), INTENT(IN) :: vectora, vectorb
+    REAL(iwp) :: s
+    REAL(c_double) :: out
+    CALL pf_check(c_dot(ctx, vectora%ptr, vectorb%ptr, vectora%n, out), 'DOT_PRODUCT_P')
+    s = out
+  END FUNCTION dot_product_p
+
+  FUNCTION sum_p(vectora) RESULT(s)
+    TYPE(pf_dvec), INTENT(IN) :: vectora
+    REAL(iwp) :: s
+    REAL(c_double) :: out
+    CALL pf_check(c_sum(ctx, vectora%ptr, vectora%n, out), 'SUM_P')
+    s = out
+  END FUNCTION sum_p
+
+  !> checon_par(loads, tol, converged, oldlds): oldlds = loads on return.
+  SUBROUTINE checon_par(loads, tol, converged, oldlds)
+    TYPE(pf_dvec), INTENT(IN) :: loads
+    REAL(iwp), INTENT(IN) :: tol
+    LOGICAL, INTENT(OUT) :: converged
+    TYPE(pf_dvec), INTENT(INOUT) :: oldlds
+    INTEGER(c_int) :: conv
+    CALL pf_check(c_checon(ctx, loads%ptr, tol, conv, oldlds%ptr, loads%n), 'checon_par')
+    converged = conv /= 0
+  END SUBROUTINE checon_par
+
+  SUBROUTINE vec_fill(x, val)
+    TYPE(pf_dvec), INTENT(INOUT) :: x
+    REAL(iwp), INTENT(IN) :: val
+    CALL pf_check(c_vec_fill(ctx, x%ptr, val, x%n), 'vec_fill')
+  END SUBROUTINE vec_fill
+
+  !> w = x + y*a
+  SUBROUTINE vec_waxpy(w, x, y, a)
+    TYPE(pf_dvec), INTENT(INOUT) :: w
+    TYPE(pf_dvec), INTENT(IN) :: x, y
+    REAL(iwp), INTENT(IN) :: a
+    CALL pf_check(c_vec_waxpy(ctx, w%ptr, x%ptr, y%ptr, a, w%n), 'vec_waxpy')
+  END SUBROUTINE vec_waxpy
+
+  !> z = x*y
+  SUBROUTINE vec_mul(z, x, y)
+    TYPE(pf_dvec), INTENT(INOUT) :: z
+    TYPE(pf_dvec), INTENT(IN) :: x, y
+    CALL pf_check(c_vec_mul(ctx, z%ptr, x%ptr, y%ptr, z%n), 'vec_mul')
+  END SUBROUTINE vec_mul
+
+  !> y = 1/x
+  SUBROUTINE vec_recip(y, x)
+    TYPE(pf_dvec), INTENT(INOUT) :: y
+    TYPE(pf_dvec), INTENT(IN) :: x
+    CALL pf_check(c_vec_recip(ctx, y%ptr, x%ptr, y%n), 'vec_recip')
+  END SUBROUTINE vec_recip
+
+  !> The whole PCG loop on the device (pf_pcg_solve).
+  SUBROUTINE pf_pcg(r0, x, limit, tol, stop_mode, precision, iters, converged)
+    REAL(iwp), INTENT(IN) :: r0(:), tol
+    REAL(iwp), INTENT(OUT) :: x(:)
+    INTEGER, INTENT(IN) :: limit, stop_mode, precision
+    INTEGER, INTENT(OUT) :: iters
+    LOGICAL, INTENT(OUT) :: converged
+    TYPE(pf_pcg_opts) :: o
+    TYPE(pf_pcg_result) :: res
+    INTEGER(c_int) :: rc
+    o%limit = limit
+    o%tol = tol
+    o%stop_mode = stop_mode
+    o%precision = precision
+    o%check_every = 0
+    o%refine_every = 0
+    o%refine_delta = 0.0_iwp
+    rc = c_pcg_solve(ctx, r0, x, o, res)
+    IF (rc /= PF_OK .AND. rc /= PF_NOT_CONVERGED) CALL pf_check(rc, 'pf_pcg_solve')
+    iters = res%iters
+    converged = res%converged /= 0
+  END SUBROUTINE pf_pcg
+
+END MODULE pf_gpu

@@ -1,0 +1,104 @@
+"""The Fortran binding (parafem_amd/fortran/pf_gpu.f90) and the p121 driver
+built on it (parafem_amd/bin/pf_p121).
+
+The driver reads a p121 deck and solves it either with p121's own loop
+statement for statement (GATHER / element products / SCATTER /
+DOT_PRODUCT_P / checon_par and the whole-array statements on device vectors,
+p121.f90:86-104), or with the fused library PCG.  Bars: the two give
+bit-identical displacements and iteration counts, equal to the oracle in the
+device's reduction order; on xx3-tiny the iteration count and the printed
+job data equal the reference's own results file (xx3-tiny.res).
+"""
+import os
+import re
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from parafem_amd import mesh as M, problems as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "parafem_amd", "bin", "pf_p121")
+
+needs_exe = pytest.mark.skipif(not os.path.exists(EXE), reason="pf_p121 not built (no amdflang)")
+
+
+def say(msg):
+    print(f"[fortran-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def run(deck, mode, *extra):
+    r = subprocess.run([EXE, os.path.basename(deck), mode, *extra], cwd=os.path.dirname(deck),
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, f"pf_p121 {mode} failed ({r.returncode}):\n{r.stdout}\n{r.stderr}"
+    res = open(deck + ".pf.res").read()
+    x = np.fromfile(deck + ".pf.x", dtype=np.float64)
+    iters = int(re.search(r"convergence was\s+(\d+)", res).group(1))
+    return x, iters, res
+
+
+@needs_exe
+def test_driver_usage_without_arguments():
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stdout
+
+
+@needs_exe
+def test_driver_rejects_unknown_mode(tmp_path):
+    r = subprocess.run([EXE, "deck", "bogus"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "unknown mode" in r.stdout
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_fortran_dropin_equals_fused_xx3_tiny(tmp_path, golden):
+    import oracle as O
+    src = os.path.join(golden, "xx3_tiny", "xx3-tiny")
+    for ext in ("dat", "d", "bnd", "lds"):
+        shutil.copy(f"{src}.{ext}", tmp_path)
+    deck = str(tmp_path / "xx3-tiny")
+    say("pf_p121 dropin xx3-tiny")
+    xa, ia, res = run(deck, "dropin")
+    say("pf_p121 fused xx3-tiny")
+    xb, ib, _ = run(deck, "fused")
+    assert ia == ib == 79
+    assert np.array_equal(xa, xb)
+    ref = open(src + ".res").read()
+    neq = int(re.search(r"equations solved\s+(\d+)", ref).group(1))
+    assert re.search(rf"restrained and\s+{neq} equations", res)
+    assert "The total load is: -0.1000E+03" in res
+    d = M.read_deck(src)
+    nf, neq2 = M.steering(d["nn"], 3, d["rest"])
+    g_g = M.g_g_from(nf, d["g_num"])
+    gc = M.element_coords(d["g_num"], d["coords"])
+    r = M.nodal_loads_to_rhs(nf, d["load_node"], d["load_val"], neq2)
+    Ko = O.assemble_elastic(gc, d["e"], d["v"])
+    xo, it, conv = O.pcg(g_g, Ko, r, O.diag(g_g, Ko, neq2), d["tol"], d["limit"],
+                         dot_order=O.DOT_DEVICE)
+    assert neq2 == neq == xa.size and it == ia
+    assert np.array_equal(xa, xo)
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_fortran_generated_abaqus_deck_and_mixed(tmp_path):
+    """A 6^3 hex20 cube deck in Abaqus node order (ABAQUS2SG applied on
+    read): dropin and fused bit-identical; the mixed-precision solve within
+    1e-8 of the FP64 solution."""
+    pr = P.elastic_cube(6)
+    rest = M.base_fixed_rest(pr.mesh)
+    nodes, vals = M.top_pressure_loads(pr.mesh, -1.0)
+    deck = str(tmp_path / "cube6")
+    M.write_p121_deck(deck, pr.mesh, rest, nodes, vals, 100.0, 0.3, 1e-9, 2000)
+    say("pf_p121 dropin cube6")
+    xa, ia, _ = run(deck, "dropin")
+    say("pf_p121 fused cube6")
+    xb, ib, _ = run(deck, "fused")
+    assert ia == ib and np.array_equal(xa, xb)
+    say("pf_p121 mixed cube6")
+    xm, im, _ = run(deck, "mixed", "1e-11")
+    assert np.abs(xm - xa).max() <= 1e-8 * np.abs(xa).max()
