@@ -8,7 +8,8 @@ pressure on the top face, per-element Young's modulus log-normal
 matrices assembled on the device.
 
 Headline (value): iterations/s of the mixed-precision Jacobi EBE-PCG --
-element matrices stored as FP32 hi/lo, element products from hi in FP32,
+element matrices stored as FP32 hi/lo in a packed symmetric layout, element
+products from hi in FP32,
 vectors, dots and recurrences in FP64, reliable updates with the FP64
 (hi+lo) residual.  A step is one PCG iteration (gather -> element products
 -> scatter -> dots -> recurrences -> stopping test) with every input
@@ -42,8 +43,11 @@ def log(msg):
 
 def ebe_bytes(ntot, store):
     """Algorithmic HBM bytes of one element product: the element matrix once
-    (FP64, or the FP32 hi part) and the int32 steering once."""
-    return ntot * ntot * (8 if store == "fp64" else 4) + ntot * 4
+    (full FP64, or the FP32 hi part packed symmetric: ntot/2 + 1 entries per
+    row) and the int32 steering once."""
+    if store == "fp64":
+        return ntot * ntot * 8 + ntot * 4
+    return (ntot // 2 + 1) * ntot * 4 + ntot * 4
 
 
 def iteration_bytes(nels, ntot, neq, store):
@@ -221,7 +225,7 @@ def main():
         "config": {
             "workload": f"p121 hex20 elastic cube {args.n}^3 ({pr.nels} elements, {pr.neq} equations), "
                         "log-normal E (seed 1), Jacobi EBE-PCG iteration, "
-                        + ("FP32-stored element matrices with FP64 vectors and reliable updates"
+                        + ("packed symmetric FP32-stored element matrices with FP64 vectors and reliable updates"
                            if args.precision == "mixed" else "FP64 element matrices"),
             "nels": pr.nels, "neq": pr.neq, "precision": args.precision,
             "paths": {k: {kk: vv for kk, vv in v.items() if kk != "elapsed_s"} for k, v in paths.items()},

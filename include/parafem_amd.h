@@ -43,7 +43,9 @@ extern "C" {
 
 /* Storage precision of element matrices. */
 #define PF_STORE_FP64      0  /* storkm_pp as REAL(iwp) (precision.f90:19)    */
-#define PF_STORE_FP32      1  /* FP32 hi + FP32 lo split (K ~= hi + lo)       */
+#define PF_STORE_FP32      1  /* FP32 hi + FP32 lo split (K ~= hi + lo) of the
+                                 upper triangle, packed symmetric: ntot/2 + 1
+                                 entries per row, 52% of the full matrix      */
 
 /* PCG stopping rule. */
 #define PF_STOP_CHECON     0  /* checon_par: max|dx|/max|x| <= tol

@@ -10,14 +10,17 @@ Every function below names the reference routine it restates; the C file
 cites the reference line ranges.
 
 Pinning status (how each function is checked in ``tests/test_oracle.py``):
-  sample_hex8, deemat6, shape_der, beemat6 -- against the reference's own
-      new_library.f90 compiled in oracle/_ref (when buildable) and against
-      closed forms (Gauss-Legendre, Hooke's law, partition of unity,
-      finite differences of the shape functions);
+  sample_hex8, deemat6, shape_der, beemat6 -- bit for bit against the
+      reference's own new_library.f90 compiled in oracle/_ref
+      (tests/test_ref.py) and against closed forms (Gauss-Legendre, Hooke's
+      law, partition of unity, finite differences of the shape functions);
   determinant3, invert3 -- closed forms / numpy on random matrices;
   elastic_km, laplace_kc -- symmetry, rigid-body null space, patch test,
       the hex8 unit-cube closed form, numpy restatement of B^T D B;
   gather/scatter -- adjointness and brute force;
+  elem_matvec, diag, apply -- numpy and the scipy-assembled global matrix;
+  checon -- exact binary cases;
+  dot_device, sum_p -- math.fsum within rounding; total load of the fixtures;
   pcg -- the reference's committed fixtures (xx3-tiny: 1640 equations, 79
       iterations, displacements; p121 demo: 98360 equations, total load
       -100, 295 iterations, displacement field) and scipy direct solves.
@@ -275,3 +278,38 @@ def pcg(g_g, storkm, r0, diag_assembled, tol, limit, stop=STOP_CHECON,
     if history:
         return x, int(it[0]), bool(conv), hist[: int(it[0])]
     return x, int(it[0]), bool(conv)
+
+
+# --------------------------------------------------------------------------
+# oracle/_ref: the reference's own element routines (new_library.f90,
+# geometry.f90) compiled by `make -C oracle ref` where the reference tree is
+# present.  Used by tests only, to pin the restatements above.
+# --------------------------------------------------------------------------
+_REF = None
+REF_PATH = os.path.join(_HERE, "_ref", "libparafem_ref.so")
+
+
+def ref():
+    """The compiled reference routines, or None when oracle/_ref is not built."""
+    global _REF
+    if _REF is None and os.path.exists(REF_PATH):
+        L = C.CDLL(REF_PATH)
+        d = C.POINTER(C.c_double)
+        i = C.POINTER(C.c_int)
+        ci, cd = C.c_int, C.c_double
+        L.ref_sample_hex.argtypes = [ci, d, d]
+        L.ref_shape_der.argtypes = [ci, ci, d, ci, d]
+        L.ref_shape_fun.argtypes = [ci, ci, d, ci, d]
+        L.ref_deemat.argtypes = [cd, cd, d]
+        L.ref_beemat.argtypes = [ci, d, d]
+        L.ref_geometry_20bxz.argtypes = [ci, ci, ci, cd, cd, cd, d, i]
+        L.ref_geometry_8bxz.argtypes = [ci, ci, ci, cd, cd, cd, d, i]
+        L.ref_cube_bc20.argtypes = [ci, ci, ci, ci, i]
+        L.ref_steering.argtypes = [ci, ci, ci, ci, i, i, i]
+        L.ref_abaqus2sg.argtypes = [ci, ci, i]
+        for f in ("ref_sample_hex", "ref_shape_der", "ref_shape_fun", "ref_deemat", "ref_beemat",
+                  "ref_geometry_20bxz", "ref_geometry_8bxz", "ref_cube_bc20", "ref_steering",
+                  "ref_abaqus2sg"):
+            getattr(L, f).restype = None
+        _REF = L
+    return _REF

@@ -42,8 +42,9 @@
 void or_sample_hex8(double *s, double *wt)
 {
     const double r = 1.0 / sqrt(3.0);
+    /* new_library.f90:1406-1433: points 6 and 7 are (-,-,+) and (-,+,-) */
     static const int sg[8][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {1, -1, -1},
-                                 {-1, 1, 1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+                                 {-1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {-1, -1, -1}};
     for (int i = 0; i < 8; ++i) {
         for (int k = 0; k < 3; ++k) s[IDX2(i, k, 8)] = sg[i][k] > 0 ? r : -r;
         wt[i] = 1.0;

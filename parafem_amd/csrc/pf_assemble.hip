@@ -90,13 +90,15 @@ __device__ __forceinline__ void element_geometry(ElemLds<NOD> &L, const double *
     const double r3 = 1.0 / sqrt(3.0);
     if (valid)
         for (int k = lane; k < NOD * 3; k += 64) L.coord[k] = gc[k];
-    // SAMPLE hexahedron nip=8: point g has (xi,eta,zeta) signs from the bits
-    // of g: xi = + for g < 4, eta = + for g%4 < 2, zeta = + for g even.
+    // SAMPLE hexahedron nip=8 (new_library.f90:1406-1433): the signs of
+    // (xi,eta,zeta) at point g, in the reference's order.
+    constexpr int gs[8][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {1, -1, -1},
+                              {-1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {-1, -1, -1}};
     for (int k = lane; k < 8 * NOD; k += 64) {
         const int g = k / NOD, l = k % NOD;
-        const double xi = (g < 4) ? r3 : -r3;
-        const double eta = ((g & 3) < 2) ? r3 : -r3;
-        const double zeta = ((g & 1) == 0) ? r3 : -r3;
+        const double xi = gs[g][0] > 0 ? r3 : -r3;
+        const double eta = gs[g][1] > 0 ? r3 : -r3;
+        const double zeta = gs[g][2] > 0 ? r3 : -r3;
         double d3[3];
         shape_der3<NOD>(xi, eta, zeta, l, d3);
         L.der[g][0][l] = d3[0];
@@ -241,17 +243,29 @@ __global__ __launch_bounds__(64 * WAVES) void k_assemble_laplace8(
     K[pf_idx_pair(e, 8, a, b)] = cx * (kx * s) + cy * (ky * s) + cz * (kz * s);
 }
 
-// hi = fl32(K), lo = fl32(K - hi) in the column-quad layout: hi + lo carries
-// ~48 significant bits and is exact in FP64 arithmetic.
+// hi = fl32(K), lo = fl32(K - hi) in the packed symmetric layout (upper
+// triangle): hi + lo carries ~48 significant bits of K.
 __global__ void k_split_fp32(const double *__restrict__ K, float *__restrict__ hi, float *__restrict__ lo,
                              int nt, long n)
 {
-    const long nn = (long)nt * nt;
+    const long per = pf_sym_per(nt);
+    const int D = pf_sym_diags(nt), Q4 = D & ~3, R = D - Q4;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const long e = q / nn, w = q - e * nn;
-        const int jq = (int)(w / (4 * nt)), rem = (int)(w - (long)jq * 4 * nt);
-        const int i = rem >> 2, j = 4 * jq + (rem & 3);
-        const double k = K[pf_idx_pair(e, nt, i, j)];
+        const long e = q / per;
+        const int w = (int)(q - e * per);
+        int i, d;
+        if (w < Q4 * nt) {
+            const int dq = w / (4 * nt), rem = w - dq * 4 * nt;
+            i = rem >> 2;
+            d = 4 * dq + (rem & 3);
+        } else {
+            const int w2 = w - Q4 * nt;
+            i = w2 / R;
+            d = Q4 + (w2 - i * R);
+        }
+        int j = i + d;
+        if (j >= nt) j -= nt;
+        const double k = K[pf_idx_pair(e, nt, i < j ? i : j, i < j ? j : i)];
         const float h = (float)k;
         hi[q] = h;
         lo[q] = (float)(k - (double)h);
@@ -269,7 +283,7 @@ __global__ void k_from_colmajor(const double *__restrict__ src, double *__restri
     }
 }
 
-// internal layout -> storkm_pp chunk (FP64 pairs, or hi + lo quads).
+// internal layout -> storkm_pp chunk (FP64 pairs, or packed symmetric hi + lo).
 __global__ void k_to_colmajor(const double *__restrict__ K, const float *__restrict__ hi,
                               const float *__restrict__ lo, double *__restrict__ dst, int nt, long e0, long nel)
 {
@@ -280,7 +294,7 @@ __global__ void k_to_colmajor(const double *__restrict__ K, const float *__restr
         if (K) {
             dst[q] = K[pf_idx_pair(e0 + el, nt, i, j)];
         } else {
-            const long t = pf_idx_quad(e0 + el, nt, i, j);
+            const long t = pf_idx_sym_ij(e0 + el, nt, i, j);
             dst[q] = (double)hi[t] + (double)lo[t];
         }
     }
@@ -333,7 +347,7 @@ int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const
 
 int pf_launch_split_fp32(pf_ctx *c)
 {
-    const long n = (long)c->ntot * c->ntot * c->nels;
+    const long n = pf_sym_per(c->ntot) * c->nels;
     hipLaunchKernelGGL(k_split_fp32, dim3(8192), dim3(256), 0, c->stream, c->K64, c->K32hi, c->K32lo, c->ntot, n);
     PF_HIP(hipGetLastError());
     return PF_OK;

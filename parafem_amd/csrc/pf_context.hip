@@ -93,10 +93,11 @@ static void free_problem_vectors(pf_ctx *c)
 
 static void free_matrices(pf_ctx *c)
 {
-    size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+    const size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+    const size_t ns = (size_t)pf_sym_per(c->ntot) * c->nels;
     free_arr(c, c->K64, nk);
-    free_arr(c, c->K32hi, nk);
-    free_arr(c, c->K32lo, nk);
+    free_arr(c, c->K32hi, ns);
+    free_arr(c, c->K32lo, ns);
     c->have_K = false;
 }
 
@@ -345,8 +346,9 @@ static int finish_matrices(pf_ctx *c)
     if (rc) return rc;
     PF_HIP(hipMemcpyAsync(c->diag, c->diag0, nv * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     if (c->store == PF_STORE_FP32) {
-        rc = pf_alloc(c, (void **)&c->K32hi, nk * sizeof(float));
-        if (!rc) rc = pf_alloc(c, (void **)&c->K32lo, nk * sizeof(float));
+        const size_t ns = (size_t)pf_sym_per(c->ntot) * c->nels;
+        rc = pf_alloc(c, (void **)&c->K32hi, ns * sizeof(float));
+        if (!rc) rc = pf_alloc(c, (void **)&c->K32lo, ns * sizeof(float));
         if (!rc && !c->utemp32) rc = pf_alloc(c, (void **)&c->utemp32, (size_t)c->ntot * c->nels * sizeof(float));
         if (!rc) rc = pf_launch_split_fp32(c);
         if (rc) return rc;

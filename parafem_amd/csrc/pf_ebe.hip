@@ -136,11 +136,27 @@ template <int NT> struct Lanes {
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef double d2_t __attribute__((ext_vector_type(2)));
 
+__device__ __forceinline__ double bpermute64(int addr, double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)b);
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Element products.  GATHER: src is a neq+1 vector with src[0] = 0 and the
-// element vector is src[g_g]; otherwise src is pmul (ntot,nels).
-//   KIND_F64: FP64 column pairs, FP64 chain -> out (double)
-//   KIND_F32: FP32 hi column quads, FP32 chain on fl32(p) -> out (float)
-//   KIND_ACC: (double)hi + (double)lo, FP64 chain -> out (double)
+// element vector is src[g_g]; otherwise src is pmul (ntot,nels).  Lane i of
+// an element's lane group forms row i of u_e = K_e p_e.
+//   KIND_F64: full K_e in FP64 column pairs, FP64 fused-multiply-add chain in
+//             ascending column order -> out (double);
+//   KIND_F32: packed symmetric FP32 hi, FP32 arithmetic on fl32(p) -> out
+//             (float);
+//   KIND_ACC: packed symmetric (double)hi + (double)lo, FP64 arithmetic ->
+//             out (double).
+// Packed symmetric rows: lane i holds K(i, i+d mod NT) for d = 0..NT/2 and
+// forms sum_d K(i,i+d) p(i+d) from p staged twice in LDS, then receives
+// K(i-d,i) p(i) from lane i-d (mod NT) for d = 1..NT/2-1 by ds_bpermute,
+// where lane j sends K(j,j+d) p(j): every entry is read from HBM once.
 template <int NT, int KIND, bool GATHER, typename OUT>
 __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
                                                  const float *__restrict__ Klo, const int *__restrict__ gg,
@@ -149,16 +165,18 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
 {
     if (skip(st, gate)) return;
     using L = Lanes<NT>;
-    __shared__ __attribute__((aligned(16))) double ps[PF_BLOCK / 64][64];
+    constexpr int D = pf_sym_diags(NT), Q = D / 4, R = D - 4 * Q;
+    __shared__ __attribute__((aligned(16))) double ps[PF_BLOCK / 64][128];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sub = lane / L::W, li = lane % L::W;
     const long e = ((long)blockIdx.x * (PF_BLOCK / 64) + wave) * L::EPW + sub;
     const long ec = e < nels ? e : (long)nels - 1;
     const int i = li < NT ? li : NT - 1;
     const int g = GATHER ? gg[ec * NT + i] : 0;
-    const double *pw = &ps[wave][sub * L::W];
+    const int gb = sub * L::W;   // first lane of the element's group
     if constexpr (KIND == KIND_F64) {
         // Every column pair of K_e is in flight before the dependent gather.
+        const double *pw = &ps[wave][sub * L::W];
         const d2_t *Kp = reinterpret_cast<const d2_t *>(K64) + ec * (long)(NT * NT / 2) + i;
         d2_t kc[NT / 2];
 #pragma unroll
@@ -174,41 +192,65 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
         }
         if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
     } else if constexpr (KIND == KIND_F32) {
-        // fl32(p_e) staged in LDS, read back four columns per 16-byte load.
-        float *psf = reinterpret_cast<float *>(&ps[wave][0]);
-        const f4_t *Kq = reinterpret_cast<const f4_t *>(Khi) + ec * (long)(NT * NT / 4) + i;
-        f4_t kc[NT / 4];
+        const float *Ke = Khi + ec * (long)(D * NT);
+        float kd[D];
 #pragma unroll
-        for (int j = 0; j < NT / 4; ++j) kc[j] = __builtin_nontemporal_load(&Kq[j * NT]);
-        psf[lane] = (float)(GATHER ? src[g] : src[ec * NT + i]);
+        for (int q = 0; q < Q; ++q) {
+            const f4_t v = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Ke) + q * NT + i);
+            kd[4 * q] = v.x; kd[4 * q + 1] = v.y; kd[4 * q + 2] = v.z; kd[4 * q + 3] = v.w;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) kd[4 * Q + r] = __builtin_nontemporal_load(Ke + 4 * Q * NT + R * i + r);
+        const float pv = (float)(GATHER ? src[g] : src[ec * NT + i]);
+        float *pg = reinterpret_cast<float *>(&ps[wave][0]) + 2 * gb;
+        if (li < NT) {
+            pg[li] = pv;
+            pg[li + NT] = pv;
+        }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const f4_t *pf4 = reinterpret_cast<const f4_t *>(psf + sub * L::W);
         float acc = 0.0f;
 #pragma unroll
-        for (int j = 0; j < NT / 4; ++j) {
-            const f4_t pj = pf4[j];
-            acc = __fmaf_rn(kc[j].x, pj.x, acc);
-            acc = __fmaf_rn(kc[j].y, pj.y, acc);
-            acc = __fmaf_rn(kc[j].z, pj.z, acc);
-            acc = __fmaf_rn(kc[j].w, pj.w, acc);
+        for (int d = 0; d < D; ++d) acc = __fmaf_rn(kd[d], pg[i + d], acc);
+#pragma unroll
+        for (int d = 1; d < D - 1; ++d) {
+            const int from = gb + (i >= d ? i - d : i - d + NT);
+            const float t = __fmul_rn(kd[d], pv);
+            acc = __fadd_rn(acc, __int_as_float(__builtin_amdgcn_ds_bpermute(from << 2, __float_as_int(t))));
         }
         if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
     } else {
-        const f4_t *Kh = reinterpret_cast<const f4_t *>(Khi) + ec * (long)(NT * NT / 4) + i;
-        const f4_t *Kl = reinterpret_cast<const f4_t *>(Klo) + ec * (long)(NT * NT / 4) + i;
-        ps[wave][lane] = GATHER ? src[g] : src[ec * NT + i];
+        const float *Kh = Khi + ec * (long)(D * NT);
+        const float *Kl = Klo + ec * (long)(D * NT);
+        double kd[D];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const f4_t h = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Kh) + q * NT + i);
+            const f4_t l = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Kl) + q * NT + i);
+            kd[4 * q] = (double)h.x + (double)l.x;
+            kd[4 * q + 1] = (double)h.y + (double)l.y;
+            kd[4 * q + 2] = (double)h.z + (double)l.z;
+            kd[4 * q + 3] = (double)h.w + (double)l.w;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            kd[4 * Q + r] = (double)__builtin_nontemporal_load(Kh + 4 * Q * NT + R * i + r) +
+                            (double)__builtin_nontemporal_load(Kl + 4 * Q * NT + R * i + r);
+        const double pv = GATHER ? src[g] : src[ec * NT + i];
+        double *pg = &ps[wave][0] + 2 * gb;
+        if (li < NT) {
+            pg[li] = pv;
+            pg[li + NT] = pv;
+        }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < NT / 4; ++j) {
-            const f4_t h = __builtin_nontemporal_load(&Kh[j * NT]);
-            const f4_t l = __builtin_nontemporal_load(&Kl[j * NT]);
-            acc = __fma_rn((double)h.x + (double)l.x, pw[4 * j], acc);
-            acc = __fma_rn((double)h.y + (double)l.y, pw[4 * j + 1], acc);
-            acc = __fma_rn((double)h.z + (double)l.z, pw[4 * j + 2], acc);
-            acc = __fma_rn((double)h.w + (double)l.w, pw[4 * j + 3], acc);
+        for (int d = 0; d < D; ++d) acc = __fma_rn(kd[d], pg[i + d], acc);
+#pragma unroll
+        for (int d = 1; d < D - 1; ++d) {
+            const int from = gb + (i >= d ? i - d : i - d + NT);
+            acc = __dadd_rn(acc, bpermute64(from << 2, __dmul_rn(kd[d], pv)));
         }
         if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
     }

@@ -39,6 +39,27 @@ __host__ __device__ inline long pf_idx_quad(long e, int nt, int i, int j)
     return e * (long)nt * nt + (long)(j >> 2) * 4 * nt + 4 * i + (j & 3);
 }
 
+// Packed symmetric storage of the FP32 hi and lo parts.  Row i keeps the
+// nt/2 + 1 entries K(i, (i+d) mod nt), d = 0..nt/2, taken from the upper
+// triangle K(min, max); the entries of row i left of the diagonal band are
+// the band entries of other rows (K(i, i-d) = K(i-d, (i-d)+d)).  Per element:
+// d in quads, lane-major (16-byte loads), then the D mod 4 remainder.
+__host__ __device__ constexpr int pf_sym_diags(int nt) { return nt / 2 + 1; }
+__host__ __device__ constexpr long pf_sym_per(int nt) { return (long)pf_sym_diags(nt) * nt; }
+__host__ __device__ inline long pf_idx_sym(long e, int nt, int i, int d)
+{
+    const int D = pf_sym_diags(nt), Q4 = D & ~3, R = D - Q4;
+    const long base = e * (long)D * nt;
+    return d < Q4 ? base + (long)(d >> 2) * 4 * nt + 4 * i + (d & 3) : base + (long)Q4 * nt + (long)R * i + (d - Q4);
+}
+// Packed slot of K(i, j).
+__host__ __device__ inline long pf_idx_sym_ij(long e, int nt, int i, int j)
+{
+    int d = j - i;
+    if (d < 0) d += nt;
+    return d <= nt / 2 ? pf_idx_sym(e, nt, i, d) : pf_idx_sym(e, nt, j, nt - d);
+}
+
 // Device-resident PCG scalars.  Written only by the one-block fold kernels;
 // read at the start of the following kernels.
 struct pf_state {

@@ -249,9 +249,17 @@ def test_split_run_equals_solve(ctx):
     assert np.array_equal(ctx.pcg_x(), x)
 
 
+def _upper_sym(K):
+    """Element matrices rebuilt from their upper triangles K(r, c), r <= c.
+    storkm arrays are (nels, column, row) in numpy order, so the upper
+    triangle is numpy's lower one."""
+    return np.tril(K) + np.transpose(np.tril(K, -1), (0, 2, 1))
+
+
 def test_storkm_roundtrip_layouts(ctx):
-    """storkm_pp in, storkm_pp out: exact for FP64 storage; hi+lo within
-    2^-46 of K for FP32 storage."""
+    """storkm_pp in, storkm_pp out: exact for FP64 storage; for the packed
+    symmetric FP32 storage, hi+lo within 2^-46 of the upper triangle mirrored
+    (the assembled matrices are symmetric to rounding)."""
     pr = _distorted(3, seed=6)
     say("storkm round trip")
     Ko = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem)
@@ -261,8 +269,54 @@ def test_storkm_roundtrip_layouts(ctx):
     assert np.array_equal(ctx.storkm_get(), Ko)
     ctx.storkm_set(Ko, store="fp32")
     K2 = ctx.storkm_get()
-    assert np.all(np.abs(K2 - Ko) <= 2.0 ** -46 * np.abs(Ko))
+    Ku = _upper_sym(Ko)
+    assert np.all(np.abs(K2 - Ku) <= 2.0 ** -46 * np.abs(Ku))
+    assert np.array_equal(K2, np.transpose(K2, (0, 2, 1)))
+    assert np.abs(Ku - Ko).max() <= 1e-14 * np.abs(Ko).max()
     assert np.array_equal(ctx.diag_get(), O.diag(pr.g_g, Ko, pr.neq))
+
+
+@pytest.mark.parametrize("nt_case", ["hex20", "hex8_elastic", "hex8_laplace"])
+def test_packed_symmetric_products(ctx, nt_case):
+    """FP64 element products and K x of the packed symmetric FP32 storage
+    (hi+lo, the reliable-update residual) against numpy on the unpacked
+    matrices, for ntot = 60, 24 and 8: within 1e-13 of the row scale."""
+    rng = np.random.default_rng(11)
+    if nt_case == "hex20":
+        pr = _distorted(3, seed=7)
+        Ko = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem)
+        gc, g_g, neq = pr.g_coord(), pr.g_g, pr.neq
+    elif nt_case == "hex8_laplace":
+        pr = P.laplace_cube(4)
+        Ko = O.assemble_laplace(pr.g_coord(), 1.0, 1.0, 1.0, pr.k_elem)
+        gc, g_g, neq = pr.g_coord(), pr.g_g, pr.neq
+    else:
+        m = M.box_hex8(3, 3, 3)
+        nf, neq = M.steering(m.nn, 3, M.base_fixed_rest(m))
+        g_g = M.g_g_from(nf, m.g_num)
+        gc = m.g_coord()
+        A = rng.normal(size=(m.nels, 24, 24))
+        Ko = A + np.transpose(A, (0, 2, 1))
+    say(f"packed symmetric products {nt_case} ntot={g_g.shape[1]}")
+    ctx.mesh_set(gc)
+    ctx.steer_set(g_g, neq)
+    ctx.storkm_set(Ko, store="fp32")
+    Ks = ctx.storkm_get()
+    nels, nt = g_g.shape
+    pm = rng.normal(size=(nels, nt))
+    dpm = DeviceArray.from_host(ctx, pm.reshape(-1))
+    dut = DeviceArray(ctx, nels * nt)
+    ctx.elem_matvec(dpm, dut)
+    ref = np.einsum("eji,ej->ei", Ks, pm)
+    scale = np.abs(Ks).sum(axis=1) * np.abs(pm).max()
+    assert np.all(np.abs(dut.download().reshape(nels, nt) - ref) <= 1e-13 * scale)
+    p = rng.normal(size=neq)
+    y = ctx.apply(p)
+    pe = np.where(g_g > 0, np.concatenate([[0.0], p])[g_g], 0.0)
+    ue = np.einsum("eji,ej->ei", Ks, pe)
+    yr = np.zeros(neq + 1)
+    np.add.at(yr, g_g.reshape(-1), ue.reshape(-1))
+    assert np.allclose(y, yr[1:], rtol=0, atol=1e-12 * np.abs(yr).max())
 
 
 @pytest.mark.parametrize("n", [6, 12])

@@ -25,9 +25,11 @@ def test_sample_is_2x2x2_gauss_legendre():
     pts, w = O.sample_hex8()
     assert np.allclose(np.abs(pts), R3, rtol=0, atol=1e-16)
     assert np.all(w == 1.0)
-    # order of SAMPLE (new_library.f90:1406-1433): xi slowest, zeta fastest
+    # order of SAMPLE (new_library.f90:1406-1433), whose points 6 and 7 are
+    # (-,-,+) and (-,+,-)
     assert np.all(pts[:4, 0] > 0) and np.all(pts[4:, 0] < 0)
-    assert list(np.sign(pts[:, 2])) == [1, -1, 1, -1, 1, -1, 1, -1]
+    assert list(np.sign(pts[:, 1])) == [1, 1, -1, -1, 1, -1, 1, -1]
+    assert list(np.sign(pts[:, 2])) == [1, -1, 1, -1, 1, 1, -1, -1]
     # exact for polynomials of degree 3 per direction: int x^2 y^2 z^2 = 8/27
     f = np.prod(pts ** 2, axis=1)
     assert abs(np.dot(w, f) - 8.0 / 27.0) < 1e-15
@@ -226,6 +228,55 @@ def test_gather_scatter_adjoint_and_bruteforce():
             if g:
                 ref[g - 1] += v[e, i]
     assert np.allclose(sv, ref, rtol=1e-14, atol=1e-14)
+
+
+def _global_sparse(g_g, K, neq):
+    """Global stiffness assembled independently with scipy (COO sums
+    duplicates); K is storkm in numpy order (nels, column, row)."""
+    rows, cols, vals = [], [], []
+    for e in range(g_g.shape[0]):
+        g = g_g[e]
+        Ke = K[e].T
+        m = g > 0
+        gi = g[m] - 1
+        rows.append(np.repeat(gi, gi.size)); cols.append(np.tile(gi, gi.size))
+        vals.append(Ke[np.ix_(m, m)].reshape(-1))
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(neq, neq))
+
+
+def test_elem_matvec_diag_apply_against_global_matrix():
+    """a10/a17: element products against dense numpy products, the assembled
+    diagonal and K x against the scipy-assembled global matrix."""
+    pr = P.elastic_cube(3, lognormal_e=True)
+    K = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem)
+    A = _global_sparse(pr.g_g, K, pr.neq)
+    rng = np.random.default_rng(5)
+    pm = rng.normal(size=pr.g_g.shape)
+    ut = O.elem_matvec(K, pm)
+    ref = np.einsum("eji,ej->ei", K, pm)
+    assert np.allclose(ut, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    dg = O.diag(pr.g_g, K, pr.neq)
+    assert np.allclose(dg, A.diagonal(), rtol=1e-14, atol=0)
+    x = rng.normal(size=pr.neq)
+    y = O.apply(pr.g_g, K, x)
+    assert np.allclose(y, A @ x, rtol=0, atol=1e-12 * np.abs(A @ x).max())
+
+
+def test_checon_closed_cases():
+    """a22: converged iff max|xnew - x| <= tol * max|xnew|; x <- xnew."""
+    xnew = np.array([1.0, -4.0, 2.0, 0.5])
+    x = xnew.copy()
+    x[2] += 0.0625                     # max diff 2^-4, max|xnew| 4 -> crit 2^-6
+    conv, xo = O.checon(xnew, 0.015625, x)
+    assert conv and np.array_equal(xo, xnew)
+    conv, _ = O.checon(xnew, 0.0156, x)
+    assert not conv
+    x[1] = -4.5                        # the difference at the largest entry counts
+    conv, _ = O.checon(xnew, 0.1, x)
+    assert not conv
+    conv, _ = O.checon(xnew, 0.125, x)
+    assert conv
 
 
 def test_device_order_dot_is_a_sum():

@@ -1,0 +1,117 @@
+"""Pins the oracle and the mesh / deck code against the reference's own
+routines, compiled from src/modules/shared/{precision,geometry,new_library}.f90
+into oracle/_ref by ``make -C oracle ref`` (skipped when that library is not
+built, e.g. where the reference tree is absent).
+
+Bars: bit-exact (the oracle is compiled without contraction, like the
+reference build in oracle/_ref).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from parafem_amd import mesh as M
+
+R = O.ref()
+pytestmark = pytest.mark.skipif(R is None, reason="oracle/_ref not built (needs the reference tree)")
+
+d_p = lambda a: a.ctypes.data_as(O.C.POINTER(O.C.c_double))  # noqa: E731
+i_p = lambda a: a.ctypes.data_as(O.C.POINTER(O.C.c_int))     # noqa: E731
+
+
+def ref_sample():
+    s = np.zeros((3, 8))          # column-major (nip,3) as a (3,nip) C array
+    w = np.zeros(8)
+    R.ref_sample_hex(8, d_p(s), d_p(w))
+    return s.T.copy(), w
+
+
+def test_sample_bitexact():
+    s, w = ref_sample()
+    so, wo = O.sample_hex8()
+    assert np.array_equal(s, so) and np.array_equal(w, wo)
+
+
+@pytest.mark.parametrize("nod", [8, 20])
+def test_shape_der_bitexact(nod):
+    pts, _ = ref_sample()
+    rng = np.random.default_rng(nod)
+    extra = rng.uniform(-1, 1, size=(5, 3))
+    for P_ in (pts, extra):
+        colmaj = np.ascontiguousarray(P_.T)
+        for i in range(1, P_.shape[0] + 1):
+            der = np.zeros((nod, 3))  # column-major (3,nod)
+            R.ref_shape_der(nod, P_.shape[0], d_p(colmaj), i, d_p(der))
+            assert np.array_equal(der.T, O.shape_der(P_, i, nod)), (nod, i)
+
+
+@pytest.mark.parametrize("e,v", [(100.0, 0.3), (1.0, 0.0), (7.5, 0.45), (210e3, 0.2)])
+def test_deemat_bitexact(e, v):
+    dee = np.zeros((6, 6))
+    R.ref_deemat(e, v, d_p(dee))
+    assert np.array_equal(dee.T, O.deemat6(e, v))
+
+
+@pytest.mark.parametrize("nod", [8, 20])
+def test_beemat_bitexact(nod):
+    rng = np.random.default_rng(3)
+    deriv = rng.normal(size=(3, nod))
+    cm = np.ascontiguousarray(deriv.T)
+    bee = np.zeros((3 * nod, 6))
+    R.ref_beemat(nod, d_p(cm), d_p(bee))
+    assert np.array_equal(bee.T, O.beemat6(deriv))
+
+
+@pytest.mark.parametrize("nxe,nye,nze", [(3, 2, 4), (1, 1, 1), (5, 3, 2)])
+def test_box_hex20_is_geometry_20bxz(nxe, nye, nze):
+    m = M.box_hex20(nxe, nye, nze, 1.0, 1.0, 1.0)
+    for iel in range(1, m.nels + 1):
+        coord = np.zeros((3, 20))
+        num = np.zeros(20, dtype=np.int32)
+        R.ref_geometry_20bxz(iel, nxe, nze, 1.0, 1.0, 1.0, d_p(coord), i_p(num))
+        assert np.array_equal(num, m.g_num[iel - 1]), iel
+        assert np.array_equal(coord.T, m.coords[num - 1]), iel
+
+
+@pytest.mark.parametrize("nxe,nye,nze", [(3, 2, 4), (2, 2, 2)])
+def test_box_hex8_is_geometry_8bxz(nxe, nye, nze):
+    m = M.box_hex8(nxe, nye, nze, 1.0, 1.0, 1.0)
+    for iel in range(1, m.nels + 1):
+        coord = np.zeros((3, 8))
+        num = np.zeros(8, dtype=np.int32)
+        R.ref_geometry_8bxz(iel, nxe, nze, 1.0, 1.0, 1.0, d_p(coord), i_p(num))
+        assert np.array_equal(num, m.g_num[iel - 1]), iel
+        assert np.array_equal(coord.T, m.coords[num - 1]), iel
+
+
+@pytest.mark.parametrize("n", [2, 4, 5])
+def test_cube_bc20(n):
+    ours = M.cube_bc20(n, n, n)
+    rest = np.zeros((4, len(ours)), dtype=np.int32)     # column-major (nr,4)
+    R.ref_cube_bc20(n, n, n, len(ours), i_p(rest))
+    assert np.array_equal(rest.T, ours)
+
+
+def test_steering_xx3_tiny_is_find_g3(golden):
+    d = M.read_deck(os.path.join(golden, "xx3_tiny", "xx3-tiny"))
+    nf, neq = M.steering(d["nn"], 3, d["rest"])
+    g_g = M.g_g_from(nf, d["g_num"])
+    nels, nod = d["g_num"].shape
+    rest = np.ascontiguousarray(d["rest"].T.astype(np.int32))
+    gn = np.ascontiguousarray(d["g_num"].astype(np.int32))
+    out = np.zeros((nels, nod * 3), dtype=np.int32)
+    R.ref_steering(nod, 3, nels, rest.shape[1], i_p(gn), i_p(rest), i_p(out))
+    assert np.array_equal(out, g_g)
+    assert out.max() == neq == 1640
+
+
+@pytest.mark.parametrize("nod", [8, 20])
+def test_abaqus_to_sg(nod):
+    rng = np.random.default_rng(nod)
+    g = np.ascontiguousarray(rng.integers(1, 1000, size=(7, nod)).astype(np.int32))
+    out = g.copy()
+    R.ref_abaqus2sg(nod, g.shape[0], i_p(out))
+    assert np.array_equal(out, M.abaqus_to_sg(g))
+    assert np.array_equal(M.sg_to_abaqus(out), g)
