@@ -94,14 +94,24 @@ def solve(ctx, pr, precision, store, tol):
                "solves_per_s": round(1.0 / t_solve, 5), "true_relres": true_rel}
 
 
-def cpu_baseline(n_sample: int, budget_s: float = 12.0):
-    """The CPU oracle's FP64 EBE-PCG on an n^3 sample of the same workload,
-    all host threads (capped at 16), rate scaled to the 10^6-element cube."""
-    import oracle as O
+def build_problem(workload, n):
     from parafem_amd import problems as P
+    if workload == "hex20":
+        return P.elastic_cube(n, lognormal_e=True)
+    return P.laplace_cube(n, lognormal_k=True)
+
+
+def cpu_baseline(workload: str, n_sample: int, nels_full: int, budget_s: float = 12.0):
+    """The CPU oracle's FP64 EBE-PCG on an n^3 sample of the same workload,
+    all host threads (capped at 16), rate scaled by elements to the full
+    problem."""
+    import oracle as O
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
-    pr = P.elastic_cube(n_sample, lognormal_e=True)
-    K = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem, nthreads=threads)
+    pr = build_problem(workload, n_sample)
+    if pr.kind == "elastic":
+        K = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem, nthreads=threads)
+    else:
+        K = O.assemble_laplace(pr.g_coord(), pr.kx, pr.ky, pr.kz, pr.k_elem, nthreads=threads)
     dg = O.diag(pr.g_g, K, pr.neq)
 
     def run(n):
@@ -116,13 +126,13 @@ def cpu_baseline(n_sample: int, budget_s: float = 12.0):
     t = run(n + 2) - t1
     rate = n / t
     return {
-        "value": rate * pr.nels / 1_000_000,
+        "value": rate * pr.nels / nels_full,
         "unit": "iter/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n_sample}^3 hex20 cube ({pr.nels} elements, {pr.neq} equations), {n} FP64 "
+        "sample": f"{n_sample}^3 {workload} cube ({pr.nels} elements, {pr.neq} equations), {n} FP64 "
                   f"PCG iterations in {t:.1f} s = {rate:.2f} iter/s, scaled by elements to "
-                  f"1,000,000",
+                  f"{nels_full:,}",
     }
 
 
@@ -131,7 +141,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=100, help="elements per cube edge")
+    ap.add_argument("--workload", default="hex20", choices=["hex20", "hex8"],
+                    help="hex20: p121 elastic cube (default, 100^3); hex8: p123 Laplace cube (200^3)")
+    ap.add_argument("--n", type=int, default=0, help="elements per cube edge (0 = the workload's size)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
                     help="path whose iteration rate is the headline value")
     ap.add_argument("--solve-tol", type=float, default=1e-10,
@@ -158,8 +170,10 @@ def main():
         print(json.dumps({"copy_bytes": n * 8, "launches": 5}), flush=True)
         return
 
+    if args.n == 0:
+        args.n = 100 if args.workload == "hex20" else 200
     t0 = time.perf_counter()
-    pr = P.elastic_cube(args.n, lognormal_e=True)
+    pr = build_problem(args.workload, args.n)
     log(f"problem built: nels={pr.nels} neq={pr.neq} ({time.perf_counter() - t0:.1f} s)")
     ctx = Context(0)
     paths = {}
@@ -205,7 +219,8 @@ def main():
     cpu = None
     if args.cpu_sample > 0:
         log("cpu baseline ...")
-        cpu = cpu_baseline(args.cpu_sample)
+        cpu = cpu_baseline(args.workload, args.cpu_sample if args.workload == "hex20" else 2 * args.cpu_sample,
+                           pr.nels)
         log(f"cpu baseline {cpu['value']:.3f} iter/s ({cpu['cores']} threads)")
 
     head = paths[args.precision]
@@ -223,8 +238,11 @@ def main():
         "dtype": "f32+f64" if args.precision == "mixed" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"p121 hex20 elastic cube {args.n}^3 ({pr.nels} elements, {pr.neq} equations), "
-                        "log-normal E (seed 1), Jacobi EBE-PCG iteration, "
+            "workload": (f"p121 hex20 elastic cube {args.n}^3 ({pr.nels} elements, {pr.neq} equations), "
+                         "log-normal E (seed 1)" if args.workload == "hex20" else
+                         f"p123 hex8 Laplace cube {args.n}^3 ({pr.nels} elements, {pr.neq} equations), "
+                         "log-normal k (seed 2), T=0 on x=0 by penalty, unit flux on x=n")
+                        + ", Jacobi EBE-PCG iteration, "
                         + ("packed symmetric FP32-stored element matrices with FP64 vectors and reliable updates"
                            if args.precision == "mixed" else "FP64 element matrices"),
             "nels": pr.nels, "neq": pr.neq, "precision": args.precision,
@@ -234,7 +252,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_ebe element products (" + ("FP32 hi" if args.precision == "mixed" else "FP64") + ")",
+            "kernel": f"k_ebe<{pr.ntot}> element products (" + ("FP32 hi, packed symmetric" if args.precision == "mixed" else "FP64") + ")",
             "achieved": head["ebe_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -138,6 +138,15 @@ int pf_assemble_laplace(pf_ctx *ctx, double kx, double ky, double kz, const doub
  * caller keep its own assembly (the driver's loop A) and use the device PCG. */
 int pf_storkm_set(pf_ctx *ctx, const double *storkm_pp, int store);
 int pf_storkm_get(pf_ctx *ctx, double *storkm_pp);
+/* One element matrix km(ntot,ntot) (host, FP64, column-major) used by every
+ * element: the identical-K_e solve of PCG_KM (src/modules/mpi/maths.f90:
+ * 1152-1323, utemp_pp = MATMUL(km, pmul_pp)).  FP64 only; storkm_get then
+ * returns km for every element. */
+int pf_storkm_shared(pf_ctx *ctx, const double *km);
+/* Jacobi preconditioner given by the caller, already inverted
+ * (diag_precon_pp, host, neq), as PCG_KM receives it (maths.f90:1152-1170):
+ * the PCG uses it instead of 1/diag until the next assembly or upload. */
+int pf_precon_set(pf_ctx *ctx, const double *diag_precon);
 /* Assembled diagonal of the element matrices (host, neq): the
  * diag_precon_tmp -> SCATTER step of p121.f90:65-69 before inversion. */
 int pf_diag_get(pf_ctx *ctx, double *diag);
@@ -195,6 +204,9 @@ int pf_pcg_begin(pf_ctx *ctx, const double *r0, int r0_on_device, const pf_pcg_o
 int pf_pcg_run(pf_ctx *ctx, int n_iters);
 int pf_pcg_poll(pf_ctx *ctx, pf_pcg_result *res);
 int pf_pcg_x(pf_ctx *ctx, double *x, int x_on_device);
+/* The recurrence residual r at the end of the solve (PCG_KM's r_pp is
+ * INOUT, maths.f90:1164). */
+int pf_pcg_r(pf_ctx *ctx, double *r, int r_on_device);
 /* y = K x over the assembled operator in FP64 (host vectors, neq), with the
  * p123 fixed-freedom rows applied.  Used to report true residuals. */
 int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);

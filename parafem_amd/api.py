@@ -104,6 +104,9 @@ def lib():
         "pf_dot_product_p": (i32, [vp, vp, vp, i64, pd]),
         "pf_sum_p": (i32, [vp, vp, i64, pd]),
         "pf_checon_par": (i32, [vp, vp, dbl, pi, vp, i64]),
+        "pf_storkm_shared": (i32, [vp, pd]),
+        "pf_precon_set": (i32, [vp, pd]),
+        "pf_pcg_r": (i32, [vp, pd, i32]),
         "pf_vec_fill": (i32, [vp, vp, dbl, i64]),
         "pf_vec_waxpy": (i32, [vp, vp, vp, vp, dbl, i64]),
         "pf_vec_mul": (i32, [vp, vp, vp, vp, i64]),
@@ -294,6 +297,19 @@ class Context:
         assert k.shape == (self.nels, self.ntot, self.ntot)
         _check(lib().pf_storkm_set(self.h, _pd(k), STORE[store]), "pf_storkm_set")
 
+    def storkm_shared(self, km: np.ndarray):
+        """One element matrix for every element (PCG_KM's km), numpy order
+        (column, row) like one storkm_pp block."""
+        k = _f64(km)
+        assert k.shape == (self.ntot, self.ntot)
+        _check(lib().pf_storkm_shared(self.h, _pd(k)), "pf_storkm_shared")
+
+    def precon_set(self, diag_precon: np.ndarray):
+        """Caller's inverted Jacobi preconditioner (neq)."""
+        d = _f64(diag_precon)
+        assert d.shape == (self.neq,)
+        _check(lib().pf_precon_set(self.h, _pd(d)), "pf_precon_set")
+
     def storkm_get(self) -> np.ndarray:
         out = np.empty((self.nels, self.ntot, self.ntot))
         _check(lib().pf_storkm_get(self.h, _pd(out)), "pf_storkm_get")
@@ -421,6 +437,12 @@ class Context:
         x = np.empty(self.neq)
         _check(lib().pf_pcg_x(self.h, x.ctypes.data, 0), "pf_pcg_x")
         return x
+
+    def pcg_r(self) -> np.ndarray:
+        """Recurrence residual at the end of the solve."""
+        r = np.empty(self.neq)
+        _check(lib().pf_pcg_r(self.h, _pd(r), 0), "pf_pcg_r")
+        return r
 
     def timing_enable(self, on: bool = True):
         """Per-kernel HIP-event timing of pcg_run (resets the accumulators)."""

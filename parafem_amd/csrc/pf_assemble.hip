@@ -285,14 +285,15 @@ __global__ void k_from_colmajor(const double *__restrict__ src, double *__restri
 
 // internal layout -> storkm_pp chunk (FP64 pairs, or packed symmetric hi + lo).
 __global__ void k_to_colmajor(const double *__restrict__ K, const float *__restrict__ hi,
-                              const float *__restrict__ lo, double *__restrict__ dst, int nt, long e0, long nel)
+                              const float *__restrict__ lo, double *__restrict__ dst, int nt, long e0, long nel,
+                              long kstride)
 {
     const long nn = (long)nt * nt, n = nel * nn;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
         const long el = q / nn, w = q - el * nn;
         const int j = (int)(w / nt), i = (int)(w - (long)j * nt);
         if (K) {
-            dst[q] = K[pf_idx_pair(e0 + el, nt, i, j)];
+            dst[q] = K[pf_idx_pair((e0 + el) * kstride, nt, i, j)];
         } else {
             const long t = pf_idx_sym_ij(e0 + el, nt, i, j);
             dst[q] = (double)hi[t] + (double)lo[t];
@@ -303,7 +304,7 @@ __global__ void k_to_colmajor(const double *__restrict__ K, const float *__restr
 // Assembled diagonal: diag(q) = sum over the slots of q, ascending element,
 // of storkm(i,i,e) (p121.f90:65-69 then SCATTER).  diag has a leading 0 slot.
 __global__ void k_diag(const int *__restrict__ ptr, const int *__restrict__ slot, const double *__restrict__ K,
-                       int ntot, double *__restrict__ diag, int neq)
+                       int ntot, long kstride, double *__restrict__ diag, int neq)
 {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < neq; q += gridDim.x * blockDim.x) {
         double s = 0.0;
@@ -311,7 +312,7 @@ __global__ void k_diag(const int *__restrict__ ptr, const int *__restrict__ slot
             const long sl = slot[k];
             const long e = sl / ntot;
             const int i = (int)(sl - e * ntot);
-            s = __dadd_rn(s, K[pf_idx_pair(e, ntot, i, i)]);
+            s = __dadd_rn(s, K[pf_idx_pair(e * kstride, ntot, i, i)]);
         }
         diag[q + 1] = s;
     }
@@ -363,7 +364,7 @@ int pf_launch_from_colmajor(pf_ctx *c, const double *src, long e0, long nel)
 int pf_launch_to_colmajor(pf_ctx *c, double *dst, long e0, long nel)
 {
     hipLaunchKernelGGL(k_to_colmajor, dim3(8192), dim3(256), 0, c->stream, c->K64, c->K32hi, c->K32lo, dst,
-                       c->ntot, e0, nel);
+                       c->ntot, e0, nel, pf_k64_estride(c));
     PF_HIP(hipGetLastError());
     return PF_OK;
 }
@@ -372,7 +373,7 @@ int pf_launch_diag(pf_ctx *c, double *diag)
 {
     const int grid = (c->neq + 255) / 256 < 8192 ? (c->neq + 255) / 256 : 8192;
     hipLaunchKernelGGL(k_diag, dim3(grid), dim3(256), 0, c->stream, c->csr_ptr, c->csr_slot, c->K64, c->ntot,
-                       diag, c->neq);
+                       pf_k64_estride(c), diag, c->neq);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }

@@ -93,12 +93,13 @@ static void free_problem_vectors(pf_ctx *c)
 
 static void free_matrices(pf_ctx *c)
 {
-    const size_t nk = (size_t)c->ntot * c->ntot * c->nels;
     const size_t ns = (size_t)pf_sym_per(c->ntot) * c->nels;
-    free_arr(c, c->K64, nk);
+    free_arr(c, c->K64, pf_k64_count(c));
     free_arr(c, c->K32hi, ns);
     free_arr(c, c->K32lo, ns);
     c->have_K = false;
+    c->k_shared = false;
+    c->precon_given = false;
 }
 
 // Drop every buffer of the current problem (a context can be reused for a
@@ -326,13 +327,14 @@ int pf_steer_set(pf_ctx *c, int ntot, int nels, int neq, const int *g_g)
     return PF_OK;
 }
 
-static int alloc_k64(pf_ctx *c, int store)
+static int alloc_k64(pf_ctx *c, int store, bool shared = false)
 {
     if (store != PF_STORE_FP64 && store != PF_STORE_FP32) return PF_ERR_ARG;
+    if (shared && store != PF_STORE_FP64) return PF_ERR_ARG;
     free_matrices(c);
-    const size_t nk = (size_t)c->ntot * c->ntot * c->nels;
     c->store = store;
-    return pf_alloc(c, (void **)&c->K64, nk * sizeof(double));
+    c->k_shared = shared;
+    return pf_alloc(c, (void **)&c->K64, pf_k64_count(c) * sizeof(double));
 }
 
 // After a FP64 assembly or upload into K64: the assembled diagonal
@@ -340,7 +342,7 @@ static int alloc_k64(pf_ctx *c, int store)
 // hi/lo split, after which the FP64 copy is released.
 static int finish_matrices(pf_ctx *c)
 {
-    const size_t nk = (size_t)c->ntot * c->ntot * c->nels;
+    const size_t nk = pf_k64_count(c);
     const size_t nv = (size_t)c->neq + 1;
     int rc = pf_launch_diag(c, c->diag0);
     if (rc) return rc;
@@ -445,6 +447,39 @@ int pf_storkm_set(pf_ctx *c, const double *storkm, int store)
     pf_free(c, stage, (size_t)chunk * nn * sizeof(double));
     if (rc) return rc;
     return finish_matrices(c);
+}
+
+int pf_storkm_shared(pf_ctx *c, const double *km)
+{
+    if (!c || !km) return PF_ERR_ARG;
+    if (!c->g_g) return PF_ERR_STATE;
+    PF_HIP(hipSetDevice(c->device));
+    int rc = alloc_k64(c, PF_STORE_FP64, true);
+    if (rc) return rc;
+    const size_t nn = (size_t)c->ntot * c->ntot;
+    double *stage = nullptr;
+    rc = pf_alloc(c, (void **)&stage, nn * sizeof(double));
+    if (!rc) {
+        hipError_t he = hipMemcpyAsync(stage, km, nn * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        if (he != hipSuccess) rc = pf_set_hip_error(he, "km upload", __FILE__, __LINE__);
+        if (!rc) rc = pf_launch_from_colmajor(c, stage, 0, 1);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    pf_free(c, stage, nn * sizeof(double));
+    if (rc) return rc;
+    return finish_matrices(c);
+}
+
+int pf_precon_set(pf_ctx *c, const double *diag_precon)
+{
+    if (!c || !diag_precon) return PF_ERR_ARG;
+    if (!c->have_K) return PF_ERR_STATE;
+    PF_HIP(hipSetDevice(c->device));
+    PF_HIP(hipMemcpyAsync(c->dinv + 1, diag_precon, sizeof(double) * c->neq, hipMemcpyHostToDevice, c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    c->precon_given = true;
+    c->pcg_ready = false;
+    return PF_OK;
 }
 
 int pf_storkm_get(pf_ctx *c, double *storkm)

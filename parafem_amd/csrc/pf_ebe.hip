@@ -161,7 +161,7 @@ template <int NT, int KIND, bool GATHER, typename OUT>
 __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
                                                  const float *__restrict__ Klo, const int *__restrict__ gg,
                                                  const double *__restrict__ src, OUT *__restrict__ out, int nels,
-                                                 const pf_state *__restrict__ st, int gate)
+                                                 long kstride, const pf_state *__restrict__ st, int gate)
 {
     if (skip(st, gate)) return;
     using L = Lanes<NT>;
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
     if constexpr (KIND == KIND_F64) {
         // Every column pair of K_e is in flight before the dependent gather.
         const double *pw = &ps[wave][sub * L::W];
-        const d2_t *Kp = reinterpret_cast<const d2_t *>(K64) + ec * (long)(NT * NT / 2) + i;
+        const d2_t *Kp = reinterpret_cast<const d2_t *>(K64) + ec * kstride * (NT * NT / 2) + i;
         d2_t kc[NT / 2];
 #pragma unroll
         for (int j = 0; j < NT / 2; ++j) kc[j] = __builtin_nontemporal_load(&Kp[j * NT]);
@@ -459,12 +459,13 @@ __global__ __launch_bounds__(PF_BLOCK) void k_pupdate(double *__restrict__ p, co
 __global__ __launch_bounds__(PF_BLOCK) void k_pcg_init(double *__restrict__ x, const double *__restrict__ r,
                                                       double *__restrict__ d, double *__restrict__ p,
                                                       double *__restrict__ dinv, const double *__restrict__ diag,
-                                                      double *__restrict__ f, int neq, double *partials)
+                                                      double *__restrict__ f, int neq, int keep_dinv,
+                                                      double *partials)
 {
     double v[2] = {0.0, 0.0};
     const int stride = gridDim.x * PF_BLOCK;
     for (int q = blockIdx.x * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
-        const double di = 1.0 / diag[q];
+        const double di = keep_dinv ? dinv[q] : 1.0 / diag[q];
         const double rq = r[q];
         const double dq = __dmul_rn(di, rq);
         dinv[q] = di;
@@ -612,7 +613,7 @@ int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, in
 {
     const int grid = (c->nels + Lanes<NT>::EPB - 1) / Lanes<NT>::EPB;
     hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64, c->K32hi,
-                       c->K32lo, c->g_g, src, out, c->nels, st, gate);
+                       c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }
@@ -783,7 +784,7 @@ int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opt
     PF_HIP(hipMemcpyAsync(c->state, c->state_host, sizeof(pf_state), hipMemcpyHostToDevice, c->stream));
     const int G = pf_grid_for(c->neq);
     hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, c->dinv, c->diag,
-                       c->f, c->neq, c->partials);
+                       c->f, c->neq, c->precon_given ? 1 : 0, c->partials);
     hipLaunchKernelGGL(k_fold_init, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
     PF_HIP(hipGetLastError());
     // The host mirror is read back by pf_pcg_poll; wait so it is not reused
@@ -905,6 +906,16 @@ int pf_pcg_x(pf_ctx *c, double *x, int on_device)
     if (!c || !x) return PF_ERR_ARG;
     if (!c->pcg_ready) return PF_ERR_STATE;
     PF_HIP(hipMemcpyAsync(x, c->x + 1, sizeof(double) * c->neq,
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
+int pf_pcg_r(pf_ctx *c, double *r, int on_device)
+{
+    if (!c || !r) return PF_ERR_ARG;
+    if (!c->pcg_ready) return PF_ERR_STATE;
+    PF_HIP(hipMemcpyAsync(r, c->r + 1, sizeof(double) * c->neq,
                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
     PF_HIP(hipStreamSynchronize(c->stream));
     return PF_OK;

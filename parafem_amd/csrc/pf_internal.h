@@ -28,15 +28,10 @@ enum { PF_Q_PU = 0, PF_Q_RD = 1, PF_Q_RR = 2, PF_Q_MAXX = 3, PF_Q_MAXDX = 4, PF_
 // ntot x ntot matrix:
 //   FP64 "column pairs": e*ntot^2 + (j/2)*2*ntot + 2*i + (j%2)
 //     -> lane i reads columns (j, j+1) as one 16-byte load
-//   FP32 "column quads": e*ntot^2 + (j/4)*4*ntot + 4*i + (j%4)
-//     -> lane i reads columns j..j+3 as one 16-byte load
+//   FP32 hi/lo: packed symmetric (below).
 __host__ __device__ inline long pf_idx_pair(long e, int nt, int i, int j)
 {
     return e * (long)nt * nt + (long)(j >> 1) * 2 * nt + 2 * i + (j & 1);
-}
-__host__ __device__ inline long pf_idx_quad(long e, int nt, int i, int j)
-{
-    return e * (long)nt * nt + (long)(j >> 2) * 4 * nt + 4 * i + (j & 3);
 }
 
 // Packed symmetric storage of the FP32 hi and lo parts.  Row i keeps the
@@ -103,8 +98,9 @@ struct pf_ctx {
     // element matrices (internal layouts above)
     int store = PF_STORE_FP64;
     double *K64 = nullptr;             // FP64 column pairs (PF_STORE_FP64)
-    float *K32hi = nullptr;            // FP32 column quads, hi = fl32(K)
-    float *K32lo = nullptr;            // FP32 column quads, lo = fl32(K - hi)
+    bool k_shared = false;             // K64 holds one matrix used by every element
+    float *K32hi = nullptr;            // packed symmetric FP32, hi = fl32(K)
+    float *K32lo = nullptr;            // packed symmetric FP32, lo = fl32(K - hi)
     bool have_K = false;
     // p123 fixed freedoms: fixstore[eq] = store value, 0 for free equations
     double *fixstore = nullptr;
@@ -123,6 +119,7 @@ struct pf_ctx {
     double *scalar_host = nullptr;     // pinned scratch for primitive results
     pf_pcg_opts opts{};
     bool pcg_ready = false;
+    bool precon_given = false;         // dinv set by pf_precon_set, not 1/diag
 
     // Optional per-kernel timing of pf_pcg_run: one event pair per launch.
     bool timing = false;
@@ -140,6 +137,14 @@ bool pf_timing_begin(pf_ctx *c, int k, hipEvent_t *start, hipEvent_t *stop);
         hipError_t _e = (call);                             \
         if (_e != hipSuccess) return pf_set_hip_error(_e, #call, __FILE__, __LINE__); \
     } while (0)
+
+// FP64 element-matrix entries held (one matrix when shared) and the element
+// stride of the kernels' K64 indexing (0 when shared).
+inline size_t pf_k64_count(const pf_ctx *c)
+{
+    return (size_t)c->ntot * c->ntot * (c->k_shared ? 1 : (size_t)c->nels);
+}
+inline long pf_k64_estride(const pf_ctx *c) { return c->k_shared ? 0 : 1; }
 
 int pf_set_hip_error(hipError_t e, const char *what, const char *file, int line);
 int pf_alloc(pf_ctx *c, void **p, size_t bytes);

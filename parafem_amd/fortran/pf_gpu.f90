@@ -53,7 +53,7 @@ MODULE pf_gpu
 
   PUBLIC :: pf_init, pf_finalize, pf_alloc, pf_release, pf_put, pf_get
   PUBLIC :: pf_mesh, pf_steer, pf_assemble_elastic, pf_assemble_laplace
-  PUBLIC :: pf_storkm, pf_diag, pf_fixed, pf_pcg, pf_check
+  PUBLIC :: pf_storkm, pf_storkm_get, pf_diag, pf_fixed, pf_pcg, pf_check, pcg_km
   PUBLIC :: gather, scatter, scatter_noadd, elem_matvec
   PUBLIC :: dot_product_p, sum_p, checon_par
   PUBLIC :: vec_fill, vec_waxpy, vec_mul, vec_recip
@@ -124,6 +124,27 @@ MODULE pf_gpu
       TYPE(c_ptr), VALUE :: c
       REAL(c_double), INTENT(IN) :: storkm(*)
       INTEGER(c_int), VALUE :: store
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_storkm_get(c, storkm) BIND(C, name="pf_storkm_get")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), INTENT(OUT) :: storkm(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_storkm_shared(c, km) BIND(C, name="pf_storkm_shared")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), INTENT(IN) :: km(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_precon_set(c, dp) BIND(C, name="pf_precon_set")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), INTENT(IN) :: dp(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_pcg_r(c, r, on_device) BIND(C, name="pf_pcg_r")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), INTENT(OUT) :: r(*)
+      INTEGER(c_int), VALUE :: on_device
     END FUNCTION
     INTEGER(c_int) FUNCTION c_diag_get(c, diag) BIND(C, name="pf_diag_get")
       IMPORT :: c_int, c_ptr, c_double
@@ -312,6 +333,41 @@ CONTAINS
     INTEGER, INTENT(IN) :: store
     CALL pf_check(c_storkm_set(ctx, storkm_pp, INT(store, c_int)), 'pf_storkm_set')
   END SUBROUTINE pf_storkm
+
+  !> storkm_pp(ntot,ntot,nels) from the device (FP64, or hi+lo).
+  SUBROUTINE pf_storkm_get(storkm_pp)
+    REAL(iwp), INTENT(OUT) :: storkm_pp(:, :, :)
+    CALL pf_check(c_storkm_get(ctx, storkm_pp), 'pf_storkm_get')
+  END SUBROUTINE pf_storkm_get
+
+  !> PCG_KM(diag_precon_pp, km, limit, nels_pp, r_pp, timest, tol, xnew_pp,
+  !> iters) (src/modules/mpi/maths.f90:1152-1323): every element has the
+  !> matrix km; diag_precon_pp is the inverted Jacobi preconditioner; stops
+  !> by checon_par; r_pp returns the final recurrence residual.  The steering
+  !> must be set (pf_steer) with nels_pp elements; timest is not touched.
+  SUBROUTINE pcg_km(diag_precon_pp, km, limit, nels_pp, r_pp, timest, tol, xnew_pp, iters)
+    REAL(iwp), INTENT(IN) :: diag_precon_pp(:), km(:, :), tol
+    INTEGER, INTENT(IN) :: limit, nels_pp
+    REAL(iwp), INTENT(INOUT) :: r_pp(:), xnew_pp(:), timest(:)
+    INTEGER, INTENT(OUT) :: iters
+    TYPE(pf_pcg_opts) :: o
+    TYPE(pf_pcg_result) :: res
+    INTEGER(c_int) :: rc
+    IF (nels_pp < 1 .OR. SIZE(km, 1) /= SIZE(km, 2)) CALL pf_check(1_c_int, 'PCG_KM arguments')
+    CALL pf_check(c_storkm_shared(ctx, km), 'PCG_KM km')
+    CALL pf_check(c_precon_set(ctx, diag_precon_pp), 'PCG_KM diag_precon_pp')
+    o%limit = limit
+    o%tol = tol
+    o%stop_mode = PF_STOP_CHECON
+    o%precision = PF_PREC_FP64
+    o%check_every = 0
+    o%refine_every = 0
+    o%refine_delta = 0.0_iwp
+    rc = c_pcg_solve(ctx, r_pp, xnew_pp, o, res)
+    IF (rc /= PF_OK .AND. rc /= PF_NOT_CONVERGED) CALL pf_check(rc, 'PCG_KM')
+    CALL pf_check(c_pcg_r(ctx, r_pp, 0_c_int), 'PCG_KM r_pp')
+    iters = res%iters
+  END SUBROUTINE pcg_km
 
   !> Assembled diagonal (diag_precon before inversion, p121.f90:65-69).
   SUBROUTINE pf_diag(diag)

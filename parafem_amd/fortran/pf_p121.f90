@@ -3,7 +3,7 @@
 ! MI355X through the pf_gpu module -- a 3-D elastic solid of 20-node bricks
 ! solved by Jacobi-preconditioned EBE-PCG.
 !
-!   pf_p121 <deck> [dropin|fused|mixed] [relres-tol]
+!   pf_p121 <deck> [dropin|fused|mixed|km] [relres-tol]
 !
 ! Reads <deck>.dat/.d/.bnd/.lds in the formats READ_P121, READ_G_NUM_PP,
 ! READ_G_COORD_PP, READ_REST and READ_LOADS read (src/modules/mpi/
@@ -17,6 +17,8 @@
 !   fused  : the same iteration as one library call (pf_pcg, FP64, checon).
 !   mixed  : FP32-stored element matrices with FP64 vectors and reliable
 !            updates, stopped on the relative residual (default 1e-10).
+!   km     : PCG_KM (maths.f90:1152-1323) with element 1's matrix for all
+!            elements, for uniform meshes and materials.
 !------------------------------------------------------------------------------
 PROGRAM pf_p121
   USE pf_gpu
@@ -30,13 +32,14 @@ PROGRAM pf_p121
   CHARACTER(LEN=15) :: element
   INTEGER, ALLOCATABLE :: g_num(:, :), rest(:, :), nf(:, :), g_g_pp(:, :), node(:)
   REAL(iwp), ALLOCATABLE :: coord(:, :), g_coord_pp(:, :, :), val(:, :), r(:), diag(:), x(:)
+  REAL(iwp), ALLOCATABLE :: storkm(:, :, :), timest(:)
   INTEGER, ALLOCATABLE :: row(:)
   TYPE(pf_dvec) :: p_pp, r_pp, x_pp, xnew_pp, u_pp, d_pp, diag_precon_pp, pmul_pp, utemp_pp
   INTEGER(8) :: t0, t1, rate
 
   CALL GET_COMMAND_ARGUMENT(1, argv, nlen)
   IF (nlen == 0) THEN
-    WRITE(*, '(A)') 'usage: pf_p121 <deck> [dropin|fused|mixed] [relres-tol]'
+    WRITE(*, '(A)') 'usage: pf_p121 <deck> [dropin|fused|mixed|km] [relres-tol]'
     STOP 2
   END IF
   mode = 'dropin'
@@ -46,7 +49,7 @@ PROGRAM pf_p121
     CALL GET_COMMAND_ARGUMENT(3, word)
     READ(word, *) rtol
   END IF
-  IF (mode /= 'dropin' .AND. mode /= 'fused' .AND. mode /= 'mixed') THEN
+  IF (mode /= 'dropin' .AND. mode /= 'fused' .AND. mode /= 'mixed' .AND. mode /= 'km') THEN
     WRITE(*, '(A,A)') 'pf_p121: unknown mode ', TRIM(mode)
     STOP 2
   END IF
@@ -201,6 +204,16 @@ PROGRAM pf_p121
     CALL pf_get(xnew_pp, x)
   ELSE IF (mode == 'fused') THEN
     CALL pf_pcg(r, x, limit, tol, PF_STOP_CHECON, PF_PREC_FP64, iters, converged)
+  ELSE IF (mode == 'km') THEN
+    ! PCG_KM with element 1's matrix for every element (a uniform mesh and
+    ! material, as PCG_KM assumes) and the inverted assembled diagonal.
+    ALLOCATE(storkm(ntot, ntot, nels), timest(20))
+    CALL pf_storkm_get(storkm)
+    diag = 1.0_iwp / diag
+    x = 0.0_iwp
+    CALL pcg_km(diag, storkm(:, :, 1), limit, nels, r, timest, tol, x, iters)
+    converged = iters < limit
+    DEALLOCATE(storkm, timest)
   ELSE
     CALL pf_pcg(r, x, 100000, rtol, PF_STOP_RELRES, PF_PREC_MIXED, iters, converged)
   END IF

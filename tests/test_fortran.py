@@ -88,7 +88,7 @@ def test_fortran_dropin_equals_fused_xx3_tiny(tmp_path, golden):
 def test_fortran_generated_abaqus_deck_and_mixed(tmp_path):
     """A 6^3 hex20 cube deck in Abaqus node order (ABAQUS2SG applied on
     read): dropin and fused bit-identical; the mixed-precision solve within
-    1e-8 of the FP64 solution."""
+    1e-8 of the FP64 solution; PCG_KM bit-identical to the oracle."""
     pr = P.elastic_cube(6)
     rest = M.base_fixed_rest(pr.mesh)
     nodes, vals = M.top_pressure_loads(pr.mesh, -1.0)
@@ -102,3 +102,18 @@ def test_fortran_generated_abaqus_deck_and_mixed(tmp_path):
     say("pf_p121 mixed cube6")
     xm, im, _ = run(deck, "mixed", "1e-11")
     assert np.abs(xm - xa).max() <= 1e-8 * np.abs(xa).max()
+    # PCG_KM with element 1's matrix for all elements and the driver's
+    # inverted assembled diagonal: bit-identical to the oracle doing the same
+    import oracle as O
+    say("pf_p121 km cube6")
+    xk, ik, _ = run(deck, "km")
+    d = M.read_deck(deck)
+    nf, neq = M.steering(d["nn"], 3, d["rest"])
+    g_g = M.g_g_from(nf, d["g_num"])
+    gc = M.element_coords(d["g_num"], d["coords"])
+    r = M.nodal_loads_to_rhs(nf, d["load_node"], d["load_val"], neq)
+    Ko = O.assemble_elastic(gc, d["e"], d["v"])
+    full = np.repeat(Ko[:1], Ko.shape[0], axis=0)
+    xo, it, _ = O.pcg(g_g, full, r, O.diag(g_g, Ko, neq), d["tol"], d["limit"], dot_order=O.DOT_DEVICE)
+    assert ik == it and np.array_equal(xk, xo)
+    assert np.abs(xk - xa).max() <= 1e-6 * np.abs(xa).max()

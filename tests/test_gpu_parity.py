@@ -343,3 +343,33 @@ def test_mixed_precision_laplace_penalty(ctx):
     xm, rm = P.solve(ctx, pr, 1e-10, 20000, stop="relres", precision="mixed")
     assert rm.converged
     assert np.abs(xm - x64).max() <= 1e-8 * np.abs(x64).max()
+
+
+def test_pcg_km_shared_matrix_bitexact(ctx):
+    """PCG_KM (maths.f90:1152-1323): one km for every element gives the same
+    iteration count and a bit-identical x as the oracle with km repeated; a
+    caller's diag_precon (powers of two, so 1/(1/d) is exact) is used as
+    given; r_pp comes back as the recurrence residual."""
+    pr = P.elastic_cube(4)
+    say(f"pcg_km shared matrix neq={pr.neq}")
+    Ko = O.assemble_elastic(pr.g_coord(), pr.e, pr.v)
+    kb = Ko[0].copy()
+    full = np.repeat(kb[None], pr.nels, axis=0)
+    ctx.mesh_set(pr.g_coord())
+    ctx.steer_set(pr.g_g, pr.neq)
+    ctx.storkm_shared(kb)
+    dg = O.diag(pr.g_g, full, pr.neq)
+    assert np.array_equal(ctx.diag_get(), dg)
+    assert np.array_equal(ctx.storkm_get(), full)
+    x, res = ctx.pcg_solve(pr.r0, 3000, 1e-8, stop="checon")
+    xo, it, conv = O.pcg(pr.g_g, full, pr.r0, dg, 1e-8, 3000, dot_order=O.DOT_DEVICE)
+    assert res.converged and conv and res.iters == it
+    assert np.array_equal(x, xo)
+    rng = np.random.default_rng(2)
+    dinv = 2.0 ** rng.integers(-9, -5, size=pr.neq).astype(np.float64)
+    ctx.precon_set(dinv)
+    x2, res2 = ctx.pcg_solve(pr.r0, 3000, 1e-8, stop="checon")
+    xo2, it2, _ = O.pcg(pr.g_g, full, pr.r0, 1.0 / dinv, 1e-8, 3000, dot_order=O.DOT_DEVICE)
+    assert res2.iters == it2 and np.array_equal(x2, xo2)
+    r = ctx.pcg_r()
+    assert np.linalg.norm(r - (pr.r0 - ctx.apply(x2))) <= 1e-8 * np.linalg.norm(pr.r0)
