@@ -94,6 +94,19 @@ def solve(ctx, pr, precision, store, tol):
                "solves_per_s": round(1.0 / t_solve, 5), "true_relres": true_rel}
 
 
+def profiled_traffic(workload, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload (calibrated against a
+    plain copy), or None when there is none."""
+    path = os.path.join(HERE, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            entry = json.load(fh)[workload]
+        return entry["bytes_per_dispatch"].get(kernel), entry["source"]
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def build_problem(workload, n):
     from parafem_amd import problems as P
     if workload == "hex20":
@@ -224,6 +237,10 @@ def main():
         log(f"cpu baseline {cpu['value']:.3f} iter/s ({cpu['cores']} threads)")
 
     head = paths[args.precision]
+    mixed = args.precision == "mixed"
+    kname = f"k_ebe<{pr.ntot}, {1 if mixed else 0}, true, {'float' if mixed else 'double'}>"
+    traffic, tsource = profiled_traffic(args.workload, kname) if args.n == (100 if args.workload == "hex20" else 200) \
+        else (None, None)
     out = {
         "metric": METRIC,
         "value": head["iter_per_s"],
@@ -252,12 +269,14 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_ebe<{pr.ntot}> element products (" + ("FP32 hi, packed symmetric" if args.precision == "mixed" else "FP64") + ")",
+            "kernel": kname + " element products (" + ("FP32 hi, packed symmetric" if mixed else "FP64") + ")",
             "achieved": head["ebe_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(head["ebe_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": f"profiles/{tsource}" if tsource else None,
+            "algorithmic_bytes": pr.nels * head["bytes_per_element"],
             "bytes_per_element": head["bytes_per_element"],
         },
         "cpu_baseline": cpu,
