@@ -149,13 +149,80 @@ def cpu_baseline(workload: str, n_sample: int, nels_full: int, budget_s: float =
     }
 
 
+def serve_bench(args):
+    """The serving loop on a request stream: closed-loop capacity, then the
+    stream at a fixed Poisson rate through the mixed-precision server, then
+    the same stream one request at a time on the plain FP64 path."""
+    from parafem_amd import serve as S
+    cache = S.CubeCache(keep=64)
+    closed = S.make_stream(args.requests, 0.0, seed=4, nmin=args.nmin, nmax=args.nmax)
+    for r in closed:
+        cache.get(r.n)
+    log(f"serve: {args.requests} requests, n in [{args.nmin},{args.nmax}], sizes cached")
+    srv = S.Server(slots=args.slots, cache=cache)
+    try:
+        # warm-up: compile/capture paths once
+        srv.serve(S.make_stream(args.slots, 0.0, seed=5, nmin=args.nmin, nmax=args.nmin + 2))
+        t0 = time.perf_counter()
+        res_closed = srv.serve(closed)
+        cap = S.summarize(res_closed, time.perf_counter() - t0)
+        log(f"closed loop: {cap}")
+        rate = args.rate if args.rate > 0 else 0.8 * cap["solves_per_s"]
+        stream = S.make_stream(args.requests, rate, seed=4, nmin=args.nmin, nmax=args.nmax)
+        t0 = time.perf_counter()
+        res_mix = srv.serve(stream)
+        mix = S.summarize(res_mix, time.perf_counter() - t0)
+        log(f"stream (mixed server, {args.slots} slots): {mix}")
+    finally:
+        srv.close()
+    t0 = time.perf_counter()
+    res_seq = S.serve_sequential(stream, precision="fp64", cache=cache)
+    seq = S.summarize(res_seq, time.perf_counter() - t0)
+    log(f"stream (FP64 one by one): {seq}")
+    out = {
+        "metric": METRIC,
+        "value": round(mix["solves_per_s"], 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.requests,
+        "warmup": args.slots,
+        "ms_per_step": round(1e3 / mix["solves_per_s"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"serving stream: {args.requests} hex20 elastic cube requests, n log-uniform in "
+                        f"[{args.nmin},{args.nmax}], E U[1,200], nu U[0.2,0.4], pressure U[0.1,10] (seed 4), "
+                        f"Poisson arrivals at {rate:.2f}/s, mixed-precision PCG to relres {1e-10}, "
+                        f"{args.slots} slots",
+            "rate_per_s": rate,
+            "closed_loop": cap,
+            "server": mix,
+            "fp64_one_by_one": seq,
+            "p99_speedup_vs_fp64": round(seq["p99_latency_s"] / mix["p99_latency_s"], 3),
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="hex20", choices=["hex20", "hex8"],
-                    help="hex20: p121 elastic cube (default, 100^3); hex8: p123 Laplace cube (200^3)")
+    ap.add_argument("--workload", default="hex20", choices=["hex20", "hex8", "serve"],
+                    help="hex20: p121 elastic cube (default, 100^3); hex8: p123 Laplace cube (200^3); "
+                         "serve: a stream of hex20 cube requests through the serving loop")
+    ap.add_argument("--requests", type=int, default=200, help="serve: stream length")
+    ap.add_argument("--nmin", type=int, default=10, help="serve: smallest cube edge")
+    ap.add_argument("--nmax", type=int, default=40, help="serve: largest cube edge")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="serve: Poisson arrival rate (0 = 80%% of the measured closed-loop capacity)")
+    ap.add_argument("--slots", type=int, default=8, help="serve: concurrent solves")
     ap.add_argument("--n", type=int, default=0, help="elements per cube edge (0 = the workload's size)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
                     help="path whose iteration rate is the headline value")
@@ -183,6 +250,9 @@ def main():
         print(json.dumps({"copy_bytes": n * 8, "launches": 5}), flush=True)
         return
 
+    if args.workload == "serve":
+        serve_bench(args)
+        return
     if args.n == 0:
         args.n = 100 if args.workload == "hex20" else 200
     t0 = time.perf_counter()

@@ -95,6 +95,10 @@ const char *pf_status_string(int status);
 int pf_create(int device, pf_ctx **ctx);
 /* Replaces SHUTDOWN (mp_interface.f90:97-106); frees all device state. */
 int pf_destroy(pf_ctx *ctx);
+/* Free the current problem's device buffers (mesh, maps, matrices,
+ * vectors) and keep the context for another problem (the DEALLOCATE
+ * statements at the end of p121.f90:111-123). */
+int pf_reset(pf_ctx *ctx);
 /* Order the context's work on a caller's hipStream_t (NULL = the context's
  * own stream).  The stream must outlive its use by the context. */
 int pf_set_stream(pf_ctx *ctx, void *hip_stream);
@@ -207,6 +211,15 @@ int pf_pcg_x(pf_ctx *ctx, double *x, int x_on_device);
 /* The recurrence residual r at the end of the solve (PCG_KM's r_pp is
  * INOUT, maths.f90:1164). */
 int pf_pcg_r(pf_ctx *ctx, double *r, int r_on_device);
+/* No counterpart in the reference (serving many solves from one host
+ * thread).  pf_pcg_run_graph: as pf_pcg_run, but the n iterations are
+ * captured once into a HIP graph and replayed (same kernels, same results);
+ * the capture is redone when n, the precision or the stopping rule change.
+ * pf_pcg_query: non-blocking poll -- the first call after work was enqueued
+ * snapshots the PCG state at that point of the stream; *ready becomes 1 (and
+ * res is filled) once the snapshot has arrived. */
+int pf_pcg_run_graph(pf_ctx *ctx, int n_iters);
+int pf_pcg_query(pf_ctx *ctx, pf_pcg_result *res, int *ready);
 /* y = K x over the assembled operator in FP64 (host vectors, neq), with the
  * p123 fixed-freedom rows applied.  Used to report true residuals. */
 int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);

@@ -107,6 +107,9 @@ def lib():
         "pf_storkm_shared": (i32, [vp, pd]),
         "pf_precon_set": (i32, [vp, pd]),
         "pf_pcg_r": (i32, [vp, pd, i32]),
+        "pf_pcg_run_graph": (i32, [vp, i32]),
+        "pf_pcg_query": (i32, [vp, vp, pi]),
+        "pf_reset": (i32, [vp]),
         "pf_vec_fill": (i32, [vp, vp, dbl, i64]),
         "pf_vec_waxpy": (i32, [vp, vp, vp, vp, dbl, i64]),
         "pf_vec_mul": (i32, [vp, vp, vp, vp, i64]),
@@ -437,6 +440,23 @@ class Context:
         x = np.empty(self.neq)
         _check(lib().pf_pcg_x(self.h, x.ctypes.data, 0), "pf_pcg_x")
         return x
+
+    def pcg_run_graph(self, n: int):
+        """Enqueue n iterations as one replayed HIP graph (no host sync)."""
+        _check(lib().pf_pcg_run_graph(self.h, int(n)), "pf_pcg_run_graph")
+
+    def pcg_query(self):
+        """Non-blocking poll: a PcgResult once the state snapshot taken at
+        the first call after the last enqueued work has arrived, else None."""
+        res = PcgResult()
+        ready = C.c_int(0)
+        _check(lib().pf_pcg_query(self.h, C.byref(res), C.byref(ready)), "pf_pcg_query")
+        return res if ready.value else None
+
+    def reset(self):
+        """Free the current problem's device buffers; keep the context."""
+        _check(lib().pf_reset(self.h), "pf_reset")
+        self.nels = self.ntot = self.neq = self.nod = 0
 
     def pcg_r(self) -> np.ndarray:
         """Recurrence residual at the end of the solve."""

@@ -80,8 +80,18 @@ int pf_grid_for(int64_t n)
 
 static const size_t kPartials = (size_t)PF_NQ * PF_REDUCE_MAX_GRID + 8;
 
+void pf_graph_drop(pf_ctx *c)
+{
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_iters = 0;
+}
+
 static void free_problem_vectors(pf_ctx *c)
 {
+    pf_graph_drop(c);
     size_t nv = (size_t)c->neq + 1;
     free_arr(c, c->x, nv); free_arr(c, c->r, nv); free_arr(c, c->p, nv);
     free_arr(c, c->d, nv); free_arr(c, c->u, nv); free_arr(c, c->dinv, nv);
@@ -93,6 +103,7 @@ static void free_problem_vectors(pf_ctx *c)
 
 static void free_matrices(pf_ctx *c)
 {
+    pf_graph_drop(c);
     const size_t ns = (size_t)pf_sym_per(c->ntot) * c->nels;
     free_arr(c, c->K64, pf_k64_count(c));
     free_arr(c, c->K32hi, ns);
@@ -164,6 +175,8 @@ int pf_create(int device, pf_ctx **out)
     if (!rc) rc = pf_alloc(c, (void **)&c->partials, sizeof(double) * kPartials);
     if (!rc && hipHostMalloc((void **)&c->state_host, sizeof(pf_state), hipHostMallocDefault) != hipSuccess) rc = PF_ERR_OOM;
     if (!rc && hipHostMalloc((void **)&c->scalar_host, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = PF_ERR_OOM;
+    if (!rc && hipHostMalloc((void **)&c->state_query, sizeof(pf_state), hipHostMallocDefault) != hipSuccess) rc = PF_ERR_OOM;
+    if (!rc && hipEventCreateWithFlags(&c->ev_query, hipEventDisableTiming) != hipSuccess) rc = PF_ERR_HIP;
     if (!rc) {
         hipError_t z = hipMemsetAsync(c->state, 0, sizeof(pf_state), c->stream);
         if (z == hipSuccess) z = hipStreamSynchronize(c->stream);
@@ -187,9 +200,21 @@ int pf_destroy(pf_ctx *c)
     pf_free(c, c->state, sizeof(pf_state));
     if (c->state_host) (void)hipHostFree(c->state_host);
     if (c->scalar_host) (void)hipHostFree(c->scalar_host);
+    if (c->state_query) (void)hipHostFree(c->state_query);
+    if (c->ev_query) (void)hipEventDestroy(c->ev_query);
+    pf_graph_drop(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
+    return PF_OK;
+}
+
+int pf_reset(pf_ctx *c)
+{
+    if (!c) return PF_ERR_ARG;
+    PF_HIP(hipSetDevice(c->device));
+    reset_problem(c);
+    c->query_pending = false;
     return PF_OK;
 }
 
@@ -519,6 +544,7 @@ int pf_fixed_set(pf_ctx *c, int nfix, const int *no_f, double penalty, double *s
     if (!c->have_K) return PF_ERR_STATE;
     for (int i = 0; i < nfix; ++i)
         if (no_f[i] < 1 || no_f[i] > c->neq) return PF_ERR_ARG;
+    pf_graph_drop(c);
     PF_HIP(hipSetDevice(c->device));
     size_t nv = (size_t)c->neq + 1;
     free_arr(c, c->fixstore, nv);

@@ -126,11 +126,14 @@ __device__ void fold_partials(double (&out)[NQ], const int (&qid)[NQ], const boo
     }
 }
 
-// Width of the lane group serving one element, and elements per wavefront.
-template <int NT> struct Lanes {
+// Width of the lane group serving one element, elements per wavefront, and
+// elements per lane group (EPL: a group loads EPL element matrices before it
+// multiplies, to keep enough bytes in flight when an element is small).
+template <int NT, int KIND> struct Lanes {
     static constexpr int W = NT > 32 ? 64 : (NT > 16 ? 32 : (NT > 8 ? 16 : 8));
     static constexpr int EPW = 64 / W;
-    static constexpr int EPB = EPW * (PF_BLOCK / 64);   // elements per block
+    static constexpr int EPL = NT > 32 ? 1 : NT > 8 ? (KIND == 1 ? 2 : 1) : (KIND == 1 ? 8 : 4);
+    static constexpr int EPB = EPW * EPL * (PF_BLOCK / 64);   // elements per block
 };
 
 typedef float f4_t __attribute__((ext_vector_type(4)));
@@ -164,95 +167,132 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
                                                  long kstride, const pf_state *__restrict__ st, int gate)
 {
     if (skip(st, gate)) return;
-    using L = Lanes<NT>;
+    using L = Lanes<NT, KIND>;
+    constexpr int EPL = L::EPL;
     constexpr int D = pf_sym_diags(NT), Q = D / 4, R = D - 4 * Q;
-    __shared__ __attribute__((aligned(16))) double ps[PF_BLOCK / 64][128];
+    // Per wave and element slot: p_e (FP64), or p_e twice (packed kinds).
+    __shared__ __attribute__((aligned(16))) double ps[PF_BLOCK / 64][EPL][KIND == KIND_F32 ? 64 : 128];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sub = lane / L::W, li = lane % L::W;
-    const long e = ((long)blockIdx.x * (PF_BLOCK / 64) + wave) * L::EPW + sub;
-    const long ec = e < nels ? e : (long)nels - 1;
     const int i = li < NT ? li : NT - 1;
-    const int g = GATHER ? gg[ec * NT + i] : 0;
     const int gb = sub * L::W;   // first lane of the element's group
+    const long ebase = ((long)blockIdx.x * (PF_BLOCK / 64) + wave) * L::EPW * EPL;
+    long ev[EPL], ec[EPL];
+    int g[EPL];
+#pragma unroll
+    for (int t = 0; t < EPL; ++t) {
+        ev[t] = ebase + t * L::EPW + sub;
+        ec[t] = ev[t] < nels ? ev[t] : (long)nels - 1;
+        g[t] = GATHER ? gg[ec[t] * NT + i] : 0;
+    }
     if constexpr (KIND == KIND_F64) {
         // Every column pair of K_e is in flight before the dependent gather.
-        const double *pw = &ps[wave][sub * L::W];
-        const d2_t *Kp = reinterpret_cast<const d2_t *>(K64) + ec * kstride * (NT * NT / 2) + i;
-        d2_t kc[NT / 2];
+        d2_t kc[EPL][NT / 2];
 #pragma unroll
-        for (int j = 0; j < NT / 2; ++j) kc[j] = __builtin_nontemporal_load(&Kp[j * NT]);
-        ps[wave][lane] = GATHER ? src[g] : src[ec * NT + i];
+        for (int t = 0; t < EPL; ++t) {
+            const d2_t *Kp = reinterpret_cast<const d2_t *>(K64) + ec[t] * kstride * (NT * NT / 2) + i;
+#pragma unroll
+            for (int j = 0; j < NT / 2; ++j) kc[t][j] = __builtin_nontemporal_load(&Kp[j * NT]);
+        }
+#pragma unroll
+        for (int t = 0; t < EPL; ++t) ps[wave][t][lane] = GATHER ? src[g[t]] : src[ec[t] * NT + i];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < NT / 2; ++j) {
-            acc = __fma_rn(kc[j].x, pw[2 * j], acc);
-            acc = __fma_rn(kc[j].y, pw[2 * j + 1], acc);
+        for (int t = 0; t < EPL; ++t) {
+            const double *pw = &ps[wave][t][gb];
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < NT / 2; ++j) {
+                acc = __fma_rn(kc[t][j].x, pw[2 * j], acc);
+                acc = __fma_rn(kc[t][j].y, pw[2 * j + 1], acc);
+            }
+            if (ev[t] < nels && li < NT) out[ev[t] * NT + li] = (OUT)acc;
         }
-        if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
     } else if constexpr (KIND == KIND_F32) {
-        const float *Ke = Khi + ec * (long)(D * NT);
-        float kd[D];
+        float kd[EPL][D];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const f4_t v = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Ke) + q * NT + i);
-            kd[4 * q] = v.x; kd[4 * q + 1] = v.y; kd[4 * q + 2] = v.z; kd[4 * q + 3] = v.w;
+        for (int t = 0; t < EPL; ++t) {
+            const float *Ke = Khi + ec[t] * (long)(D * NT);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const f4_t v = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Ke) + q * NT + i);
+                kd[t][4 * q] = v.x; kd[t][4 * q + 1] = v.y; kd[t][4 * q + 2] = v.z; kd[t][4 * q + 3] = v.w;
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) kd[t][4 * Q + r] = __builtin_nontemporal_load(Ke + 4 * Q * NT + R * i + r);
         }
+        float pv[EPL];
 #pragma unroll
-        for (int r = 0; r < R; ++r) kd[4 * Q + r] = __builtin_nontemporal_load(Ke + 4 * Q * NT + R * i + r);
-        const float pv = (float)(GATHER ? src[g] : src[ec * NT + i]);
-        float *pg = reinterpret_cast<float *>(&ps[wave][0]) + 2 * gb;
-        if (li < NT) {
-            pg[li] = pv;
-            pg[li + NT] = pv;
+        for (int t = 0; t < EPL; ++t) {
+            pv[t] = (float)(GATHER ? src[g[t]] : src[ec[t] * NT + i]);
+            float *pg = reinterpret_cast<float *>(&ps[wave][t][0]) + 2 * gb;
+            if (li < NT) {
+                pg[li] = pv[t];
+                pg[li + NT] = pv[t];
+            }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        float acc = 0.0f;
 #pragma unroll
-        for (int d = 0; d < D; ++d) acc = __fmaf_rn(kd[d], pg[i + d], acc);
+        for (int t = 0; t < EPL; ++t) {
+            const float *pg = reinterpret_cast<const float *>(&ps[wave][t][0]) + 2 * gb;
+            float acc = 0.0f;
 #pragma unroll
-        for (int d = 1; d < D - 1; ++d) {
-            const int from = gb + (i >= d ? i - d : i - d + NT);
-            const float t = __fmul_rn(kd[d], pv);
-            acc = __fadd_rn(acc, __int_as_float(__builtin_amdgcn_ds_bpermute(from << 2, __float_as_int(t))));
+            for (int d = 0; d < D; ++d) acc = __fmaf_rn(kd[t][d], pg[i + d], acc);
+#pragma unroll
+            for (int d = 1; d < D - 1; ++d) {
+                const int from = gb + (i >= d ? i - d : i - d + NT);
+                const float tt = __fmul_rn(kd[t][d], pv[t]);
+                acc = __fadd_rn(acc, __int_as_float(__builtin_amdgcn_ds_bpermute(from << 2, __float_as_int(tt))));
+            }
+            if (ev[t] < nels && li < NT) out[ev[t] * NT + li] = (OUT)acc;
         }
-        if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
     } else {
-        const float *Kh = Khi + ec * (long)(D * NT);
-        const float *Kl = Klo + ec * (long)(D * NT);
-        double kd[D];
+        double kd[EPL][D];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const f4_t h = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Kh) + q * NT + i);
-            const f4_t l = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Kl) + q * NT + i);
-            kd[4 * q] = (double)h.x + (double)l.x;
-            kd[4 * q + 1] = (double)h.y + (double)l.y;
-            kd[4 * q + 2] = (double)h.z + (double)l.z;
-            kd[4 * q + 3] = (double)h.w + (double)l.w;
+        for (int t = 0; t < EPL; ++t) {
+            const float *Kh = Khi + ec[t] * (long)(D * NT);
+            const float *Kl = Klo + ec[t] * (long)(D * NT);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const f4_t h = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Kh) + q * NT + i);
+                const f4_t l = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Kl) + q * NT + i);
+                kd[t][4 * q] = (double)h.x + (double)l.x;
+                kd[t][4 * q + 1] = (double)h.y + (double)l.y;
+                kd[t][4 * q + 2] = (double)h.z + (double)l.z;
+                kd[t][4 * q + 3] = (double)h.w + (double)l.w;
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                kd[t][4 * Q + r] = (double)__builtin_nontemporal_load(Kh + 4 * Q * NT + R * i + r) +
+                                   (double)__builtin_nontemporal_load(Kl + 4 * Q * NT + R * i + r);
         }
+        double pv[EPL];
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            kd[4 * Q + r] = (double)__builtin_nontemporal_load(Kh + 4 * Q * NT + R * i + r) +
-                            (double)__builtin_nontemporal_load(Kl + 4 * Q * NT + R * i + r);
-        const double pv = GATHER ? src[g] : src[ec * NT + i];
-        double *pg = &ps[wave][0] + 2 * gb;
-        if (li < NT) {
-            pg[li] = pv;
-            pg[li + NT] = pv;
+        for (int t = 0; t < EPL; ++t) {
+            pv[t] = GATHER ? src[g[t]] : src[ec[t] * NT + i];
+            double *pg = &ps[wave][t][0] + 2 * gb;
+            if (li < NT) {
+                pg[li] = pv[t];
+                pg[li + NT] = pv[t];
+            }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double acc = 0.0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) acc = __fma_rn(kd[d], pg[i + d], acc);
+        for (int t = 0; t < EPL; ++t) {
+            const double *pg = &ps[wave][t][0] + 2 * gb;
+            double acc = 0.0;
 #pragma unroll
-        for (int d = 1; d < D - 1; ++d) {
-            const int from = gb + (i >= d ? i - d : i - d + NT);
-            acc = __dadd_rn(acc, bpermute64(from << 2, __dmul_rn(kd[d], pv)));
+            for (int d = 0; d < D; ++d) acc = __fma_rn(kd[t][d], pg[i + d], acc);
+#pragma unroll
+            for (int d = 1; d < D - 1; ++d) {
+                const int from = gb + (i >= d ? i - d : i - d + NT);
+                acc = __dadd_rn(acc, bpermute64(from << 2, __dmul_rn(kd[t][d], pv[t])));
+            }
+            if (ev[t] < nels && li < NT) out[ev[t] * NT + li] = (OUT)acc;
         }
-        if (e < nels && li < NT) out[e * NT + li] = (OUT)acc;
     }
 }
 
@@ -611,7 +651,7 @@ __global__ void k_scatter_fix(const int *__restrict__ ptr, const int *__restrict
 template <int NT, int KIND, bool GATHER, typename OUT>
 int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate)
 {
-    const int grid = (c->nels + Lanes<NT>::EPB - 1) / Lanes<NT>::EPB;
+    const int grid = (c->nels + Lanes<NT, KIND>::EPB - 1) / Lanes<NT, KIND>::EPB;
     hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64, c->K32hi,
                        c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate);
     PF_HIP(hipGetLastError());
@@ -791,6 +831,7 @@ int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opt
     // while the copy above is still in flight.
     PF_HIP(hipStreamSynchronize(c->stream));
     c->pcg_ready = true;
+    c->query_pending = false;
     return PF_OK;
 }
 
@@ -857,6 +898,67 @@ int pf_pcg_run(pf_ctx *c, int n)
     return PF_OK;
 }
 
+int pf_pcg_run_graph(pf_ctx *c, int n)
+{
+    if (!c || n < 1) return PF_ERR_ARG;
+    if (!c->pcg_ready) return PF_ERR_STATE;
+    if (c->timing) return pf_pcg_run(c, n);   // timing records an event per launch
+    if (!c->graph_exec || c->graph_iters != n || c->graph_precision != c->opts.precision ||
+        c->graph_stop != c->opts.stop_mode) {
+        pf_graph_drop(c);
+        PF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = pf_pcg_run(c, n);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : pf_set_hip_error(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        }
+        const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            c->graph_exec = nullptr;
+            return pf_set_hip_error(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+        c->graph = g;
+        c->graph_iters = n;
+        c->graph_precision = c->opts.precision;
+        c->graph_stop = c->opts.stop_mode;
+    }
+    PF_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    return PF_OK;
+}
+
+static void fill_result(const pf_state &s, pf_pcg_result *res)
+{
+    res->iters = s.iters;
+    res->converged = s.converged;
+    res->crit = s.crit;
+    res->rr0 = s.rr0;
+    res->rr = s.rr;
+    res->refinements = s.refinements;
+    res->status = s.breakdown ? PF_ERR_BREAKDOWN : (s.converged ? PF_OK : PF_NOT_CONVERGED);
+}
+
+int pf_pcg_query(pf_ctx *c, pf_pcg_result *res, int *ready)
+{
+    if (!c || !res || !ready) return PF_ERR_ARG;
+    if (!c->pcg_ready) return PF_ERR_STATE;
+    *ready = 0;
+    if (!c->query_pending) {
+        PF_HIP(hipMemcpyAsync(c->state_query, c->state, sizeof(pf_state), hipMemcpyDeviceToHost, c->stream));
+        PF_HIP(hipEventRecord(c->ev_query, c->stream));
+        c->query_pending = true;
+    }
+    const hipError_t q = hipEventQuery(c->ev_query);
+    if (q == hipErrorNotReady) return PF_OK;
+    if (q != hipSuccess) return pf_set_hip_error(q, "hipEventQuery", __FILE__, __LINE__);
+    c->query_pending = false;
+    fill_result(*c->state_query, res);
+    *ready = 1;
+    return PF_OK;
+}
+
 int pf_pcg_timing_enable(pf_ctx *c, int on)
 {
     if (!c) return PF_ERR_ARG;
@@ -890,14 +992,7 @@ int pf_pcg_poll(pf_ctx *c, pf_pcg_result *res)
     if (!c->pcg_ready) return PF_ERR_STATE;
     PF_HIP(hipMemcpyAsync(c->state_host, c->state, sizeof(pf_state), hipMemcpyDeviceToHost, c->stream));
     PF_HIP(hipStreamSynchronize(c->stream));
-    const pf_state &s = *c->state_host;
-    res->iters = s.iters;
-    res->converged = s.converged;
-    res->crit = s.crit;
-    res->rr0 = s.rr0;
-    res->rr = s.rr;
-    res->refinements = s.refinements;
-    res->status = s.breakdown ? PF_ERR_BREAKDOWN : (s.converged ? PF_OK : PF_NOT_CONVERGED);
+    fill_result(*c->state_host, res);
     return PF_OK;
 }
 

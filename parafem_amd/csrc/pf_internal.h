@@ -120,6 +120,14 @@ struct pf_ctx {
     pf_pcg_opts opts{};
     bool pcg_ready = false;
     bool precon_given = false;         // dinv set by pf_precon_set, not 1/diag
+    // Non-blocking state read-back (pf_pcg_query).
+    pf_state *state_query = nullptr;   // pinned
+    hipEvent_t ev_query = nullptr;
+    bool query_pending = false;
+    // Captured iteration chunk (pf_pcg_run_graph) and what it was captured for.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_iters = 0, graph_precision = -1, graph_stop = -1;
 
     // Optional per-kernel timing of pf_pcg_run: one event pair per launch.
     bool timing = false;
@@ -127,6 +135,9 @@ struct pf_ctx {
     std::vector<std::pair<int, int>> ev_used[PF_NKERNELS];   // (start, stop) indices
     size_t ev_next = 0;
 };
+
+// Drop a captured iteration chunk (problem, options or buffers changed).
+void pf_graph_drop(pf_ctx *c);
 
 // Event pair handles for timing kernel `k` (returns false when timing is off).
 bool pf_timing_begin(pf_ctx *c, int k, hipEvent_t *start, hipEvent_t *stop);

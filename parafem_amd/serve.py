@@ -1,0 +1,250 @@
+"""A request loop that serves many EBE-PCG solves of uneven size on one GPU.
+
+Each request is a p121-style hex20 elastic cube (n x n x n elements, unit
+edges, base fixed, uniform pressure on the top face) with its own E, nu and
+pressure.  The loop admits arrived requests in arrival order while a context
+("slot") and the HBM budget allow, builds the problem on that slot (mesh and
+steering upload, element matrices assembled on the device), and then drives
+every admitted solve from one host thread: each slot always has one chunk of
+PCG iterations (a replayed HIP graph) queued behind a non-blocking state
+snapshot, so the GPU runs the solves of all slots concurrently on their own
+streams while the host only polls.  A finished solve's displacements are read
+back, its device buffers freed, and the slot takes the next request.
+
+The arithmetic of a solve does not depend on what else runs: every kernel is
+deterministic and a context's state is its own, so a result is bit-identical
+alone and in the mix.
+
+Stream generator (``make_stream``): numpy ``default_rng(seed)``; arrival gaps
+exponential with the given rate (the first request arrives at 0); n =
+round(exp(U(ln nmin, ln nmax))); E ~ U[1,200]; nu ~ U[0.2,0.4]; pressure ~
+U[0.1,10].  The load is pressure times the unit-pressure consistent nodal
+load of the top face.
+"""
+from __future__ import annotations
+
+import time
+from collections import OrderedDict, deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import mesh as M
+from .api import Context
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float      # seconds after the start of the stream
+    n: int
+    e: float
+    v: float
+    pressure: float
+
+
+@dataclass
+class Result:
+    rid: int
+    n: int
+    neq: int
+    iters: int
+    converged: bool
+    refinements: int
+    t_admit: float
+    t_done: float
+    latency: float
+    x: np.ndarray | None = None
+    checksum: float = 0.0
+
+
+def make_stream(count: int, rate: float, seed: int = 4, nmin: int = 10, nmax: int = 80) -> list[Request]:
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, count) if rate > 0 else np.zeros(count)
+    arrival = np.concatenate([[0.0], np.cumsum(gaps[1:])])
+    n = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), count))).astype(int)
+    e = rng.uniform(1.0, 200.0, count)
+    v = rng.uniform(0.2, 0.4, count)
+    pr = rng.uniform(0.1, 10.0, count)
+    return [Request(i, float(arrival[i]), int(n[i]), float(e[i]), float(v[i]), float(pr[i])) for i in range(count)]
+
+
+class CubeCache:
+    """Host-side mesh, steering and unit load of the n^3 cube, kept for the
+    most recently used sizes."""
+
+    def __init__(self, keep: int = 24):
+        self.keep = keep
+        self._d: OrderedDict[int, dict] = OrderedDict()
+
+    def get(self, n: int) -> dict:
+        if n in self._d:
+            self._d.move_to_end(n)
+            return self._d[n]
+        m = M.box_hex20(n, n, n, 1.0, 1.0, 1.0)
+        rest = M.base_fixed_rest(m)
+        nf, neq = M.steering(m.nn, 3, rest)
+        g_g = M.g_g_from(nf, m.g_num)
+        nodes, vals = M.top_pressure_loads(m, -1.0)
+        r_unit = M.nodal_loads_to_rhs(nf, nodes, vals, neq)
+        entry = {"g_coord": m.g_coord(), "g_g": g_g, "neq": neq, "nels": m.nels, "r_unit": r_unit}
+        self._d[n] = entry
+        while len(self._d) > self.keep:
+            self._d.popitem(last=False)
+        return entry
+
+
+def device_bytes(n: int, precision: str) -> int:
+    """Peak device bytes of one request (assembly writes FP64 matrices before
+    the FP32 split)."""
+    nels = n ** 3
+    neq = 3 * ((2 * n + 1) * (n + 1) * (n + 1) + (n + 1) * (n + 1) * n)   # upper bound
+    per_el = 28800 + 480 + 240 + 240 + 480
+    if precision == "mixed":
+        per_el += 2 * 7440 + 240
+    return nels * per_el + neq * 8 * 12 + (16 << 20)
+
+
+def solve_alone(ctx: Context, cache: CubeCache, req: Request, precision: str, tol: float,
+                limit: int = 100000):
+    """One request by itself on `ctx` (blocking): (x, PcgResult)."""
+    c = cache.get(req.n)
+    ctx.mesh_set(c["g_coord"])
+    ctx.steer_set(c["g_g"], c["neq"])
+    ctx.assemble_elastic(req.e, req.v, None, store="fp32" if precision == "mixed" else "fp64")
+    x, res = ctx.pcg_solve(c["r_unit"] * req.pressure, limit, tol, stop="relres", precision=precision,
+                           check_every=64)
+    ctx.reset()
+    return x, res
+
+
+@dataclass
+class _Slot:
+    ctx: Context
+    req: Request | None = None
+    t_admit: float = 0.0
+    bytes: int = 0
+    chunks: int = 0
+
+
+@dataclass
+class Server:
+    slots: int = 8
+    hbm_budget: float = 160e9
+    chunk: int = 32
+    precision: str = "mixed"
+    tol: float = 1e-10
+    limit: int = 100000
+    keep_x: bool = False
+    device: int = 0
+    cache: CubeCache = field(default_factory=CubeCache)
+
+    def __post_init__(self):
+        self._slots = [_Slot(Context(self.device)) for _ in range(self.slots)]
+
+    def close(self):
+        for s in self._slots:
+            s.ctx.close()
+
+    def _admit(self, s: _Slot, req: Request, now: float):
+        c = self.cache.get(req.n)
+        ctx = s.ctx
+        ctx.mesh_set(c["g_coord"])
+        ctx.steer_set(c["g_g"], c["neq"])
+        ctx.assemble_elastic(req.e, req.v, None, store="fp32" if self.precision == "mixed" else "fp64")
+        ctx.pcg_begin(c["r_unit"] * req.pressure, self.limit, self.tol, stop="relres", precision=self.precision)
+        s.req, s.t_admit, s.bytes, s.chunks = req, now, device_bytes(req.n, self.precision), 0
+        ctx.pcg_run_graph(self.chunk)
+        ctx.pcg_query()                    # snapshot after the first chunk
+        ctx.pcg_run_graph(self.chunk)      # and one chunk queued behind it
+        s.chunks = 2
+
+    def serve(self, requests: list[Request], log=None) -> list[Result]:
+        """Run the stream in real time (arrival times relative to the start)."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        queue: deque[Request] = deque()
+        free = list(self._slots)
+        active: list[_Slot] = []
+        used = 0.0
+        out: list[Result] = []
+        t0 = time.perf_counter()
+        while pending or queue or active:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.popleft())
+            while queue and free:
+                need = device_bytes(queue[0].n, self.precision)
+                if active and used + need > self.hbm_budget:
+                    break
+                s = free.pop()
+                req = queue.popleft()
+                self._admit(s, req, time.perf_counter() - t0)
+                used += s.bytes
+                active.append(s)
+            progressed = False
+            for s in list(active):
+                res = s.ctx.pcg_query()
+                if res is None:
+                    continue
+                progressed = True
+                if res.converged or res.status != 0 or res.iters >= self.limit:
+                    x = s.ctx.pcg_x()
+                    t = time.perf_counter() - t0
+                    r = s.req
+                    out.append(Result(r.rid, r.n, s.ctx.neq, res.iters, bool(res.converged), res.refinements,
+                                      s.t_admit, t, t - r.arrival, x if self.keep_x else None,
+                                      float(np.abs(x).sum())))
+                    if log:
+                        log(f"request {r.rid} n={r.n} iters={res.iters} latency={t - r.arrival:.3f}s")
+                    s.ctx.reset()
+                    used -= s.bytes
+                    s.req = None
+                    active.remove(s)
+                    free.append(s)
+                else:
+                    s.ctx.pcg_query()                  # snapshot after the queued chunk
+                    s.ctx.pcg_run_graph(self.chunk)    # keep one chunk behind it
+                    s.chunks += 1
+            if not progressed and not (queue and free):
+                if not active and pending:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
+                else:
+                    time.sleep(20e-6)
+        return sorted(out, key=lambda r: r.rid)
+
+
+def serve_sequential(requests: list[Request], precision: str = "fp64", tol: float = 1e-10,
+                     device: int = 0, keep_x: bool = False, cache: CubeCache | None = None) -> list[Result]:
+    """The same stream one request at a time (blocking solves, arrival order,
+    real time)."""
+    cache = cache or CubeCache()
+    ctx = Context(device)
+    out = []
+    t0 = time.perf_counter()
+    try:
+        for r in sorted(requests, key=lambda q: q.arrival):
+            now = time.perf_counter() - t0
+            if r.arrival > now:
+                time.sleep(r.arrival - now)
+            ta = time.perf_counter() - t0
+            x, res = solve_alone(ctx, cache, r, precision, tol)
+            t = time.perf_counter() - t0
+            out.append(Result(r.rid, r.n, x.size, res.iters, bool(res.converged), res.refinements, ta, t,
+                              t - r.arrival, x if keep_x else None, float(np.abs(x).sum())))
+    finally:
+        ctx.close()
+    return out
+
+
+def summarize(results: list[Result], wall: float) -> dict:
+    lat = np.array([r.latency for r in results])
+    return {
+        "requests": len(results),
+        "solves_per_s": len(results) / wall if wall > 0 else None,
+        "wall_s": wall,
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_latency_s": float(lat.mean()),
+        "all_converged": all(r.converged for r in results),
+        "iterations": int(sum(r.iters for r in results)),
+    }

@@ -373,3 +373,47 @@ def test_pcg_km_shared_matrix_bitexact(ctx):
     assert res2.iters == it2 and np.array_equal(x2, xo2)
     r = ctx.pcg_r()
     assert np.linalg.norm(r - (pr.r0 - ctx.apply(x2))) <= 1e-8 * np.linalg.norm(pr.r0)
+
+
+def test_graph_replay_matches_eager(ctx):
+    """Iteration chunks replayed from a captured HIP graph give the same
+    state as the eager launches after every chunk, with eager work and host
+    synchronisation between the replays."""
+    pr = P.elastic_cube(5, lognormal_e=True)
+    say(f"graph replay neq={pr.neq}")
+    for precision in ("fp64", "mixed"):
+        r = P.setup(ctx, pr, store="fp64" if precision == "fp64" else "fp32")
+        eager = []
+        ctx.pcg_begin(r, 5000, 1e-10, stop="relres", precision=precision)
+        for _ in range(6):
+            ctx.pcg_run(16)
+            eager.append((ctx.pcg_x(), ctx.pcg_poll().iters))
+        ctx.pcg_begin(r, 5000, 1e-10, stop="relres", precision=precision)
+        for k in range(6):
+            ctx.pcg_run_graph(16)
+            while ctx.pcg_query() is None:
+                pass
+            y = ctx.apply(eager[k][0])          # eager work between replays
+            assert np.all(np.isfinite(y))
+            assert ctx.pcg_poll().iters == eager[k][1]
+            assert np.array_equal(ctx.pcg_x(), eager[k][0]), (precision, k)
+
+
+def test_serving_mix_equals_alone():
+    """Continuous batching: requests solved concurrently on several slots
+    give bit-identical displacements and iteration counts to the same
+    requests solved alone; the FP64 one-by-one path agrees within 1e-8."""
+    from parafem_amd import serve as S
+    reqs = S.make_stream(10, rate=0.0, seed=4, nmin=3, nmax=9)
+    say(f"serving mix sizes={[r.n for r in reqs]}")
+    srv = S.Server(slots=4, chunk=16, keep_x=True)
+    try:
+        mix = srv.serve(reqs)
+    finally:
+        srv.close()
+    assert [m.rid for m in mix] == list(range(10)) and all(m.converged for m in mix)
+    seq = S.serve_sequential(reqs, precision="mixed", keep_x=True)
+    f64 = S.serve_sequential(reqs, precision="fp64", keep_x=True)
+    for a, b, c in zip(mix, seq, f64):
+        assert a.iters == b.iters and np.array_equal(a.x, b.x), a.rid
+        assert np.abs(a.x - c.x).max() <= 1e-8 * np.abs(c.x).max()
