@@ -224,6 +224,44 @@ int pf_pcg_query(pf_ctx *ctx, pf_pcg_result *res, int *ready);
  * p123 fixed-freedom rows applied.  Used to report true residuals. */
 int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);
 
+/* ------------------------------------------------- multi-RHS block solve */
+/* No counterpart in the reference: up to 64 load cases on one mesh solved
+ * together.  The element product for all of them, K_e (ntot x ntot) times
+ * the gathered (ntot x k) block, runs on the matrix cores (FP32 MFMA for the
+ * inner solves, FP64 MFMA for the true residuals); the iteration per column
+ * is p121's Jacobi PCG (p121.f90:86-104), batched as PCG_KM batches
+ * MATMUL(km, pmul_pp) (src/modules/mpi/maths.f90:1152-1323).  Solver: FP64
+ * defect correction, each step an FP32 block PCG (per-column alpha/beta,
+ * columns frozen once their residual fell by inner_tol) on K d = r, then
+ * x += d and r = f - K x in FP64.  Needs FP64 element matrices
+ * (PF_STORE_FP64, full or shared) and no p123 fixed freedoms. */
+typedef struct {
+    double tol;          /* per column: ||f - K x||_2 <= tol ||f||_2          */
+    double inner_tol;    /* FP32 inner solve: residual reduction (0 = 1e-5)   */
+    int    inner_limit;  /* iterations per inner solve (0 = 20000)            */
+    int    outer_limit;  /* defect-correction steps (0 = 10)                  */
+    int    check_every;  /* host polls the inner solve every n iterations (0=32) */
+} pf_block_opts;
+
+typedef struct {
+    int    outer;        /* defect-correction steps taken                     */
+    int    inner_iters;  /* inner iterations over all steps                   */
+    int    converged;    /* columns that met tol                               */
+    double max_relres;   /* largest true relative residual over the columns   */
+    int    status;       /* PF_OK or PF_NOT_CONVERGED                          */
+} pf_block_result;
+
+/* f, x: host, neq x k column-major (column c = load case c), 1 <= k <= 64. */
+int pf_block_solve(pf_ctx *ctx, int k, const double *f, double *x, const pf_block_opts *opts,
+                   pf_block_result *res);
+/* y = K x for k columns through the block element product: precision
+ * PF_PREC_FP64 (FP64 MFMA) or PF_PREC_MIXED (fl32(x), FP32 MFMA, FP32
+ * scatter).  Host, neq x k column-major. */
+int pf_block_apply(pf_ctx *ctx, int k, const double *x, double *y, int precision);
+/* Measurement: average ms of the FP32 matrix-core element product over reps
+ * launches, HIP events on the context's stream (after a block solve). */
+int pf_block_time_ebe(pf_ctx *ctx, int reps, double *ms);
+
 /* ------------------------------------------------------------ timing */
 /* No counterpart in the reference (its timing is CPU_TIME around phases,
  * src/modules/shared/timing.f90:35-62).  While enabled, pf_pcg_run records a

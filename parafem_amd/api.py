@@ -63,6 +63,16 @@ class PcgResult(C.Structure):
                 ("status", C.c_int)]
 
 
+class BlockOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("inner_tol", C.c_double), ("inner_limit", C.c_int),
+                ("outer_limit", C.c_int), ("check_every", C.c_int)]
+
+
+class BlockResult(C.Structure):
+    _fields_ = [("outer", C.c_int), ("inner_iters", C.c_int), ("converged", C.c_int),
+                ("max_relres", C.c_double), ("status", C.c_int)]
+
+
 _lib = None
 
 
@@ -110,6 +120,9 @@ def lib():
         "pf_pcg_run_graph": (i32, [vp, i32]),
         "pf_pcg_query": (i32, [vp, vp, pi]),
         "pf_reset": (i32, [vp]),
+        "pf_block_solve": (i32, [vp, i32, pd, pd, vp, vp]),
+        "pf_block_apply": (i32, [vp, i32, pd, pd, i32]),
+        "pf_block_time_ebe": (i32, [vp, i32, pd]),
         "pf_vec_fill": (i32, [vp, vp, dbl, i64]),
         "pf_vec_waxpy": (i32, [vp, vp, vp, vp, dbl, i64]),
         "pf_vec_mul": (i32, [vp, vp, vp, vp, i64]),
@@ -457,6 +470,33 @@ class Context:
         """Free the current problem's device buffers; keep the context."""
         _check(lib().pf_reset(self.h), "pf_reset")
         self.nels = self.ntot = self.neq = self.nod = 0
+
+    def block_solve(self, f, tol: float = 1e-10, inner_tol: float = 0.0, inner_limit: int = 0,
+                    outer_limit: int = 0, check_every: int = 0):
+        """Multi-RHS solve: f (neq, k), k <= 64 -> (x (neq, k), BlockResult)."""
+        F = np.asfortranarray(f, dtype=np.float64)
+        assert F.ndim == 2 and F.shape[0] == self.neq and 1 <= F.shape[1] <= 64
+        X = np.zeros(F.shape, order="F")
+        o = BlockOpts(float(tol), float(inner_tol), int(inner_limit), int(outer_limit), int(check_every))
+        res = BlockResult()
+        rc = lib().pf_block_solve(self.h, F.shape[1], _pd(F), _pd(X), C.byref(o), C.byref(res))
+        _check(rc, "pf_block_solve", ok=(PF_OK, PF_NOT_CONVERGED))
+        return X, res
+
+    def block_apply(self, x, precision: str = "fp64") -> np.ndarray:
+        """Y = K X through the matrix-core block product (x (neq, k))."""
+        Xf = np.asfortranarray(x, dtype=np.float64)
+        assert Xf.ndim == 2 and Xf.shape[0] == self.neq and 1 <= Xf.shape[1] <= 64
+        Y = np.zeros(Xf.shape, order="F")
+        _check(lib().pf_block_apply(self.h, Xf.shape[1], _pd(Xf), _pd(Y), PREC[precision]), "pf_block_apply")
+        return Y
+
+    def block_time_ebe(self, reps: int = 10) -> float:
+        """Average ms of the FP32 matrix-core element product over reps
+        launches (HIP events on the context's stream); after block_solve."""
+        ms = C.c_double()
+        _check(lib().pf_block_time_ebe(self.h, int(reps), C.byref(ms)), "pf_block_time_ebe")
+        return ms.value
 
     def pcg_r(self) -> np.ndarray:
         """Recurrence residual at the end of the solve."""

@@ -92,6 +92,7 @@ void pf_graph_drop(pf_ctx *c)
 static void free_problem_vectors(pf_ctx *c)
 {
     pf_graph_drop(c);
+    pf_block_free(c);
     size_t nv = (size_t)c->neq + 1;
     free_arr(c, c->x, nv); free_arr(c, c->r, nv); free_arr(c, c->p, nv);
     free_arr(c, c->d, nv); free_arr(c, c->u, nv); free_arr(c, c->dinv, nv);
@@ -104,6 +105,7 @@ static void free_problem_vectors(pf_ctx *c)
 static void free_matrices(pf_ctx *c)
 {
     pf_graph_drop(c);
+    pf_block_free(c);
     const size_t ns = (size_t)pf_sym_per(c->ntot) * c->nels;
     free_arr(c, c->K64, pf_k64_count(c));
     free_arr(c, c->K32hi, ns);

@@ -80,6 +80,34 @@ struct pf_state {
     int    since_refine;
 };
 
+// Multi-RHS block solve (pf_block_solve): up to PF_BLK right-hand sides as
+// the columns of row-major (neq+1) x PF_BLK block vectors.  Column c is kept
+// at position pf_blk_pos(c) so that lane group (c % 16) of an MFMA tile reads
+// its four columns c % 16 + 16 nt, nt = 0..3, as one vector.
+#define PF_BLK 64
+#define PF_BLK_MAXG 2048
+__host__ __device__ inline int pf_blk_pos(int c) { return (c & 15) * 4 + (c >> 4); }
+
+struct pf_blk_state {
+    double up[PF_BLK], alpha[PF_BLK], beta[PF_BLK], rr0[PF_BLK], rr[PF_BLK];
+    int frozen[PF_BLK];
+    int iters, limit, done, spare;
+    double tol2;
+};
+
+struct pf_block {
+    float *Kt32 = nullptr;     // element matrices as FP32 MFMA A-operand tiles (nels * 4096)
+    double *Kt64 = nullptr;    // the same in FP64
+    float *P = nullptr, *U = nullptr, *R = nullptr, *D = nullptr, *X = nullptr;   // FP32 inner vectors
+    float *dinv = nullptr;     // (neq+1) Jacobi, FP32
+    float *Ut32 = nullptr;     // (nels*ntot) x PF_BLK element products, FP32
+    double *Ut64 = nullptr;    // the same in FP64 (true residual)
+    double *X64 = nullptr, *R64 = nullptr, *F64 = nullptr, *U64 = nullptr;
+    double *stage = nullptr;   // neq x PF_BLK host-order staging (upload/download)
+    double *partials = nullptr;   // 3 x PF_BLK_MAXG x PF_BLK
+    pf_blk_state *st = nullptr, *st_host = nullptr;
+};
+
 struct pf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;      // stream in use
@@ -120,6 +148,7 @@ struct pf_ctx {
     pf_pcg_opts opts{};
     bool pcg_ready = false;
     bool precon_given = false;         // dinv set by pf_precon_set, not 1/diag
+    pf_block *blk = nullptr;           // multi-RHS buffers (allocated on first use)
     // Non-blocking state read-back (pf_pcg_query).
     pf_state *state_query = nullptr;   // pinned
     hipEvent_t ev_query = nullptr;
@@ -138,6 +167,8 @@ struct pf_ctx {
 
 // Drop a captured iteration chunk (problem, options or buffers changed).
 void pf_graph_drop(pf_ctx *c);
+// Free the multi-RHS buffers (problem or matrices changed).
+void pf_block_free(pf_ctx *c);
 
 // Event pair handles for timing kernel `k` (returns false when timing is off).
 bool pf_timing_begin(pf_ctx *c, int k, hipEvent_t *start, hipEvent_t *stop);

@@ -119,6 +119,28 @@ def laplace_cube(n: int, lognormal_k: bool = True, seed: int = 2, flux: float = 
                    fix_val=np.zeros(fix_eq.size))
 
 
+def block_cube(n: int, k: int = 64, seed: int = 3, e: float = 100.0, v: float = 0.3):
+    """Multi-RHS workload: hex20 cube (unit edges, E, nu uniform, base
+    fixed) with k load cases; case c is a top-face traction whose three
+    components are drawn U[-1,1] per top-face node (numpy default_rng(seed),
+    array (ntop, 3, k)), turned into consistent nodal loads.
+    Returns (Problem with r0 = case 0, F (neq, k))."""
+    m = M.box_hex20(n, n, n, 1.0, 1.0, 1.0)
+    rest = M.base_fixed_rest(m)
+    nf, neq = M.steering(m.nn, 3, rest)
+    g_g = M.g_g_from(nf, m.g_num)
+    top = np.nonzero(np.abs(m.coords[:, 2]) < 1e-12)[0]
+    rng = np.random.default_rng(seed)
+    t = np.zeros((m.nn, 3, k))
+    t[top] = rng.uniform(-1.0, 1.0, size=(top.size, 3, k))
+    f = M.face_traction_loads(m, t)
+    F = np.zeros((neq, k))
+    msk = nf > 0
+    F[nf[msk] - 1] = f[msk]
+    pr = Problem("elastic", m, g_g, neq, F[:, 0].copy(), nf, e=e, v=v)
+    return pr, F
+
+
 def setup(ctx, prob: Problem, store: str = "fp64", assemble: bool = True):
     """Upload mesh and steering and build the element matrices on the device
     (p121.f90:40-69 / p123.f90:43-92).  For p123 also the penalty rows and the
