@@ -224,6 +224,15 @@ int pf_pcg_query(pf_ctx *ctx, pf_pcg_result *res, int *ready);
  * p123 fixed-freedom rows applied.  Used to report true residuals. */
 int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);
 
+/* ------------------------------------------------------- post-solve */
+/* Stresses at the centroid of every element, p121.f90:112-123 (which
+ * prints element 1's): eld = GATHER(xnew), SHAPE_DER at points = 0,
+ * jac = MATMUL(der, coord), INVERT, deriv, BEEMAT, eps = MATMUL(bee, eld),
+ * sigma = MATMUL(dee, eps) with DEEMAT(e_elem[iel] or e, v).  x: host neq;
+ * e_elem: host nels or NULL; sigma: host (6, nels) column-major.  Needs the
+ * mesh and an elastic steering (ntot = 3 nod). */
+int pf_stress_centroid(pf_ctx *ctx, const double *x, double e, double v, const double *e_elem, double *sigma);
+
 /* ------------------------------------------------- multi-RHS block solve */
 /* No counterpart in the reference: up to 64 load cases on one mesh solved
  * together.  The element product for all of them, K_e (ntot x ntot) times

@@ -56,6 +56,7 @@ def lib():
         L = C.CDLL(path)
         d = C.POINTER(C.c_double)
         i = C.POINTER(C.c_int)
+        L.or_centroid_stress.argtypes = [C.c_int, C.c_int, d, i, d, C.c_double, C.c_double, d, d]
         L.or_sample_hex8.argtypes = [d, d]
         L.or_deemat6.argtypes = [d, C.c_double, C.c_double]
         L.or_shape_der.argtypes = [d, d, C.c_int, C.c_int, C.c_int]
@@ -156,6 +157,19 @@ def elastic_km(coord, dee):
     km = np.zeros(9 * nod * nod)
     lib().or_elastic_km(_d(c), nod, _d(dd), _d(km))
     return km.reshape(3 * nod, 3 * nod).T.copy()
+
+
+def centroid_stress(g_coord, g_g, x, e, v, e_elem=None):
+    """Centroid stresses of every element (p121.f90:112-123) -> (nels, 6)."""
+    g = _f64(g_coord)
+    gg = _i32(g_g)
+    nels, ntot = gg.shape
+    xx = _f64(x)
+    ee = _f64(e_elem) if e_elem is not None else None
+    out = np.zeros(6 * nels)
+    lib().or_centroid_stress(nels, ntot // 3, _d(g), _i(gg), _d(xx), float(e), float(v),
+                             _d(ee) if ee is not None else None, _d(out))
+    return out.reshape(nels, 6)
 
 
 def laplace_kc(coord, kx=1.0, ky=1.0, kz=1.0):

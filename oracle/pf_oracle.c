@@ -227,6 +227,39 @@ void or_elastic_km(const double *coord, int nod, const double *dee, double *km)
     free(btd);
 }
 
+/* Centroid stresses, p121.f90:112-123 for every element: eld = GATHER(x),
+ * SHAPE_DER at points = 0 (nip = 1), jac, INVERT, deriv, BEEMAT,
+ * eps = MATMUL(bee, eld), sigma = MATMUL(dee, eps) with DEEMAT(e_iel, v).
+ * sigma (6, nels) column-major. */
+void or_centroid_stress(int nels, int nod, const double *g_coord, const int *g_g, const double *x,
+                        double e, double v, const double *e_elem, double *sigma)
+{
+    int ntot = 3 * nod;
+    double points[3] = {0.0, 0.0, 0.0}, der[60], deriv[60], eld[60], eps[6], dee[36];
+    double *bee = (double *)malloc(sizeof(double) * 6 * ntot);
+    or_shape_der(der, points, 1, 1, nod);
+    for (int iel = 0; iel < nels; ++iel) {
+        jac_deriv(der, g_coord + (size_t)iel * nod * 3, nod, deriv);
+        or_beemat6(bee, deriv, nod);
+        for (int j = 0; j < ntot; ++j) {
+            int g = g_g[(size_t)iel * ntot + j];
+            eld[j] = g ? x[g - 1] : 0.0;
+        }
+        for (int r = 0; r < 6; ++r) {
+            double s = 0.0;
+            for (int j = 0; j < ntot; ++j) s = s + bee[IDX2(r, j, 6)] * eld[j];
+            eps[r] = s;
+        }
+        or_deemat6(dee, e_elem ? e_elem[iel] : e, v);
+        for (int r = 0; r < 6; ++r) {
+            double s = 0.0;
+            for (int t = 0; t < 6; ++t) s = s + dee[IDX2(r, t, 6)] * eps[t];
+            sigma[(size_t)iel * 6 + r] = s;
+        }
+    }
+    free(bee);
+}
+
 /* One hex8 Laplace element of p123.f90:70-84:
  * kc{x,y,z} += MATMUL(col,row)*det*w; storkc = kcx*kx + kcy*ky + kcz*kz. */
 void or_laplace_kc(const double *coord, int nod, double kx, double ky, double kz, double *kc)

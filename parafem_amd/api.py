@@ -123,6 +123,7 @@ def lib():
         "pf_block_solve": (i32, [vp, i32, pd, pd, vp, vp]),
         "pf_block_apply": (i32, [vp, i32, pd, pd, i32]),
         "pf_block_time_ebe": (i32, [vp, i32, pd]),
+        "pf_stress_centroid": (i32, [vp, pd, dbl, dbl, pd, pd]),
         "pf_vec_fill": (i32, [vp, vp, dbl, i64]),
         "pf_vec_waxpy": (i32, [vp, vp, vp, vp, dbl, i64]),
         "pf_vec_mul": (i32, [vp, vp, vp, vp, i64]),
@@ -470,6 +471,16 @@ class Context:
         """Free the current problem's device buffers; keep the context."""
         _check(lib().pf_reset(self.h), "pf_reset")
         self.nels = self.ntot = self.neq = self.nod = 0
+
+    def stress_centroid(self, x, e: float, v: float, e_elem=None) -> np.ndarray:
+        """Centroid stresses (nels, 6) of the displacement x (p121.f90:112-123)."""
+        xx = _f64(x)
+        assert xx.size == self.neq
+        out = np.zeros(6 * self.nels)
+        ee = _f64(e_elem) if e_elem is not None else None
+        _check(lib().pf_stress_centroid(self.h, _pd(xx), float(e), float(v),
+                                        _pd(ee) if ee is not None else None, _pd(out)), "pf_stress_centroid")
+        return out.reshape(self.nels, 6)
 
     def block_solve(self, f, tol: float = 1e-10, inner_tol: float = 0.0, inner_limit: int = 0,
                     outer_limit: int = 0, check_every: int = 0):

@@ -471,3 +471,15 @@ def test_block_solve_64_load_cases(ctx):
     X5, res5 = ctx.block_solve(F[:, :5], 1e-10)
     assert res5.converged == 5
     assert np.abs(X5 - X[:, :5]).max() <= 1e-8 * np.abs(X[:, :5]).max()
+
+
+def test_centroid_stress_bitexact(ctx):
+    """Centroid stresses of every element (p121.f90:112-123) on a distorted
+    mesh with per-element E: bit-identical to the oracle."""
+    pr = _distorted(3, seed=13)
+    say(f"centroid stress nels={pr.nels}")
+    P.setup(ctx, pr)
+    x, res = ctx.pcg_solve(pr.r0, 5000, 1e-10, stop="relres")
+    s = ctx.stress_centroid(x, pr.e, pr.v, pr.e_elem)
+    so = O.centroid_stress(pr.g_coord(), pr.g_g, x, pr.e, pr.v, pr.e_elem)
+    assert np.array_equal(s, so)

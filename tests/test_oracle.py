@@ -263,6 +263,20 @@ def test_elem_matvec_diag_apply_against_global_matrix():
     assert np.allclose(y, A @ x, rtol=0, atol=1e-12 * np.abs(A @ x).max())
 
 
+def test_centroid_stress_of_uniform_uniaxial_field():
+    """Centroid stresses of the exact uniaxial field (patch test) on a
+    distorted-free cube: sigma_zz = -q, every other component 0, in every
+    element; linear in E (per-element moduli)."""
+    pr = P.uniaxial_cube(3, e=100.0, v=0.3, q=2.5)
+    s = O.centroid_stress(pr.g_coord(), pr.g_g, pr.info["exact"], pr.e, pr.v)
+    ref = np.zeros_like(s)
+    ref[:, 2] = -2.5
+    assert np.allclose(s, ref, rtol=0, atol=1e-12)
+    ee = np.linspace(1.0, 3.0, pr.nels)
+    s2 = O.centroid_stress(pr.g_coord(), pr.g_g, pr.info["exact"], pr.e, pr.v, e_elem=ee * pr.e)
+    assert np.allclose(s2, s * ee[:, None], rtol=1e-13, atol=1e-12)
+
+
 def test_checon_closed_cases():
     """a22: converged iff max|xnew - x| <= tol * max|xnew|; x <- xnew."""
     xnew = np.array([1.0, -4.0, 2.0, 0.5])
@@ -418,3 +432,8 @@ def test_demo_fixture(golden):
     ref = np.load(os.path.join(golden, "p121_demo_displ.npz"))["disp"].astype(np.float64)
     unit = 10.0 ** (np.floor(np.log10(np.maximum(np.abs(ref), 1e-30))) - 3)
     assert np.all(np.abs(disp - ref) <= 0.5 * unit * 1.001 + 6e-6 * np.abs(ref).max())
+    # p121_demo.res:9-11: centroid stresses of element 1, printed E12.4
+    sig = O.centroid_stress(m.g_coord(), g_g, x, meta["e"], meta["v"])[0]
+    pub = np.array(meta["res_sigma_elem1"])
+    punit = 10.0 ** (np.floor(np.log10(np.abs(pub))) - 3)
+    assert np.all(np.abs(sig - pub) <= 0.5 * punit * 1.001 + 1e-5 * np.abs(pub).max()), (sig, pub)
