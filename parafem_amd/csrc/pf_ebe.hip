@@ -136,6 +136,15 @@ template <int NT, int KIND> struct Lanes {
     static constexpr int EPB = EPW * EPL * (PF_BLOCK / 64);   // elements per block
 };
 
+// Workgroups are dealt to the 8 XCDs round-robin by index.  Renumber them so
+// that each XCD works on one contiguous range of logical blocks: neighbouring
+// elements share nodes, and their gathers then hit the same L2.
+__device__ __forceinline__ long xcd_block(int b, int nb)
+{
+    const int per = nb >> 3, rem = nb & 7, x = b & 7, i = b >> 3;
+    return (long)x * per + (x < rem ? x : rem) + i;
+}
+
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef double d2_t __attribute__((ext_vector_type(2)));
 
@@ -176,7 +185,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
     const int sub = lane / L::W, li = lane % L::W;
     const int i = li < NT ? li : NT - 1;
     const int gb = sub * L::W;   // first lane of the element's group
-    const long ebase = ((long)blockIdx.x * (PF_BLOCK / 64) + wave) * L::EPW * EPL;
+    const long ebase = (xcd_block(blockIdx.x, gridDim.x) * (PF_BLOCK / 64) + wave) * L::EPW * EPL;
     long ev[EPL], ec[EPL];
     int g[EPL];
 #pragma unroll
