@@ -5,7 +5,8 @@
 // element accumulates 16x16 output tiles with v_mfma_f32_16x16x4f32 (inner
 // solves) or v_mfma_f64_16x16x4f64 (true residuals).  Tile operand layout of
 // these instructions (lane l of 64): A(16x4) element [l%16][l/16], B(4x16)
-// element [l/16][l%16], D(16x16) register v holds [4(l/16)+v][l%16].
+// element [l/16][l%16]; D(16x16) register v holds [4(l/16)+v][l%16] for
+// the FP32 instruction and [4v+l/16][l%16] for the FP64 one.
 // Element matrices are stored pre-arranged as A operands: for tile row mt
 // and k-step quad ktq, lane l reads four consecutive values (k-steps
 // 4ktq..4ktq+3) -- one 16-byte (FP32) or 32-byte (FP64) load per lane,
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(256) void k_blk_ebe(const T *__restrict__ Kt, long 
         for (int m = 0; m < MP; ++m)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = 16 * (m0 + m) + 4 * rq + v;
+                const int row = 16 * (m0 + m) + (sizeof(T) == 4 ? 4 * rq + v : 4 * v + rq);
                 if (m0 + m < TL::MT && row < NT)
                     *reinterpret_cast<T4 *>(Ut + (e * NT + row) * PF_BLK + 4 * cl) =
                         T4{acc[m][0][v], acc[m][1][v], acc[m][2][v], acc[m][3][v]};

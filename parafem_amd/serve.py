@@ -187,7 +187,9 @@ class Server:
                 if res is None:
                     continue
                 progressed = True
-                if res.converged or res.status != 0 or res.iters >= self.limit:
+                # converged, broken down (status 6) or at the limit: the solve
+                # is frozen on the device; anything else is still running
+                if res.converged or res.status == 6 or res.iters >= self.limit:
                     x = s.ctx.pcg_x()
                     t = time.perf_counter() - t0
                     r = s.req
