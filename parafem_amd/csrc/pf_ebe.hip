@@ -305,6 +305,27 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
     }
 }
 
+// Sum of utemp over the element slots [a, b) of one equation, in slot
+// (= ascending element) order.  The slot indices of up to 8 slots -- every
+// equation of a hex8 or hex20 mesh of bricks has at most 8 -- are loaded
+// before the dependent utemp loads; longer lists continue in a loop.
+template <typename UT>
+__device__ __forceinline__ double slot_sum(const int *__restrict__ slot, const UT *__restrict__ utemp, int a, int b)
+{
+    int sl[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sl[j] = a + j < b ? slot[a + j] : -1;
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = sl[j] >= 0 ? (double)utemp[sl[j]] : 0.0;
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (sl[j] >= 0) s = __dadd_rn(s, v[j]);
+    for (int k = a + 8; k < b; ++k) s = __dadd_rn(s, (double)utemp[slot[k]]);
+    return s;
+}
+
 // SCATTER fused with p.u.  u and p have the leading 0 slot.
 template <typename UT>
 __global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__ ptr, const int *__restrict__ slot,
@@ -317,9 +338,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__
     double acc[1] = {0.0};
     const int stride = gridDim.x * PF_BLOCK;
     for (int q = blockIdx.x * PF_BLOCK + threadIdx.x; q < neq; q += stride) {
-        const int a = ptr[q], b = ptr[q + 1];
-        double s = 0.0;
-        for (int k = a; k < b; ++k) s = __dadd_rn(s, (double)utemp[slot[k]]);
+        double s = slot_sum(slot, utemp, ptr[q], ptr[q + 1]);
         const double pq = p[q + 1];
         if (fixstore) {
             const double fs = fixstore[q + 1];
@@ -460,8 +479,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_scatter_res(const int *__restrict_
     double v[2] = {0.0, 0.0};
     const int stride = gridDim.x * PF_BLOCK;
     for (int q = blockIdx.x * PF_BLOCK + threadIdx.x; q < neq; q += stride) {
-        double s = 0.0;
-        for (int k = ptr[q]; k < ptr[q + 1]; ++k) s = __dadd_rn(s, utemp[slot[k]]);
+        double s = slot_sum(slot, utemp, ptr[q], ptr[q + 1]);
         if (fixstore) {
             const double fs = fixstore[q + 1];
             if (fs != 0.0) s = __dmul_rn(x[q + 1], fs);
