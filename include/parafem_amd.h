@@ -111,8 +111,11 @@ int pf_dev_alloc(pf_ctx *ctx, size_t bytes, void **dptr);
 int pf_dev_free(pf_ctx *ctx, void *dptr);
 int pf_memcpy_h2d(pf_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
 int pf_memcpy_d2h(pf_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
-/* Bytes of device memory the context holds. */
+/* Bytes of device memory the context holds (in use plus freed blocks it
+ * keeps for reuse). */
 int pf_mem_used(pf_ctx *ctx, size_t *bytes);
+/* Return the freed blocks the context keeps for reuse to the device. */
+int pf_trim(pf_ctx *ctx);
 /* Device-to-device copy by a plain streaming kernel (k_copy): moves
  * `bytes` (a multiple of 16) once through HBM in each direction.  Also the
  * calibration kernel for the HBM byte counters.  No reference counterpart. */

@@ -124,6 +124,7 @@ def lib():
         "pf_block_apply": (i32, [vp, i32, pd, pd, i32]),
         "pf_block_time_ebe": (i32, [vp, i32, pd]),
         "pf_stress_centroid": (i32, [vp, pd, dbl, dbl, pd, pd]),
+        "pf_trim": (i32, [vp]),
         "pf_vec_fill": (i32, [vp, vp, dbl, i64]),
         "pf_vec_waxpy": (i32, [vp, vp, vp, vp, dbl, i64]),
         "pf_vec_mul": (i32, [vp, vp, vp, vp, i64]),
@@ -466,6 +467,10 @@ class Context:
         ready = C.c_int(0)
         _check(lib().pf_pcg_query(self.h, C.byref(res), C.byref(ready)), "pf_pcg_query")
         return res if ready.value else None
+
+    def trim(self):
+        """Give the context's cached device blocks back to the device."""
+        _check(lib().pf_trim(self.h), "pf_trim")
 
     def reset(self):
         """Free the current problem's device buffers; keep the context."""

@@ -26,22 +26,64 @@ int pf_alloc(pf_ctx *c, void **p, size_t bytes)
 {
     *p = nullptr;
     if (bytes == 0) bytes = 256;
+    // best fit among the cached blocks no larger than 2x the request + 1 MiB
+    size_t best = (size_t)-1, pick = 0;
+    for (size_t k = 0; k < c->cache.size(); ++k) {
+        const size_t sz = c->cache[k].second;
+        if (sz >= bytes && sz <= 2 * bytes + (1u << 20) && sz < best) {
+            best = sz;
+            pick = k;
+        }
+    }
+    if (best != (size_t)-1) {
+        *p = c->cache[pick].first;
+        c->cache[pick] = c->cache.back();
+        c->cache.pop_back();
+        c->cached -= best;
+        c->live.emplace_back(*p, best);
+        c->bytes += best;
+        return PF_OK;
+    }
     hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && !c->cache.empty()) {
+        (void)hipGetLastError();
+        pf_cache_release(c);
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
         return pf_set_hip_error(e, "hipMalloc", __FILE__, __LINE__);
     }
+    c->live.emplace_back(*p, bytes);
     c->bytes += bytes;
     return PF_OK;
 }
 
 void pf_free(pf_ctx *c, void *p, size_t bytes)
 {
+    (void)bytes;
     if (!p) return;
-    (void)hipFree(p);
-    if (bytes == 0) bytes = 256;
-    c->bytes -= std::min(c->bytes, bytes);
+    for (size_t k = 0; k < c->live.size(); ++k) {
+        if (c->live[k].first == p) {
+            const size_t sz = c->live[k].second;
+            c->live[k] = c->live.back();
+            c->live.pop_back();
+            c->bytes -= std::min(c->bytes, sz);
+            c->cache.emplace_back(p, sz);
+            c->cached += sz;
+            return;
+        }
+    }
+    (void)hipFree(p);   // not from this context's allocator
+}
+
+void pf_cache_release(pf_ctx *c)
+{
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &b : c->cache) (void)hipFree(b.first);
+    c->cache.clear();
+    c->cached = 0;
 }
 
 template <class T>
@@ -205,6 +247,8 @@ int pf_destroy(pf_ctx *c)
     if (c->state_query) (void)hipHostFree(c->state_query);
     if (c->ev_query) (void)hipEventDestroy(c->ev_query);
     pf_graph_drop(c);
+    pf_cache_release(c);
+    for (auto &b : c->live) (void)hipFree(b.first);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -277,7 +321,15 @@ int pf_memcpy_d2h(pf_ctx *c, void *dst, const void *src, size_t bytes)
 int pf_mem_used(pf_ctx *c, size_t *bytes)
 {
     if (!c || !bytes) return PF_ERR_ARG;
-    *bytes = c->bytes;
+    *bytes = c->bytes + c->cached;
+    return PF_OK;
+}
+
+int pf_trim(pf_ctx *c)
+{
+    if (!c) return PF_ERR_ARG;
+    PF_HIP(hipSetDevice(c->device));
+    pf_cache_release(c);
     return PF_OK;
 }
 

@@ -112,7 +112,13 @@ struct pf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;      // stream in use
     hipStream_t own_stream = nullptr;  // created by the context
-    size_t bytes = 0;                  // device bytes held
+    size_t bytes = 0;                  // device bytes in live allocations
+    // Freed device blocks kept for reuse by later allocations of this
+    // context (hipFree waits for the device; a context that solves one
+    // problem after another reuses its buffers instead).
+    std::vector<std::pair<void *, size_t>> cache;
+    std::vector<std::pair<void *, size_t>> live;   // allocation -> block size
+    size_t cached = 0;
 
     // mesh
     int nod = 0, nels = 0;
@@ -191,6 +197,7 @@ inline long pf_k64_estride(const pf_ctx *c) { return c->k_shared ? 0 : 1; }
 int pf_set_hip_error(hipError_t e, const char *what, const char *file, int line);
 int pf_alloc(pf_ctx *c, void **p, size_t bytes);
 void pf_free(pf_ctx *c, void *p, size_t bytes);
+void pf_cache_release(pf_ctx *c);
 int pf_grid_for(int64_t n);
 
 // kernels launched across translation units ---------------------------------
