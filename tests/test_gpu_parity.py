@@ -483,3 +483,32 @@ def test_centroid_stress_bitexact(ctx):
     s = ctx.stress_centroid(x, pr.e, pr.v, pr.e_elem)
     so = O.centroid_stress(pr.g_coord(), pr.g_g, x, pr.e, pr.v, pr.e_elem)
     assert np.array_equal(s, so)
+
+
+def test_context_reuse_and_trim():
+    """A context solving problems one after another reuses its freed device
+    blocks (same result as a fresh context, bit for bit); pf_trim gives the
+    cached blocks back."""
+    from parafem_amd import Context
+    a = _distorted(3, seed=31)
+    b = _distorted(4, seed=32)
+    c1 = Context(0)
+    try:
+        xa1, _ = P.solve(c1, a, 1e-10, 5000, stop="relres")
+        c1.reset()
+        held = c1.mem_used()
+        xb1, _ = P.solve(c1, b, 1e-10, 5000, stop="relres")
+        c1.reset()
+        xa2, _ = P.solve(c1, a, 1e-10, 5000, stop="relres")
+        c1.reset()
+        assert held > 0 and c1.mem_used() >= held
+        c1.trim()
+        assert c1.mem_used() < held
+    finally:
+        c1.close()
+    c2 = Context(0)
+    try:
+        xb2, _ = P.solve(c2, b, 1e-10, 5000, stop="relres")
+    finally:
+        c2.close()
+    assert np.array_equal(xa1, xa2) and np.array_equal(xb1, xb2)
