@@ -59,6 +59,12 @@ extern "C" {
                                  reliable updates replace r by the true
                                  residual f - (hi+lo) x computed in FP64.
                                  Needs PF_STORE_FP32 and PF_STOP_RELRES.       */
+#define PF_PREC_MIXED64    2  /* as PF_PREC_MIXED, but the element products
+                                 widen hi to FP64 and use the FP64 p and FP64
+                                 sums (the products see one fixed FP32
+                                 rounding of K instead of a new one each
+                                 iteration: fewer iterations on problems
+                                 whose FP64 solve converges superlinearly).  */
 
 typedef struct pf_ctx pf_ctx;
 

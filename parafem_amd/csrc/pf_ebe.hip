@@ -41,7 +41,7 @@ namespace {
 
 constexpr int MAXG = PF_REDUCE_MAX_GRID;
 
-enum { KIND_F64 = 0, KIND_F32 = 1, KIND_ACC = 2 };   // K_e storage / arithmetic
+enum { KIND_F64 = 0, KIND_F32 = 1, KIND_ACC = 2, KIND_WIDE = 3 };   // K_e storage / arithmetic
 enum { RUN_ALWAYS = 0, RUN_LIVE = 1, RUN_REFINE = 2 };  // state gating
 
 __device__ __forceinline__ bool skip(const pf_state *st, int gate)
@@ -132,7 +132,7 @@ __device__ void fold_partials(double (&out)[NQ], const int (&qid)[NQ], const boo
 template <int NT, int KIND> struct Lanes {
     static constexpr int W = NT > 32 ? 64 : (NT > 16 ? 32 : (NT > 8 ? 16 : 8));
     static constexpr int EPW = 64 / W;
-    static constexpr int EPL = NT > 32 ? 1 : NT > 8 ? (KIND == 1 ? 2 : 1) : (KIND == 1 ? 8 : 4);
+    static constexpr int EPL = NT > 32 ? 1 : NT > 8 ? (KIND == 1 ? 2 : 1) : (KIND == 1 ? 8 : KIND == 0 ? 1 : 4);
     static constexpr int EPB = EPW * EPL * (PF_BLOCK / 64);   // elements per block
 };
 
@@ -164,6 +164,8 @@ __device__ __forceinline__ double bpermute64(int addr, double v)
 //   KIND_F32: packed symmetric FP32 hi, FP32 arithmetic on fl32(p) -> out
 //             (float);
 //   KIND_ACC: packed symmetric (double)hi + (double)lo, FP64 arithmetic ->
+//             out (double);
+//   KIND_WIDE: packed symmetric (double)hi, FP64 arithmetic on the FP64 p ->
 //             out (double).
 // Packed symmetric rows: lane i holds K(i, i+d mod NT) for d = 0..NT/2 and
 // forms sum_d K(i,i+d) p(i+d) from p staged twice in LDS, then receives
@@ -258,6 +260,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
             if (ev[t] < nels && li < NT) out[ev[t] * NT + li] = (OUT)acc;
         }
     } else {
+        constexpr bool LO = KIND == KIND_ACC;
         double kd[EPL][D];
 #pragma unroll
         for (int t = 0; t < EPL; ++t) {
@@ -266,16 +269,24 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const f4_t h = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Kh) + q * NT + i);
-                const f4_t l = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Kl) + q * NT + i);
-                kd[t][4 * q] = (double)h.x + (double)l.x;
-                kd[t][4 * q + 1] = (double)h.y + (double)l.y;
-                kd[t][4 * q + 2] = (double)h.z + (double)l.z;
-                kd[t][4 * q + 3] = (double)h.w + (double)l.w;
+                if constexpr (LO) {
+                    const f4_t l = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Kl) + q * NT + i);
+                    kd[t][4 * q] = (double)h.x + (double)l.x;
+                    kd[t][4 * q + 1] = (double)h.y + (double)l.y;
+                    kd[t][4 * q + 2] = (double)h.z + (double)l.z;
+                    kd[t][4 * q + 3] = (double)h.w + (double)l.w;
+                } else {
+                    kd[t][4 * q] = (double)h.x;
+                    kd[t][4 * q + 1] = (double)h.y;
+                    kd[t][4 * q + 2] = (double)h.z;
+                    kd[t][4 * q + 3] = (double)h.w;
+                }
             }
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                kd[t][4 * Q + r] = (double)__builtin_nontemporal_load(Kh + 4 * Q * NT + R * i + r) +
-                                   (double)__builtin_nontemporal_load(Kl + 4 * Q * NT + R * i + r);
+                kd[t][4 * Q + r] = LO ? (double)__builtin_nontemporal_load(Kh + 4 * Q * NT + R * i + r) +
+                                            (double)__builtin_nontemporal_load(Kl + 4 * Q * NT + R * i + r)
+                                      : (double)__builtin_nontemporal_load(Kh + 4 * Q * NT + R * i + r);
         }
         double pv[EPL];
 #pragma unroll
@@ -448,7 +459,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_fold_update(const double *partials
         out[1] = o[1];
     }
     if (threadIdx.x == 0) {
-        if (st->precision == PF_PREC_MIXED) {
+        if (st->precision != PF_PREC_FP64) {
             // Reliable update: replace the recurrence residual by the true
             // one when it has dropped by delta since the last replacement,
             // periodically if asked, and before accepting convergence.
@@ -829,7 +840,7 @@ int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opt
     if (opts->stop_mode != PF_STOP_CHECON && opts->stop_mode != PF_STOP_RELRES) return PF_ERR_ARG;
     if (opts->precision == PF_PREC_FP64) {
         if (!c->K64) return PF_ERR_ARG;   // FP64 iterations need FP64 storage
-    } else if (opts->precision == PF_PREC_MIXED) {
+    } else if (opts->precision == PF_PREC_MIXED || opts->precision == PF_PREC_MIXED64) {
         if (!c->K32hi || opts->stop_mode != PF_STOP_RELRES) return PF_ERR_ARG;
         if (opts->refine_delta < 0.0 || opts->refine_delta >= 1.0 || opts->refine_every < 0) return PF_ERR_ARG;
     } else {
@@ -868,6 +879,7 @@ int pf_pcg_run(pf_ctx *c, int n)
     if (!c->pcg_ready) return PF_ERR_STATE;
     const int G = pf_grid_for(c->neq);
     const bool mixed = c->opts.precision == PF_PREC_MIXED;
+    const bool wide = c->opts.precision == PF_PREC_MIXED64;
     hipEvent_t t0, t1;
     for (int it = 0; it < n; ++it) {
         int rc;
@@ -875,6 +887,8 @@ int pf_pcg_run(pf_ctx *c, int n)
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
         if (mixed)
             rc = launch_ebe<KIND_F32, true, float>(c, c->p, c->utemp32, c->state, RUN_LIVE);
+        else if (wide)
+            rc = launch_ebe<KIND_WIDE, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE);
         else
             rc = launch_ebe<KIND_F64, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE);
         if (rc) return rc;
@@ -903,7 +917,7 @@ int pf_pcg_run(pf_ctx *c, int n)
                                c->state);
         }
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
-        if (mixed) {
+        if (mixed || wide) {
             // True-residual replacement; the kernels do nothing unless the
             // fold above flagged this iteration.
             tm = pf_timing_begin(c, PF_K_REFINE, &t0, &t1);

@@ -168,7 +168,7 @@ def solve(ctx, prob: Problem, tol: float, limit: int, stop: str = "checon",
     """Full flow: set-up, assembly, PCG.  Returns (x, SolveResult).  The
     mixed solver stores the element matrices as FP32 hi/lo."""
     if store is None:
-        store = "fp32" if precision == "mixed" else "fp64"
+        store = "fp64" if precision == "fp64" else "fp32"
     r = setup(ctx, prob, store=store)
     return ctx.pcg_solve(r, limit, tol, stop=stop, precision=precision, **kw)
 

@@ -100,7 +100,7 @@ def device_bytes(n: int, precision: str) -> int:
     nels = n ** 3
     neq = 3 * ((2 * n + 1) * (n + 1) * (n + 1) + (n + 1) * (n + 1) * n)   # upper bound
     per_el = 28800 + 480 + 240 + 240 + 480
-    if precision == "mixed":
+    if precision != "fp64":
         per_el += 2 * 7440 + 240
     return nels * per_el + neq * 8 * 12 + (16 << 20)
 
@@ -111,7 +111,7 @@ def solve_alone(ctx: Context, cache: CubeCache, req: Request, precision: str, to
     c = cache.get(req.n)
     ctx.mesh_set(c["g_coord"])
     ctx.steer_set(c["g_g"], c["neq"])
-    ctx.assemble_elastic(req.e, req.v, None, store="fp32" if precision == "mixed" else "fp64")
+    ctx.assemble_elastic(req.e, req.v, None, store="fp64" if precision == "fp64" else "fp32")
     x, res = ctx.pcg_solve(c["r_unit"] * req.pressure, limit, tol, stop="relres", precision=precision,
                            check_every=64)
     ctx.reset()
@@ -151,7 +151,7 @@ class Server:
         ctx = s.ctx
         ctx.mesh_set(c["g_coord"])
         ctx.steer_set(c["g_g"], c["neq"])
-        ctx.assemble_elastic(req.e, req.v, None, store="fp32" if self.precision == "mixed" else "fp64")
+        ctx.assemble_elastic(req.e, req.v, None, store="fp64" if self.precision == "fp64" else "fp32")
         ctx.pcg_begin(c["r_unit"] * req.pressure, self.limit, self.tol, stop="relres", precision=self.precision)
         s.req, s.t_admit, s.bytes, s.chunks = req, now, device_bytes(req.n, self.precision), 0
         ctx.pcg_run_graph(self.chunk)

@@ -319,21 +319,22 @@ def test_packed_symmetric_products(ctx, nt_case):
     assert np.allclose(y, yr[1:], rtol=0, atol=1e-12 * np.abs(yr).max())
 
 
-@pytest.mark.parametrize("n", [6, 12])
-def test_mixed_precision_reaches_fp64_solution(ctx, n):
-    """FP32-stored element matrices, FP32 element products, FP64 vectors and
-    reliable updates: relres 1e-10 on the true residual, displacements within
-    1e-8 of the FP64 solve of the same system at relres 1e-12."""
+@pytest.mark.parametrize("n,precision", [(6, "mixed"), (12, "mixed"), (6, "mixed64"), (12, "mixed64")])
+def test_mixed_precision_reaches_fp64_solution(ctx, n, precision):
+    """FP32-stored element matrices, FP32 (mixed) or FP64-widened (mixed64)
+    element products, FP64 vectors and reliable updates: relres 1e-10 on the
+    true residual, displacements within 1e-8 of the FP64 solve of the same
+    system at relres 1e-12; mixed64 within 20% of the FP64 iteration count."""
     pr = P.elastic_cube(n, lognormal_e=True)
-    say(f"mixed precision n={n} neq={pr.neq}")
+    say(f"{precision} n={n} neq={pr.neq}")
     x64, r64 = P.solve(ctx, pr, 1e-12, 20000, stop="relres")
-    xm, rm = P.solve(ctx, pr, 1e-10, 20000, stop="relres", precision="mixed")
+    xm, rm = P.solve(ctx, pr, 1e-10, 20000, stop="relres", precision=precision)
     assert r64.converged and rm.converged and rm.refinements >= 1
     y = ctx.apply(xm)
     assert np.linalg.norm(y - pr.r0) <= 1.5e-10 * np.linalg.norm(pr.r0)
     assert np.abs(xm - x64).max() <= 1e-8 * np.abs(x64).max()
     x64b, r64b = P.solve(ctx, pr, 1e-10, 20000, stop="relres")
-    assert rm.iters <= 1.5 * r64b.iters + 10
+    assert rm.iters <= (1.5 if precision == "mixed" else 1.2) * r64b.iters + 10
 
 
 def test_mixed_precision_laplace_penalty(ctx):
