@@ -297,7 +297,7 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serve: Poisson arrival rate (0 = 80%% of the measured closed-loop capacity)")
     ap.add_argument("--slots", type=int, default=8, help="serve: concurrent solves")
-    ap.add_argument("--serve-precision", default="mixed64", choices=["mixed", "mixed64"],
+    ap.add_argument("--serve-precision", default="split", choices=["mixed", "mixed64", "split"],
                     help="serve: arithmetic of the element products in the server")
     ap.add_argument("--n", type=int, default=0, help="elements per cube edge (0 = the workload's size)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
@@ -339,7 +339,7 @@ def main():
     log(f"problem built: nels={pr.nels} neq={pr.neq} ({time.perf_counter() - t0:.1f} s)")
     ctx = Context(0)
     paths = {}
-    order = ["fp64", "mixed64", "mixed"] if args.precision == "mixed" else ["mixed", "mixed64", "fp64"]
+    order = ["fp64", "split", "mixed64", "mixed"] if args.precision == "mixed" else ["mixed", "mixed64", "split", "fp64"]
     for prec in order:
         store = "fp64" if prec == "fp64" else "fp32"
         t0 = time.perf_counter()
@@ -373,7 +373,7 @@ def main():
         solves = {}
         x64, solves["fp64"] = solve(ctx, pr, "fp64", "fp64", args.solve_tol)
         log(f"fp64 solve: {solves['fp64']}")
-        for prec in ("mixed", "mixed64"):
+        for prec in ("mixed", "mixed64", "split"):
             xm, solves[prec] = solve(ctx, pr, prec, "fp32", args.solve_tol)
             solves[prec]["rel_err_vs_fp64"] = float(np.abs(xm - x64).max() / np.abs(x64).max())
             solves[f"speedup_{prec}_vs_fp64"] = round(solves["fp64"]["time_s"] / solves[prec]["time_s"], 3)

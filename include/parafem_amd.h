@@ -65,6 +65,10 @@ extern "C" {
                                  rounding of K instead of a new one each
                                  iteration: fewer iterations on problems
                                  whose FP64 solve converges superlinearly).  */
+#define PF_PREC_SPLIT      3  /* FP32 hi + lo storage, element products of
+                                 (double)hi + (double)lo in FP64 every
+                                 iteration: the FP64 operator to ~2^-46, the
+                                 FP64 iteration, half the FP64 path's bytes. */
 
 typedef struct pf_ctx pf_ctx;
 

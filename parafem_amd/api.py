@@ -38,7 +38,7 @@ PF_ERR_NODEVICE = 7
 
 STORE = {"fp64": 0, "fp32": 1}
 STOP = {"checon": 0, "relres": 1}
-PREC = {"fp64": 0, "mixed": 1, "mixed64": 2}
+PREC = {"fp64": 0, "mixed": 1, "mixed64": 2, "split": 3}
 
 
 class LibraryNotBuilt(RuntimeError):

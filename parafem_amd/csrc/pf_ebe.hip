@@ -459,7 +459,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_fold_update(const double *partials
         out[1] = o[1];
     }
     if (threadIdx.x == 0) {
-        if (st->precision != PF_PREC_FP64) {
+        if (st->precision == PF_PREC_MIXED || st->precision == PF_PREC_MIXED64) {
             // Reliable update: replace the recurrence residual by the true
             // one when it has dropped by delta since the last replacement,
             // periodically if asked, and before accepting convergence.
@@ -840,6 +840,8 @@ int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opt
     if (opts->stop_mode != PF_STOP_CHECON && opts->stop_mode != PF_STOP_RELRES) return PF_ERR_ARG;
     if (opts->precision == PF_PREC_FP64) {
         if (!c->K64) return PF_ERR_ARG;   // FP64 iterations need FP64 storage
+    } else if (opts->precision == PF_PREC_SPLIT) {
+        if (!c->K32hi) return PF_ERR_ARG;
     } else if (opts->precision == PF_PREC_MIXED || opts->precision == PF_PREC_MIXED64) {
         if (!c->K32hi || opts->stop_mode != PF_STOP_RELRES) return PF_ERR_ARG;
         if (opts->refine_delta < 0.0 || opts->refine_delta >= 1.0 || opts->refine_every < 0) return PF_ERR_ARG;
@@ -880,6 +882,7 @@ int pf_pcg_run(pf_ctx *c, int n)
     const int G = pf_grid_for(c->neq);
     const bool mixed = c->opts.precision == PF_PREC_MIXED;
     const bool wide = c->opts.precision == PF_PREC_MIXED64;
+    const bool split = c->opts.precision == PF_PREC_SPLIT;
     hipEvent_t t0, t1;
     for (int it = 0; it < n; ++it) {
         int rc;
@@ -889,6 +892,8 @@ int pf_pcg_run(pf_ctx *c, int n)
             rc = launch_ebe<KIND_F32, true, float>(c, c->p, c->utemp32, c->state, RUN_LIVE);
         else if (wide)
             rc = launch_ebe<KIND_WIDE, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE);
+        else if (split)
+            rc = launch_ebe<KIND_ACC, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE);
         else
             rc = launch_ebe<KIND_F64, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE);
         if (rc) return rc;

@@ -132,7 +132,7 @@ class Server:
     slots: int = 8
     hbm_budget: float = 160e9
     chunk: int = 32
-    precision: str = "mixed"
+    precision: str = "split"
     tol: float = 1e-10
     limit: int = 100000
     keep_x: bool = False

