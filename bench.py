@@ -41,19 +41,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def ebe_bytes(ntot, store):
+def ebe_bytes(ntot, store, precision="mixed"):
     """Algorithmic HBM bytes of one element product: the element matrix once
     (full FP64, or the FP32 hi part packed symmetric: ntot/2 + 1 entries per
-    row) and the int32 steering once."""
+    row, hi and lo for the split precision) and the int32 steering once."""
     if store == "fp64":
         return ntot * ntot * 8 + ntot * 4
-    return (ntot // 2 + 1) * ntot * 4 + ntot * 4
+    parts = 2 if precision == "split" else 1
+    return parts * (ntot // 2 + 1) * ntot * 4 + ntot * 4
 
 
-def iteration_bytes(nels, ntot, neq, store):
+def iteration_bytes(nels, ntot, neq, store, precision="mixed"):
     """Per-iteration algorithmic bytes: element matrices + steering + 15 FP64
     vector streams of length neq."""
-    return nels * ebe_bytes(ntot, store) + 15 * neq * 8
+    return nels * ebe_bytes(ntot, store, precision) + 15 * neq * 8
 
 
 def time_iterations(ctx, pr, precision, steps, warmup):
@@ -348,7 +349,7 @@ def main():
         t_setup = time.perf_counter() - t0
         elapsed, tk, nprof = time_iterations(ctx, pr, prec, args.steps, args.warmup)
         ebe_ms = tk["ebe"][0] / max(tk["ebe"][1], 1)
-        per_elem = ebe_bytes(pr.ntot, store)   # mixed64 reads the same FP32 hi bytes
+        per_elem = ebe_bytes(pr.ntot, store, prec)
         achieved = pr.nels * per_elem / (ebe_ms * 1e-3) / 1e9
         iter_ms = sum(v[0] for v in tk.values()) / nprof
         paths[prec] = {
@@ -360,7 +361,7 @@ def main():
             "ebe_ms": ebe_ms,
             "ebe_gbs": round(achieved, 1),
             "bytes_per_element": per_elem,
-            "iteration_gbs": round(iteration_bytes(pr.nels, pr.ntot, pr.neq, store) / (iter_ms * 1e-3) / 1e9, 1),
+            "iteration_gbs": round(iteration_bytes(pr.nels, pr.ntot, pr.neq, store, prec) / (iter_ms * 1e-3) / 1e9, 1),
             "setup_assembly_s": round(t_setup, 3),
             "device_gib": round(ctx.mem_used() / 2**30, 2),
         }
