@@ -175,7 +175,7 @@ template <int NT, int KIND, bool GATHER, typename OUT>
 __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
                                                  const float *__restrict__ Klo, const int *__restrict__ gg,
                                                  const double *__restrict__ src, OUT *__restrict__ out, int nels,
-                                                 long kstride, const pf_state *__restrict__ st, int gate)
+                                                 long kstride, const pf_state *__restrict__ st, int gate, long nlb)
 {
     if (skip(st, gate)) return;
     using L = Lanes<NT, KIND>;
@@ -187,7 +187,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
     const int sub = lane / L::W, li = lane % L::W;
     const int i = li < NT ? li : NT - 1;
     const int gb = sub * L::W;   // first lane of the element's group
-    const long ebase = (xcd_block(blockIdx.x, gridDim.x) * (PF_BLOCK / 64) + wave) * L::EPW * EPL;
+    // nlb logical blocks of PF_BLOCK/64 * EPW * EPL elements; a grid smaller
+    // than nlb (the gated true-residual products) loops over them.
+    for (long lb = xcd_block(blockIdx.x, gridDim.x); lb < nlb; lb += gridDim.x) {
+    const long ebase = (lb * (PF_BLOCK / 64) + wave) * L::EPW * EPL;
     long ev[EPL], ec[EPL];
     int g[EPL];
 #pragma unroll
@@ -313,6 +316,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
             }
             if (ev[t] < nels && li < NT) out[ev[t] * NT + li] = (OUT)acc;
         }
+    }
+    __builtin_amdgcn_wave_barrier();   // this wave's LDS reads finish before the next block's writes
     }
 }
 
@@ -689,9 +694,11 @@ __global__ void k_scatter_fix(const int *__restrict__ ptr, const int *__restrict
 template <int NT, int KIND, bool GATHER, typename OUT>
 int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate)
 {
-    const int grid = (c->nels + Lanes<NT, KIND>::EPB - 1) / Lanes<NT, KIND>::EPB;
+    const long nlb = (c->nels + Lanes<NT, KIND>::EPB - 1) / Lanes<NT, KIND>::EPB;
+    // A gated launch that usually does nothing dispatches few workgroups.
+    const int grid = gate == RUN_REFINE ? (int)std::min<long>(nlb, 2048) : (int)nlb;
     hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64, c->K32hi,
-                       c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate);
+                       c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }
