@@ -186,7 +186,7 @@ def block_bench(args):
         iters_one.append(r.iters)
         err = float(np.abs(X[:, c] - x).max() / np.abs(x).max())
         log(f"FP64 single solve {c}: {t_one[-1]:.3f} s, {r.iters} iterations, block vs single {err:.2e}")
-    t_seq = float(np.mean(t_one)) * k
+    t_seq = float(np.mean(t_one)) * k if t_one else None
     out = {
         "metric": METRIC,
         "value": round(k / t_block, 3),
@@ -207,7 +207,8 @@ def block_bench(args):
             "block_solve_s": t_block, "outer_steps": res.outer, "inner_iterations": res.inner_iters,
             "converged": res.converged, "max_relres": res.max_relres,
             "fp64_sequential_s_estimated": t_seq, "fp64_sample_solves": len(t_one),
-            "fp64_iterations": iters_one, "speedup_vs_fp64_sequential": round(t_seq / t_block, 3),
+            "fp64_iterations": iters_one,
+            "speedup_vs_fp64_sequential": round(t_seq / t_block, 3) if t_seq else None,
         },
         "roofline": {
             "bound": "mfma", "kernel": "k_blk_ebe<60, float> (v_mfma_f32_16x16x4f32)",

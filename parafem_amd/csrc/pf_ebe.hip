@@ -171,7 +171,7 @@ __device__ __forceinline__ double bpermute64(int addr, double v)
 // forms sum_d K(i,i+d) p(i+d) from p staged twice in LDS, then receives
 // K(i-d,i) p(i) from lane i-d (mod NT) for d = 1..NT/2-1 by ds_bpermute,
 // where lane j sends K(j,j+d) p(j): every entry is read from HBM once.
-template <int NT, int KIND, bool GATHER, typename OUT>
+template <int NT, int KIND, bool GATHER, typename OUT, bool LOOP = false>
 __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
                                                  const float *__restrict__ Klo, const int *__restrict__ gg,
                                                  const double *__restrict__ src, OUT *__restrict__ out, int nels,
@@ -187,9 +187,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
     const int sub = lane / L::W, li = lane % L::W;
     const int i = li < NT ? li : NT - 1;
     const int gb = sub * L::W;   // first lane of the element's group
-    // nlb logical blocks of PF_BLOCK/64 * EPW * EPL elements; a grid smaller
-    // than nlb (the gated true-residual products) loops over them.
-    for (long lb = xcd_block(blockIdx.x, gridDim.x); lb < nlb; lb += gridDim.x) {
+    // nlb logical blocks of PF_BLOCK/64 * EPW * EPL elements; with LOOP a
+    // grid smaller than nlb (the gated true-residual products) loops over
+    // them, otherwise every workgroup takes one.
+    auto body = [&](long lb) {
     const long ebase = (lb * (PF_BLOCK / 64) + wave) * L::EPW * EPL;
     long ev[EPL], ec[EPL];
     int g[EPL];
@@ -317,7 +318,15 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
             if (ev[t] < nels && li < NT) out[ev[t] * NT + li] = (OUT)acc;
         }
     }
-    __builtin_amdgcn_wave_barrier();   // this wave's LDS reads finish before the next block's writes
+    };
+    const long lb0 = xcd_block(blockIdx.x, gridDim.x);
+    if constexpr (LOOP) {
+        for (long lb = lb0; lb < nlb; lb += gridDim.x) {
+            body(lb);
+            __builtin_amdgcn_wave_barrier();   // this wave's LDS reads finish before the next block's writes
+        }
+    } else {
+        body(lb0);
     }
 }
 
@@ -695,10 +704,15 @@ template <int NT, int KIND, bool GATHER, typename OUT>
 int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate)
 {
     const long nlb = (c->nels + Lanes<NT, KIND>::EPB - 1) / Lanes<NT, KIND>::EPB;
-    // A gated launch that usually does nothing dispatches few workgroups.
-    const int grid = gate == RUN_REFINE ? (int)std::min<long>(nlb, 2048) : (int)nlb;
-    hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64, c->K32hi,
-                       c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
+    if (gate == RUN_REFINE) {
+        // usually does nothing: dispatch few workgroups that loop
+        const int grid = (int)std::min<long>(nlb, 2048);
+        hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, true>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64,
+                           c->K32hi, c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
+    } else {
+        hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, false>), dim3((int)nlb), dim3(PF_BLOCK), 0, c->stream,
+                           c->K64, c->K32hi, c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
+    }
     PF_HIP(hipGetLastError());
     return PF_OK;
 }
