@@ -187,8 +187,20 @@ __device__ void blk_fold(const double *partials, int G, double *out)
 {
     __shared__ double ws[4][PF_BLK];
     const int col = threadIdx.x & 63, part = threadIdx.x >> 6;
+    // thread (col, part) adds blocks part, part+4, ... in that order, the
+    // loads of a batch of 16 issued before the dependent additions
     double s = 0.0;
-    for (int b = part; b < G; b += 4) s = __dadd_rn(s, partials[(long)b * PF_BLK + col]);
+    for (int b0 = part; b0 < G; b0 += 64) {
+        double t[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int b = b0 + 4 * u;
+            t[u] = b < G ? partials[(long)b * PF_BLK + col] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (b0 + 4 * u < G) s = __dadd_rn(s, t[u]);
+    }
     ws[part][col] = s;
     __syncthreads();
     if (threadIdx.x < PF_BLK) {
@@ -211,8 +223,9 @@ __global__ __launch_bounds__(256) void k_blk_fold_alpha(const double *partials, 
     }
 }
 
-// x += alpha p, r -= alpha u, d = dinv r; partials of (r,d) and (r,r).
-__global__ __launch_bounds__(256) void k_blk_update(float *__restrict__ X, float *__restrict__ R, float *__restrict__ D,
+// x += alpha p, r -= alpha u; partials of (r, dinv r) and (r,r).  The
+// preconditioned residual d = dinv r is not stored: k_blk_pupdate forms it.
+__global__ __launch_bounds__(256) void k_blk_update(float *__restrict__ X, float *__restrict__ R,
                                                    const float *__restrict__ P, const float *__restrict__ U,
                                                    const float *__restrict__ dinv, int neq, double *partials,
                                                    const pf_blk_state *st)
@@ -228,18 +241,16 @@ __global__ __launch_bounds__(256) void k_blk_update(float *__restrict__ X, float
         f4_t x = *reinterpret_cast<const f4_t *>(X + o), r = *reinterpret_cast<const f4_t *>(R + o);
         const f4_t p = *reinterpret_cast<const f4_t *>(P + o), u = *reinterpret_cast<const f4_t *>(U + o);
         const float di = dinv[q + 1];
-        f4_t d;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             x[k] = __fmaf_rn(a[k], p[k], x[k]);
             r[k] = __fmaf_rn(-a[k], u[k], r[k]);
-            d[k] = di * r[k];
-            rd[k] = __dadd_rn(rd[k], __dmul_rn((double)r[k], (double)d[k]));
+            const float d = di * r[k];
+            rd[k] = __dadd_rn(rd[k], __dmul_rn((double)r[k], (double)d));
             rr[k] = __dadd_rn(rr[k], __dmul_rn((double)r[k], (double)r[k]));
         }
         *reinterpret_cast<f4_t *>(X + o) = x;
         *reinterpret_cast<f4_t *>(R + o) = r;
-        *reinterpret_cast<f4_t *>(D + o) = d;
     }
     blk_block_partials(rd, partials);
     blk_block_partials(rr, partials + (long)PF_BLK_MAXG * PF_BLK);
@@ -276,8 +287,9 @@ __global__ __launch_bounds__(256) void k_blk_fold_update(const double *partials,
     }
 }
 
-__global__ __launch_bounds__(256) void k_blk_pupdate(float *__restrict__ P, const float *__restrict__ D, int neq,
-                                                    const pf_blk_state *st)
+// p = dinv r + beta p
+__global__ __launch_bounds__(256) void k_blk_pupdate(float *__restrict__ P, const float *__restrict__ R,
+                                                    const float *__restrict__ dinv, int neq, const pf_blk_state *st)
 {
     if (st->done) return;
     const int cl = threadIdx.x & 15, rl = threadIdx.x >> 4;
@@ -287,9 +299,10 @@ __global__ __launch_bounds__(256) void k_blk_pupdate(float *__restrict__ P, cons
     for (int q = blockIdx.x * 16 + rl; q < neq; q += gridDim.x * 16) {
         const long o = (long)(q + 1) * PF_BLK + 4 * cl;
         f4_t p = *reinterpret_cast<const f4_t *>(P + o);
-        const f4_t d = *reinterpret_cast<const f4_t *>(D + o);
+        const f4_t r = *reinterpret_cast<const f4_t *>(R + o);
+        const float di = dinv[q + 1];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) p[k] = __fmaf_rn(b[k], p[k], d[k]);
+        for (int k = 0; k < 4; ++k) p[k] = __fmaf_rn(b[k], p[k], di * r[k]);
         *reinterpret_cast<f4_t *>(P + o) = p;
     }
 }
@@ -602,17 +615,23 @@ int pf_block_time_ebe(pf_ctx *c, int reps, double *ms)
     PF_HIP(hipEventCreate(&t0));
     PF_HIP(hipEventCreate(&t1));
     int rc = blk_launch_ebe<float, 4>(c, b->Kt32, b->P, b->Ut32, nullptr);   // warm
-    hipError_t e = hipEventRecord(t0, c->stream);
-    for (int i = 0; !rc && e == hipSuccess && i < reps; ++i) rc = blk_launch_ebe<float, 4>(c, b->Kt32, b->P, b->Ut32, nullptr);
-    if (e == hipSuccess) e = hipEventRecord(t1, c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(t1);
-    float t = 0.0f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
+    hipError_t e = hipSuccess;
+    double total = 0.0;
+    // one event pair around each launch: the kernel's own duration
+    for (int i = 0; !rc && e == hipSuccess && i < reps; ++i) {
+        e = hipEventRecord(t0, c->stream);
+        if (e == hipSuccess) rc = blk_launch_ebe<float, 4>(c, b->Kt32, b->P, b->Ut32, nullptr);
+        if (e == hipSuccess && !rc) e = hipEventRecord(t1, c->stream);
+        if (e == hipSuccess && !rc) e = hipEventSynchronize(t1);
+        float t = 0.0f;
+        if (e == hipSuccess && !rc) e = hipEventElapsedTime(&t, t0, t1);
+        total += t;
+    }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (rc) return rc;
     if (e != hipSuccess) return pf_set_hip_error(e, "block timing", __FILE__, __LINE__);
-    *ms = t / reps;
+    *ms = total / reps;
     return PF_OK;
 }
 
@@ -689,10 +708,11 @@ int pf_block_solve(pf_ctx *c, int k, const double *f, double *x, const pf_block_
                 hipLaunchKernelGGL((k_blk_scatter<float, true>), dim3(G), dim3(256), 0, c->stream, c->csr_ptr,
                                    c->csr_slot, b->Ut32, b->P, b->U, c->neq, b->partials, b->st);
                 hipLaunchKernelGGL(k_blk_fold_alpha, dim3(1), dim3(256), 0, c->stream, b->partials, G, b->st);
-                hipLaunchKernelGGL(k_blk_update, dim3(G), dim3(256), 0, c->stream, b->X, b->R, b->D, b->P, b->U,
-                                   b->dinv, c->neq, b->partials, b->st);
+                hipLaunchKernelGGL(k_blk_update, dim3(G), dim3(256), 0, c->stream, b->X, b->R, b->P, b->U, b->dinv,
+                                   c->neq, b->partials, b->st);
                 hipLaunchKernelGGL(k_blk_fold_update, dim3(1), dim3(256), 0, c->stream, b->partials, G, b->st);
-                hipLaunchKernelGGL(k_blk_pupdate, dim3(G), dim3(256), 0, c->stream, b->P, b->D, c->neq, b->st);
+                hipLaunchKernelGGL(k_blk_pupdate, dim3(G), dim3(256), 0, c->stream, b->P, b->R, b->dinv, c->neq,
+                                   b->st);
             }
             PF_HIP(hipGetLastError());
             PF_HIP(hipMemcpyAsync(b->st_host, b->st, sizeof(pf_blk_state), hipMemcpyDeviceToHost, c->stream));
