@@ -249,25 +249,26 @@ int pf_stress_centroid(pf_ctx *ctx, const double *x, double e, double v, const d
 /* ------------------------------------------------- multi-RHS block solve */
 /* No counterpart in the reference: up to 64 load cases on one mesh solved
  * together.  The element product for all of them, K_e (ntot x ntot) times
- * the gathered (ntot x k) block, runs on the matrix cores (FP32 MFMA for the
- * inner solves, FP64 MFMA for the true residuals); the iteration per column
- * is p121's Jacobi PCG (p121.f90:86-104), batched as PCG_KM batches
- * MATMUL(km, pmul_pp) (src/modules/mpi/maths.f90:1152-1323).  Solver: FP64
- * defect correction, each step an FP32 block PCG (per-column alpha/beta,
- * columns frozen once their residual fell by inner_tol) on K d = r, then
- * x += d and r = f - K x in FP64.  Needs FP64 element matrices
+ * the gathered (ntot x k) block, runs on the matrix cores (FP32 MFMA in the
+ * iteration, FP64 MFMA for the true residuals); the iteration per column is
+ * p121's Jacobi PCG (p121.f90:86-104), batched as PCG_KM batches
+ * MATMUL(km, pmul_pp) (src/modules/mpi/maths.f90:1152-1323).  Solver: FP32
+ * block PCG (per-column alpha/beta) with FP64 residual replacement: when a
+ * column's recurrence residual has fallen by inner_tol since the last
+ * replacement, x is accumulated in FP64, r = f - K x is recomputed in FP64
+ * and the iteration continues from it.  Needs FP64 element matrices
  * (PF_STORE_FP64, full or shared) and no p123 fixed freedoms. */
 typedef struct {
     double tol;          /* per column: ||f - K x||_2 <= tol ||f||_2          */
-    double inner_tol;    /* FP32 inner solve: residual reduction (0 = 1e-5)   */
-    int    inner_limit;  /* iterations per inner solve (0 = 20000)            */
-    int    outer_limit;  /* defect-correction steps (0 = 10)                  */
-    int    check_every;  /* host polls the inner solve every n iterations (0=32) */
+    double inner_tol;    /* residual drop that triggers a replacement (0=1e-2) */
+    int    inner_limit;  /* block iterations (0 = 100000)                     */
+    int    outer_limit;  /* residual replacements (0 = no limit)              */
+    int    check_every;  /* host polls every n iterations (0 = 32)            */
 } pf_block_opts;
 
 typedef struct {
-    int    outer;        /* defect-correction steps taken                     */
-    int    inner_iters;  /* inner iterations over all steps                   */
+    int    outer;        /* residual replacements                             */
+    int    inner_iters;  /* block iterations                                  */
     int    converged;    /* columns that met tol                               */
     double max_relres;   /* largest true relative residual over the columns   */
     int    status;       /* PF_OK or PF_NOT_CONVERGED                          */

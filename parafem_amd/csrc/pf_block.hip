@@ -14,12 +14,13 @@
 // position pf_blk_pos(c), so the B operand of lane l for all four column
 // tiles is one vector load from the gathered row.
 //
-// Solver: FP64 defect correction around FP32 block PCG.  Each outer step
-// solves K d = r with per-column Jacobi PCG in FP32 (vectors FP32, dots
-// accumulated in FP64, every column with its own alpha and beta, a column
-// frozen once its residual fell by inner_tol), adds d to the FP64 solution
-// and recomputes r = f - K x in FP64 with FP64 element matrices.  Stops when
-// every column's true relative residual is below tol.
+// Solver: one FP32 block PCG (vectors FP32, dots accumulated in FP64, every
+// column with its own alpha and beta) with residual replacement in FP64:
+// when a column's recurrence residual has fallen by delta since its last
+// replacement (or below tol), the FP32 solution is added into the FP64 one,
+// r = f - K x is recomputed with the FP64 element matrices, and the FP32
+// recurrence continues from fl32(r) with the same search directions.  A
+// column stops once its true relative residual is below tol.
 //
 // The reference has no multi-RHS solve; the single-RHS iteration it batches
 // is p121.f90:86-104 and its identical-K_e variant PCG_KM
@@ -256,16 +257,17 @@ __global__ __launch_bounds__(256) void k_blk_update(float *__restrict__ X, float
     blk_block_partials(rr, partials + (long)PF_BLK_MAXG * PF_BLK);
 }
 
-// beta = (r,d)/up, per-column convergence (rr <= tol2 rr0), iteration count,
-// done once every column is frozen or at the limit.
+// beta = (r,d)/up per active column; a column whose recurrence residual fell
+// by delta since its last replacement, or below tol ||f||, asks for a
+// residual replacement (trig); done at the iteration limit.
 __global__ __launch_bounds__(256) void k_blk_fold_update(const double *partials, int G, pf_blk_state *st)
 {
     if (st->done) return;
     __shared__ double rd[PF_BLK], rr[PF_BLK];
-    __shared__ int active;
+    __shared__ int trig;
     blk_fold(partials, G, rd);
     blk_fold(partials + (long)PF_BLK_MAXG * PF_BLK, G, rr);
-    if (threadIdx.x == 0) active = 0;
+    if (threadIdx.x == 0) trig = 0;
     __syncthreads();
     if (threadIdx.x < PF_BLK) {
         const int c = threadIdx.x;
@@ -274,16 +276,16 @@ __global__ __launch_bounds__(256) void k_blk_fold_update(const double *partials,
             st->beta[c] = isfinite(b) ? b : 0.0;
             st->up[c] = rd[c];
             st->rr[c] = rr[c];
-            if (!(rr[c] > st->tol2 * st->rr0[c])) st->frozen[c] = 1;
+            if (!(rr[c] > st->delta2 * st->rr_ref[c]) || !(rr[c] > st->tol2 * st->ff[c])) atomicAdd(&trig, 1);
         } else {
             st->beta[c] = 0.0;
         }
-        if (!st->frozen[c]) atomicAdd(&active, 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         st->iters += 1;
-        if (active == 0 || st->iters >= st->limit) st->done = 1;
+        if (trig) st->trig = 1;
+        if (st->iters >= st->limit) st->done = 1;
     }
 }
 
@@ -307,11 +309,13 @@ __global__ __launch_bounds__(256) void k_blk_pupdate(float *__restrict__ P, cons
     }
 }
 
-// Inner start: R = fl32(R64), X = 0, D = dinv R, P = D; partials (r,d), (r,r).
+// R = fl32(R64) and partials (r, dinv r), (r,r); START also X = 0 and
+// P = dinv R (the first search directions).  Without START this is the
+// residual replacement: x and p are kept.
+template <bool START>
 __global__ __launch_bounds__(256) void k_blk_inner_init(float *__restrict__ X, float *__restrict__ R,
-                                                       float *__restrict__ D, float *__restrict__ P,
-                                                       const double *__restrict__ R64, const float *__restrict__ dinv,
-                                                       int neq, double *partials)
+                                                       float *__restrict__ P, const double *__restrict__ R64,
+                                                       const float *__restrict__ dinv, int neq, double *partials)
 {
     const int cl = threadIdx.x & 15, rl = threadIdx.x >> 4;
     double rd[4] = {0, 0, 0, 0}, rr[4] = {0, 0, 0, 0};
@@ -326,15 +330,18 @@ __global__ __launch_bounds__(256) void k_blk_inner_init(float *__restrict__ X, f
             rd[k] = __dadd_rn(rd[k], __dmul_rn((double)r[k], (double)d[k]));
             rr[k] = __dadd_rn(rr[k], __dmul_rn((double)r[k], (double)r[k]));
         }
-        *reinterpret_cast<f4_t *>(X + o) = f4_t{0.f, 0.f, 0.f, 0.f};
         *reinterpret_cast<f4_t *>(R + o) = r;
-        *reinterpret_cast<f4_t *>(D + o) = d;
-        *reinterpret_cast<f4_t *>(P + o) = d;
+        if (START) {
+            *reinterpret_cast<f4_t *>(X + o) = f4_t{0.f, 0.f, 0.f, 0.f};
+            *reinterpret_cast<f4_t *>(P + o) = d;
+        }
     }
     blk_block_partials(rd, partials);
     blk_block_partials(rr, partials + (long)PF_BLK_MAXG * PF_BLK);
 }
 
+// After a start or a replacement: up = (r,d), rr = rr_ref = (r,r), no
+// pending request.  Frozen flags and the tolerances are set by the host.
 __global__ __launch_bounds__(256) void k_blk_fold_init(const double *partials, int G, pf_blk_state *st)
 {
     __shared__ double rd[PF_BLK], rr[PF_BLK];
@@ -343,22 +350,21 @@ __global__ __launch_bounds__(256) void k_blk_fold_init(const double *partials, i
     if (threadIdx.x < PF_BLK) {
         const int c = threadIdx.x;
         st->up[c] = rd[c];
-        st->rr0[c] = rr[c];
         st->rr[c] = rr[c];
+        st->rr_ref[c] = rr[c];
         st->alpha[c] = st->beta[c] = 0.0;
-        st->frozen[c] = !(rr[c] > 0.0);
     }
-    if (threadIdx.x == 0) {
-        st->iters = 0;
-        st->done = 0;
-    }
+    if (threadIdx.x == 0) st->trig = 0;
 }
 
-// X64 += X (FP32 correction); R64 = F64 - U64 with partials of (r,r).
-__global__ __launch_bounds__(256) void k_blk_correct(double *__restrict__ X64, const float *__restrict__ X, long n)
+// X64 += X (FP32 correction), then X = 0 when CLEAR.
+template <bool CLEAR>
+__global__ __launch_bounds__(256) void k_blk_correct(double *__restrict__ X64, float *__restrict__ X, long n)
 {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         X64[i + PF_BLK] = __dadd_rn(X64[i + PF_BLK], (double)X[i + PF_BLK]);
+        if (CLEAR) X[i + PF_BLK] = 0.0f;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_blk_residual(double *__restrict__ R64, const double *__restrict__ F64,
@@ -645,9 +651,9 @@ int pf_block_solve(pf_ctx *c, int k, const double *f, double *x, const pf_block_
     int rc = blk_setup(c, true);
     if (rc) return rc;
     pf_block *b = c->blk;
-    const double inner_tol = o->inner_tol > 0.0 ? o->inner_tol : 1e-5;
-    const int inner_limit = o->inner_limit > 0 ? o->inner_limit : 20000;
-    const int outer_limit = o->outer_limit > 0 ? o->outer_limit : 10;
+    const double delta = o->inner_tol > 0.0 ? o->inner_tol : 1e-2;
+    const int limit = o->inner_limit > 0 ? o->inner_limit : 100000;
+    const int max_repl = o->outer_limit > 0 ? o->outer_limit : 100000;
     const int check = o->check_every > 0 ? o->check_every : 32;
     const size_t nh = (size_t)c->neq * k;
     const size_t nv = (size_t)(c->neq + 1) * PF_BLK;
@@ -661,46 +667,46 @@ int pf_block_solve(pf_ctx *c, int k, const double *f, double *x, const pf_block_
     double ff[PF_BLK], rr[PF_BLK];
     rc = blk_true_residual(c, ff);             // x = 0: r = f, ||f||^2
     if (rc) return rc;
-    for (int outer = 0;; ++outer) {
-        if (outer > 0) {
-            rc = blk_true_residual(c, rr);
-            if (rc) return rc;
-        } else {
-            std::copy(ff, ff + PF_BLK, rr);
-        }
-        double worst = 0.0;
+    std::copy(ff, ff + PF_BLK, rr);
+    // Start: x = 0 in FP32, p = d = dinv fl32(r).
+    hipLaunchKernelGGL(k_blk_inner_init<true>, dim3(G), dim3(256), 0, c->stream, b->X, b->R, b->P, b->R64, b->dinv,
+                       c->neq, b->partials);
+    hipLaunchKernelGGL(k_blk_fold_init, dim3(1), dim3(256), 0, c->stream, b->partials, G, b->st);
+    PF_HIP(hipGetLastError());
+    bool first = true;
+    for (;;) {
+        // Columns whose true residual meets tol (or that are unused) stop.
         int conv = 0;
-        for (int col = 0; col < k; ++col) {
-            const int p = pf_blk_pos(col);
-            const double rel = ff[p] > 0.0 ? std::sqrt(rr[p] / ff[p]) : 0.0;
-            worst = std::max(worst, rel);
-            conv += rel <= o->tol;
-        }
-        R.outer = outer;
-        R.converged = conv;
-        R.max_relres = worst;
-        if (conv == k || outer >= outer_limit) break;
-        // FP32 inner block PCG on K d = r.
-        hipLaunchKernelGGL(k_blk_inner_init, dim3(G), dim3(256), 0, c->stream, b->X, b->R, b->D, b->P, b->R64,
-                           b->dinv, c->neq, b->partials);
-        hipLaunchKernelGGL(k_blk_fold_init, dim3(1), dim3(256), 0, c->stream, b->partials, G, b->st);
-        pf_blk_state s0{};
+        double worst = 0.0;
         PF_HIP(hipMemcpyAsync(b->st_host, b->st, sizeof(pf_blk_state), hipMemcpyDeviceToHost, c->stream));
         PF_HIP(hipStreamSynchronize(c->stream));
-        s0 = *b->st_host;
-        // per-column stop: ||r|| <= inner_tol ||r0||; columns already below the
-        // outer tolerance start frozen
+        pf_blk_state s0 = *b->st_host;
         for (int col = 0; col < PF_BLK; ++col) {
             const int p = pf_blk_pos(col);
-            s0.frozen[p] = s0.frozen[p] || col >= k || (ff[p] > 0.0 && std::sqrt(rr[p] / ff[p]) <= o->tol);
+            const double rel = ff[p] > 0.0 ? std::sqrt(rr[p] / ff[p]) : 0.0;
+            const bool done = col >= k || rel <= o->tol;
+            if (col < k) {
+                worst = std::max(worst, rel);
+                conv += done;
+            }
+            s0.frozen[p] = done;
+            s0.ff[p] = ff[p];
         }
-        s0.tol2 = inner_tol * inner_tol;
-        s0.limit = inner_limit;
-        s0.iters = 0;
-        s0.done = 0;
+        R.converged = conv;
+        R.max_relres = worst;
+        if (conv == k || R.outer >= max_repl || (!first && s0.iters >= limit)) break;
+        if (first) {
+            s0.iters = 0;
+            s0.limit = limit;
+            s0.done = 0;
+        }
+        s0.tol2 = o->tol * o->tol;
+        s0.delta2 = delta * delta;
+        s0.trig = 0;
         *b->st_host = s0;
         PF_HIP(hipMemcpyAsync(b->st, b->st_host, sizeof(pf_blk_state), hipMemcpyHostToDevice, c->stream));
-        PF_HIP(hipStreamSynchronize(c->stream));
+        first = false;
+        // FP32 block PCG until some column asks for a residual replacement.
         for (;;) {
             for (int it = 0; it < check; ++it) {
                 rc = blk_launch_ebe<float, 4>(c, b->Kt32, b->P, b->Ut32, b->st);
@@ -717,11 +723,21 @@ int pf_block_solve(pf_ctx *c, int k, const double *f, double *x, const pf_block_
             PF_HIP(hipGetLastError());
             PF_HIP(hipMemcpyAsync(b->st_host, b->st, sizeof(pf_blk_state), hipMemcpyDeviceToHost, c->stream));
             PF_HIP(hipStreamSynchronize(c->stream));
-            if (b->st_host->done) break;
+            if (b->st_host->trig || b->st_host->done) break;
         }
-        R.inner_iters += b->st_host->iters;
-        hipLaunchKernelGGL(k_blk_correct, dim3(8192), dim3(256), 0, c->stream, b->X64, b->X, (long)c->neq * PF_BLK);
+        R.inner_iters = b->st_host->iters;
+        // Residual replacement: x += the FP32 correction, r = f - K x in FP64,
+        // the FP32 recurrence restarts from fl32(r) with the same p.
+        hipLaunchKernelGGL(k_blk_correct<true>, dim3(8192), dim3(256), 0, c->stream, b->X64, b->X,
+                           (long)c->neq * PF_BLK);
         PF_HIP(hipGetLastError());
+        rc = blk_true_residual(c, rr);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_blk_inner_init<false>, dim3(G), dim3(256), 0, c->stream, b->X, b->R, b->P, b->R64,
+                           b->dinv, c->neq, b->partials);
+        hipLaunchKernelGGL(k_blk_fold_init, dim3(1), dim3(256), 0, c->stream, b->partials, G, b->st);
+        PF_HIP(hipGetLastError());
+        R.outer += 1;
     }
     hipLaunchKernelGGL(k_blk_to_host, dim3(8192), dim3(256), 0, c->stream, b->X64, b->stage, c->neq, k);
     PF_HIP(hipGetLastError());

@@ -89,10 +89,12 @@ struct pf_state {
 __host__ __device__ inline int pf_blk_pos(int c) { return (c & 15) * 4 + (c >> 4); }
 
 struct pf_blk_state {
-    double up[PF_BLK], alpha[PF_BLK], beta[PF_BLK], rr0[PF_BLK], rr[PF_BLK];
-    int frozen[PF_BLK];
-    int iters, limit, done, spare;
-    double tol2;
+    double up[PF_BLK], alpha[PF_BLK], beta[PF_BLK], rr[PF_BLK];
+    double rr_ref[PF_BLK];   // ||r||^2 at the column's last residual replacement
+    double ff[PF_BLK];       // ||f||^2
+    int frozen[PF_BLK];      // converged (true residual) or unused column
+    int iters, limit, done, trig;   // trig: some column asks for a replacement
+    double tol2, delta2;
 };
 
 struct pf_block {
