@@ -256,11 +256,12 @@ int pf_stress_centroid(pf_ctx *ctx, const double *x, double e, double v, const d
  * block PCG (per-column alpha/beta) with FP64 residual replacement: when a
  * column's recurrence residual has fallen by inner_tol since the last
  * replacement, x is accumulated in FP64, r = f - K x is recomputed in FP64
- * and the iteration continues from it.  Needs FP64 element matrices
+ * and the iteration continues from it (from p = dinv r again if a column's
+ * true residual grew).  Needs FP64 element matrices
  * (PF_STORE_FP64, full or shared) and no p123 fixed freedoms. */
 typedef struct {
     double tol;          /* per column: ||f - K x||_2 <= tol ||f||_2          */
-    double inner_tol;    /* residual drop that triggers a replacement (0=1e-2) */
+    double inner_tol;    /* residual drop that triggers a replacement (0=0.1) */
     int    inner_limit;  /* block iterations (0 = 100000)                     */
     int    outer_limit;  /* residual replacements (0 = no limit)              */
     int    check_every;  /* host polls every n iterations (0 = 32)            */

@@ -651,7 +651,7 @@ int pf_block_solve(pf_ctx *c, int k, const double *f, double *x, const pf_block_
     int rc = blk_setup(c, true);
     if (rc) return rc;
     pf_block *b = c->blk;
-    const double delta = o->inner_tol > 0.0 ? o->inner_tol : 1e-2;
+    const double delta = o->inner_tol > 0.0 ? o->inner_tol : 0.1;
     const int limit = o->inner_limit > 0 ? o->inner_limit : 100000;
     const int max_repl = o->outer_limit > 0 ? o->outer_limit : 100000;
     const int check = o->check_every > 0 ? o->check_every : 32;
@@ -731,10 +731,24 @@ int pf_block_solve(pf_ctx *c, int k, const double *f, double *x, const pf_block_
         hipLaunchKernelGGL(k_blk_correct<true>, dim3(8192), dim3(256), 0, c->stream, b->X64, b->X,
                            (long)c->neq * PF_BLK);
         PF_HIP(hipGetLastError());
+        double prev[PF_BLK];
+        std::copy(rr, rr + PF_BLK, prev);
         rc = blk_true_residual(c, rr);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_blk_inner_init<false>, dim3(G), dim3(256), 0, c->stream, b->X, b->R, b->P, b->R64,
-                           b->dinv, c->neq, b->partials);
+        // A column whose true residual grew since the last replacement has
+        // search directions spoilt by FP32 rounding: every column restarts
+        // from p = dinv r (x is already in the FP64 sum).
+        bool grew = false;
+        for (int col = 0; col < k; ++col) {
+            const int p = pf_blk_pos(col);
+            grew |= !b->st_host->frozen[p] && rr[p] > 4.0 * prev[p];
+        }
+        if (grew)
+            hipLaunchKernelGGL(k_blk_inner_init<true>, dim3(G), dim3(256), 0, c->stream, b->X, b->R, b->P, b->R64,
+                               b->dinv, c->neq, b->partials);
+        else
+            hipLaunchKernelGGL(k_blk_inner_init<false>, dim3(G), dim3(256), 0, c->stream, b->X, b->R, b->P, b->R64,
+                               b->dinv, c->neq, b->partials);
         hipLaunchKernelGGL(k_blk_fold_init, dim3(1), dim3(256), 0, c->stream, b->partials, G, b->st);
         PF_HIP(hipGetLastError());
         R.outer += 1;
