@@ -80,11 +80,12 @@ def time_iterations(ctx, pr, precision, steps, warmup):
 
 
 def solve(ctx, pr, precision, store, tol):
-    from parafem_amd import problems as P
+    from parafem_amd import problems as P, tune as T
     t0 = time.perf_counter()
     r = P.setup(ctx, pr, store=store)
     ctx.sync()
     t_setup = time.perf_counter() - t0
+    T.tune_ebe(ctx, r, precision)          # the shape tuned for this path (cached)
     t0 = time.perf_counter()
     x, res = ctx.pcg_solve(r, 100000, tol, stop="relres", precision=precision, check_every=64)
     t_solve = time.perf_counter() - t0
@@ -155,7 +156,7 @@ def block_bench(args):
     PCG, FP64 residual replacement) to relres 1e-10 per case, against plain
     FP64 single-RHS solves of the same cases one after another (a sample of
     them timed, the time scaled to all 64)."""
-    from parafem_amd import Context, problems as P
+    from parafem_amd import Context, problems as P, tune as T
     n = args.n or 60
     k = 64
     t0 = time.perf_counter()
@@ -164,6 +165,9 @@ def block_bench(args):
     ctx = Context(0)
     P.setup(ctx, pr, store="fp64")
     ctx.sync()
+    shape = T.tune_block(ctx)
+    log(f"block: autotuned {shape['blk_waves']} elements per workgroup, {shape['blk_rows']} row tiles per pass "
+        f"({'cache' if shape['cached'] else 'timed ' + str(shape['ms'])})")
     ctx.block_solve(F[:, :1], 1e-4, outer_limit=1)          # warm: packing, allocation, code paths
     t0 = time.perf_counter()
     X, res = ctx.block_solve(F, 1e-10)
@@ -209,6 +213,7 @@ def block_bench(args):
             "fp64_sequential_s_estimated": t_seq, "fp64_sample_solves": len(t_one),
             "fp64_iterations": iters_one,
             "speedup_vs_fp64_sequential": round(t_seq / t_block, 3) if t_seq else None,
+            "tile_shape": {"blk_waves": shape["blk_waves"], "blk_rows": shape["blk_rows"], "source": "autotuner"},
         },
         "roofline": {
             "bound": "mfma", "kernel": "k_blk_ebe<60, float> (v_mfma_f32_16x16x4f32)",
@@ -316,7 +321,7 @@ def main():
                           "sharding is a design only (see DESIGN.md); use --gpus 1"}))
         sys.exit(2)
 
-    from parafem_amd import Context, DeviceArray, problems as P
+    from parafem_amd import Context, DeviceArray, problems as P, tune as T
 
     if args.copy_calibration:
         ctx = Context(0)
@@ -348,6 +353,9 @@ def main():
         P.setup(ctx, pr, store=store)
         ctx.sync()
         t_setup = time.perf_counter() - t0
+        shape = T.tune_ebe(ctx, pr.r0, prec)
+        log(f"{prec}: autotuned elements per lane group {shape['ebe_epl']} "
+            f"({'cache' if shape['cached'] else 'timed ' + str(shape['ms'])})")
         elapsed, tk, nprof = time_iterations(ctx, pr, prec, args.steps, args.warmup)
         ebe_ms = tk["ebe"][0] / max(tk["ebe"][1], 1)
         per_elem = ebe_bytes(pr.ntot, store, prec)
@@ -365,6 +373,7 @@ def main():
             "iteration_gbs": round(iteration_bytes(pr.nels, pr.ntot, pr.neq, store, prec) / (iter_ms * 1e-3) / 1e9, 1),
             "setup_assembly_s": round(t_setup, 3),
             "device_gib": round(ctx.mem_used() / 2**30, 2),
+            "tile_shape": {"ebe_epl": shape["ebe_epl"], "source": "autotuner"},
         }
         log(f"{prec}: {paths[prec]['iter_per_s']} iter/s, element products {ebe_ms:.3f} ms "
             f"({achieved:.0f} GB/s), kernels " +

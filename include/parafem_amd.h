@@ -283,8 +283,33 @@ int pf_block_solve(pf_ctx *ctx, int k, const double *f, double *x, const pf_bloc
  * scatter).  Host, neq x k column-major. */
 int pf_block_apply(pf_ctx *ctx, int k, const double *x, double *y, int precision);
 /* Measurement: average ms of the FP32 matrix-core element product over reps
- * launches, HIP events on the context's stream (after a block solve). */
+ * launches, HIP events on the context's stream, applied to the FP32 block
+ * the last pf_block_apply(PF_PREC_MIXED) converted or the last residual of
+ * a block solve. */
 int pf_block_time_ebe(pf_ctx *ctx, int reps, double *ms);
+
+/* ------------------------------------------------------- launch shapes */
+/* No counterpart in the reference (its element loop has no launch shape):
+ * the shapes of the element-product kernels, picked per (ntot, precision,
+ * k, device) by the autotuner in parafem_amd/tune.py and set per context.
+ * 0 keeps the built-in choice.  None of them changes a result: every lane
+ * forms its row in the same order whatever the shape.
+ *   ebe_epl:   single-RHS products, elements a lane group loads before it
+ *              multiplies: 1 or 2 (ntot 60), 1, 2 or 4 (ntot 24), 1, 2, 4
+ *              or 8 (ntot 8);
+ *   blk_waves: block products, elements (wavefronts) per workgroup: 1, 2, 4;
+ *   blk_rows:  FP32 block products, 16-row output tiles per pass: 1, 2, 4.
+ * Setting them drops a captured iteration chunk. */
+typedef struct {
+    int ebe_epl;
+    int blk_waves;
+    int blk_rows;
+} pf_tune;
+int pf_tune_set(pf_ctx *ctx, const pf_tune *t);
+int pf_tune_get(pf_ctx *ctx, pf_tune *t);
+/* The device's architecture name (gcnArchName, e.g. "gfx950:sramecc+:xnack-")
+ * and compute-unit count: the autotuner's cache key. */
+int pf_device_info(pf_ctx *ctx, char *arch, int len, int *cus);
 
 /* ------------------------------------------------------------ timing */
 /* No counterpart in the reference (its timing is CPU_TIME around phases,

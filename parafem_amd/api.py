@@ -68,6 +68,10 @@ class BlockOpts(C.Structure):
                 ("outer_limit", C.c_int), ("check_every", C.c_int)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("ebe_epl", C.c_int), ("blk_waves", C.c_int), ("blk_rows", C.c_int)]
+
+
 class BlockResult(C.Structure):
     _fields_ = [("outer", C.c_int), ("inner_iters", C.c_int), ("converged", C.c_int),
                 ("max_relres", C.c_double), ("status", C.c_int)]
@@ -125,6 +129,9 @@ def lib():
         "pf_block_time_ebe": (i32, [vp, i32, pd]),
         "pf_stress_centroid": (i32, [vp, pd, dbl, dbl, pd, pd]),
         "pf_trim": (i32, [vp]),
+        "pf_tune_set": (i32, [vp, C.POINTER(Tune)]),
+        "pf_tune_get": (i32, [vp, C.POINTER(Tune)]),
+        "pf_device_info": (i32, [vp, C.c_char_p, i32, pi]),
         "pf_vec_fill": (i32, [vp, vp, dbl, i64]),
         "pf_vec_waxpy": (i32, [vp, vp, vp, vp, dbl, i64]),
         "pf_vec_mul": (i32, [vp, vp, vp, vp, i64]),
@@ -509,10 +516,28 @@ class Context:
 
     def block_time_ebe(self, reps: int = 10) -> float:
         """Average ms of the FP32 matrix-core element product over reps
-        launches (HIP events on the context's stream); after block_solve."""
+        launches (HIP events on the context's stream) on the block of the
+        last block_apply(..., "mixed") or block_solve."""
         ms = C.c_double()
         _check(lib().pf_block_time_ebe(self.h, int(reps), C.byref(ms)), "pf_block_time_ebe")
         return ms.value
+
+    def tune_set(self, ebe_epl: int = 0, blk_waves: int = 0, blk_rows: int = 0):
+        """Launch shapes of the element products (0 = built-in choice)."""
+        t = Tune(int(ebe_epl), int(blk_waves), int(blk_rows))
+        _check(lib().pf_tune_set(self.h, C.byref(t)), "pf_tune_set")
+
+    def tune_get(self) -> dict:
+        t = Tune()
+        _check(lib().pf_tune_get(self.h, C.byref(t)), "pf_tune_get")
+        return {"ebe_epl": t.ebe_epl, "blk_waves": t.blk_waves, "blk_rows": t.blk_rows}
+
+    def device_info(self):
+        """(architecture name, compute units) of the context's device."""
+        buf = C.create_string_buffer(256)
+        cus = C.c_int()
+        _check(lib().pf_device_info(self.h, buf, 256, C.byref(cus)), "pf_device_info")
+        return buf.value.decode(), cus.value
 
     def pcg_r(self) -> np.ndarray:
         """Recurrence residual at the end of the solve."""

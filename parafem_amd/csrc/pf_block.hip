@@ -509,14 +509,32 @@ int blk_setup(pf_ctx *c, bool need32)
 }
 
 template <typename T, int MP>
+void blk_launch_mp(pf_ctx *c, int waves, const T *Kt, const T *P, T *Ut, const pf_blk_state *st)
+{
+    const int grid = (c->nels + waves - 1) / waves;
+    const long ks = pf_k64_estride(c);
+    const dim3 g(grid), b(64 * waves);
+    switch (c->ntot) {
+    case 60: hipLaunchKernelGGL((k_blk_ebe<60, T, MP>), g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
+    case 24: hipLaunchKernelGGL((k_blk_ebe<24, T, MP>), g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
+    default: hipLaunchKernelGGL((k_blk_ebe<8, T, MP>), g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
+    }
+}
+
+// The element product in the context's launch shape: elements per workgroup
+// (tune_blk_waves, default 4) and, for FP32, 16-row output tiles per pass
+// (tune_blk_rows, default 4); the FP64 product makes passes of 2 tiles.
+template <typename T>
 int blk_launch_ebe(pf_ctx *c, const T *Kt, const T *P, T *Ut, const pf_blk_state *st)
 {
-    const int grid = (c->nels + 3) / 4;
-    const long ks = pf_k64_estride(c);
-    switch (c->ntot) {
-    case 60: hipLaunchKernelGGL((k_blk_ebe<60, T, MP>), dim3(grid), dim3(256), 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
-    case 24: hipLaunchKernelGGL((k_blk_ebe<24, T, MP>), dim3(grid), dim3(256), 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
-    default: hipLaunchKernelGGL((k_blk_ebe<8, T, MP>), dim3(grid), dim3(256), 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
+    const int waves = c->tune_blk_waves ? c->tune_blk_waves : 4;
+    if constexpr (sizeof(T) == 8) {
+        blk_launch_mp<T, 2>(c, waves, Kt, P, Ut, st);
+    } else {
+        const int rows = c->tune_blk_rows ? c->tune_blk_rows : 4;
+        if (rows == 1) blk_launch_mp<T, 1>(c, waves, Kt, P, Ut, st);
+        else if (rows == 2) blk_launch_mp<T, 2>(c, waves, Kt, P, Ut, st);
+        else blk_launch_mp<T, 4>(c, waves, Kt, P, Ut, st);
     }
     PF_HIP(hipGetLastError());
     return PF_OK;
@@ -528,7 +546,7 @@ int blk_true_residual(pf_ctx *c, double *rr_host)
 {
     pf_block *b = c->blk;
     const int G = blk_grid(c->neq);
-    int rc = blk_launch_ebe<double, 2>(c, b->Kt64, b->X64, b->Ut64, nullptr);
+    int rc = blk_launch_ebe<double>(c, b->Kt64, b->X64, b->Ut64, nullptr);
     if (rc) return rc;
     hipLaunchKernelGGL((k_blk_scatter<double, false>), dim3(G), dim3(256), 0, c->stream, c->csr_ptr, c->csr_slot,
                        b->Ut64, (const double *)nullptr, b->U64, c->neq, (double *)nullptr,
@@ -590,7 +608,7 @@ int pf_block_apply(pf_ctx *c, int k, const double *x, double *y, int precision)
         // fl32(x) -> FP32 products and scatter -> widened into U64
         hipLaunchKernelGGL((k_blk_convert<double, float>), dim3(8192), dim3(256), 0, c->stream, b->X64, b->R,
                            (long)(c->neq + 1) * PF_BLK);
-        rc = blk_launch_ebe<float, 4>(c, b->Kt32, b->R, b->Ut32, nullptr);
+        rc = blk_launch_ebe<float>(c, b->Kt32, b->R, b->Ut32, nullptr);
         if (rc) return rc;
         hipLaunchKernelGGL((k_blk_scatter<float, false>), dim3(G), dim3(256), 0, c->stream, c->csr_ptr, c->csr_slot,
                            b->Ut32, (const float *)nullptr, b->U, c->neq, (double *)nullptr,
@@ -598,7 +616,7 @@ int pf_block_apply(pf_ctx *c, int k, const double *x, double *y, int precision)
         hipLaunchKernelGGL((k_blk_convert<float, double>), dim3(8192), dim3(256), 0, c->stream, b->U, b->U64,
                            (long)(c->neq + 1) * PF_BLK);
     } else {
-        rc = blk_launch_ebe<double, 2>(c, b->Kt64, b->X64, b->Ut64, nullptr);
+        rc = blk_launch_ebe<double>(c, b->Kt64, b->X64, b->Ut64, nullptr);
         if (rc) return rc;
         hipLaunchKernelGGL((k_blk_scatter<double, false>), dim3(G), dim3(256), 0, c->stream, c->csr_ptr,
                            c->csr_slot, b->Ut64, (const double *)nullptr, b->U64, c->neq, (double *)nullptr,
@@ -614,19 +632,19 @@ int pf_block_apply(pf_ctx *c, int k, const double *x, double *y, int precision)
 int pf_block_time_ebe(pf_ctx *c, int reps, double *ms)
 {
     if (!c || !ms || reps < 1) return PF_ERR_ARG;
-    if (!c->blk || !c->blk->Kt32 || !c->blk->P) return PF_ERR_STATE;
+    if (!c->blk || !c->blk->Kt32 || !c->blk->R) return PF_ERR_STATE;
     PF_HIP(hipSetDevice(c->device));
     pf_block *b = c->blk;
     hipEvent_t t0, t1;
     PF_HIP(hipEventCreate(&t0));
     PF_HIP(hipEventCreate(&t1));
-    int rc = blk_launch_ebe<float, 4>(c, b->Kt32, b->P, b->Ut32, nullptr);   // warm
+    int rc = blk_launch_ebe<float>(c, b->Kt32, b->R, b->Ut32, nullptr);   // warm
     hipError_t e = hipSuccess;
     double total = 0.0;
     // one event pair around each launch: the kernel's own duration
     for (int i = 0; !rc && e == hipSuccess && i < reps; ++i) {
         e = hipEventRecord(t0, c->stream);
-        if (e == hipSuccess) rc = blk_launch_ebe<float, 4>(c, b->Kt32, b->P, b->Ut32, nullptr);
+        if (e == hipSuccess) rc = blk_launch_ebe<float>(c, b->Kt32, b->R, b->Ut32, nullptr);
         if (e == hipSuccess && !rc) e = hipEventRecord(t1, c->stream);
         if (e == hipSuccess && !rc) e = hipEventSynchronize(t1);
         float t = 0.0f;
@@ -709,7 +727,7 @@ int pf_block_solve(pf_ctx *c, int k, const double *f, double *x, const pf_block_
         // FP32 block PCG until some column asks for a residual replacement.
         for (;;) {
             for (int it = 0; it < check; ++it) {
-                rc = blk_launch_ebe<float, 4>(c, b->Kt32, b->P, b->Ut32, b->st);
+                rc = blk_launch_ebe<float>(c, b->Kt32, b->P, b->Ut32, b->st);
                 if (rc) return rc;
                 hipLaunchKernelGGL((k_blk_scatter<float, true>), dim3(G), dim3(256), 0, c->stream, c->csr_ptr,
                                    c->csr_slot, b->Ut32, b->P, b->U, c->neq, b->partials, b->st);

@@ -325,6 +325,39 @@ int pf_mem_used(pf_ctx *c, size_t *bytes)
     return PF_OK;
 }
 
+int pf_tune_set(pf_ctx *c, const pf_tune *t)
+{
+    if (!c || !t) return PF_ERR_ARG;
+    const int e = t->ebe_epl, w = t->blk_waves, r = t->blk_rows;
+    if (e < 0 || e > 8 || (e & (e - 1))) return PF_ERR_ARG;
+    if (c->ntot && e > pf_epl_max(c->ntot)) return PF_ERR_ARG;
+    if (w < 0 || w > 4 || (w & (w - 1)) || r < 0 || r > 4 || (r & (r - 1))) return PF_ERR_ARG;
+    pf_graph_drop(c);
+    c->tune_epl = e;
+    c->tune_blk_waves = w;
+    c->tune_blk_rows = r;
+    return PF_OK;
+}
+
+int pf_tune_get(pf_ctx *c, pf_tune *t)
+{
+    if (!c || !t) return PF_ERR_ARG;
+    t->ebe_epl = c->tune_epl;
+    t->blk_waves = c->tune_blk_waves;
+    t->blk_rows = c->tune_blk_rows;
+    return PF_OK;
+}
+
+int pf_device_info(pf_ctx *c, char *arch, int len, int *cus)
+{
+    if (!c || !arch || len < 1) return PF_ERR_ARG;
+    hipDeviceProp_t pr;
+    PF_HIP(hipGetDeviceProperties(&pr, c->device));
+    snprintf(arch, (size_t)len, "%s", pr.gcnArchName);
+    if (cus) *cus = pr.multiProcessorCount;
+    return PF_OK;
+}
+
 int pf_trim(pf_ctx *c)
 {
     if (!c) return PF_ERR_ARG;

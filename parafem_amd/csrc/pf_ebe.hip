@@ -171,7 +171,7 @@ __device__ __forceinline__ double bpermute64(int addr, double v)
 // forms sum_d K(i,i+d) p(i+d) from p staged twice in LDS, then receives
 // K(i-d,i) p(i) from lane i-d (mod NT) for d = 1..NT/2-1 by ds_bpermute,
 // where lane j sends K(j,j+d) p(j): every entry is read from HBM once.
-template <int NT, int KIND, bool GATHER, typename OUT, bool LOOP = false>
+template <int NT, int KIND, bool GATHER, typename OUT, bool LOOP = false, int EPLT = Lanes<NT, KIND>::EPL>
 __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
                                                  const float *__restrict__ Klo, const int *__restrict__ gg,
                                                  const double *__restrict__ src, OUT *__restrict__ out, int nels,
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
 {
     if (skip(st, gate)) return;
     using L = Lanes<NT, KIND>;
-    constexpr int EPL = L::EPL;
+    constexpr int EPL = EPLT;
     constexpr int D = pf_sym_diags(NT), Q = D / 4, R = D - 4 * Q;
     // Per wave and element slot: p_e (FP64), or p_e twice (packed kinds).
     __shared__ __attribute__((aligned(16))) double ps[PF_BLOCK / 64][EPL][KIND == KIND_F32 ? 64 : 128];
@@ -700,18 +700,38 @@ __global__ void k_scatter_fix(const int *__restrict__ ptr, const int *__restrict
     }
 }
 
+template <int NT, int KIND, bool GATHER, typename OUT, int EPL>
+void launch_ebe_epl(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate)
+{
+    constexpr int EPB = Lanes<NT, KIND>::EPW * EPL * (PF_BLOCK / 64);
+    const long nlb = (c->nels + EPB - 1) / EPB;
+    hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, false, EPL>), dim3((int)nlb), dim3(PF_BLOCK), 0, c->stream,
+                       c->K64, c->K32hi, c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
+}
+
 template <int NT, int KIND, bool GATHER, typename OUT>
 int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate)
 {
-    const long nlb = (c->nels + Lanes<NT, KIND>::EPB - 1) / Lanes<NT, KIND>::EPB;
     if (gate == RUN_REFINE) {
         // usually does nothing: dispatch few workgroups that loop
+        const long nlb = (c->nels + Lanes<NT, KIND>::EPB - 1) / Lanes<NT, KIND>::EPB;
         const int grid = (int)std::min<long>(nlb, 2048);
         hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, true>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64,
                            c->K32hi, c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
     } else {
-        hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, false>), dim3((int)nlb), dim3(PF_BLOCK), 0, c->stream,
-                           c->K64, c->K32hi, c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
+        // the tuned shape of the iteration's product (an out-of-range value,
+        // e.g. from before the mesh changed, keeps the built-in one)
+        constexpr int E0 = Lanes<NT, KIND>::EPL;
+        const int e = GATHER && c->tune_epl <= pf_epl_max(NT) ? c->tune_epl : 0;
+        if constexpr (GATHER) {
+            if (e == 1) launch_ebe_epl<NT, KIND, GATHER, OUT, 1>(c, src, out, st, gate);
+            else if (e == 2) launch_ebe_epl<NT, KIND, GATHER, OUT, 2>(c, src, out, st, gate);
+            else if (e == 4) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 24 ? 4 : E0)>(c, src, out, st, gate);
+            else if (e == 8) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 8 ? 8 : E0)>(c, src, out, st, gate);
+            else launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate);
+        } else {
+            launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate);
+        }
     }
     PF_HIP(hipGetLastError());
     return PF_OK;

@@ -165,6 +165,10 @@ struct pf_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_iters = 0, graph_precision = -1, graph_stop = -1;
+    // Launch shapes (pf_tune_set); 0 = the built-in choice.
+    int tune_epl = 0;                  // single-RHS products: elements per lane group
+    int tune_blk_waves = 0;            // block products: elements per workgroup
+    int tune_blk_rows = 0;             // block FP32 products: 16-row tiles per pass
 
     // Optional per-kernel timing of pf_pcg_run: one event pair per launch.
     bool timing = false;
@@ -172,6 +176,9 @@ struct pf_ctx {
     std::vector<std::pair<int, int>> ev_used[PF_NKERNELS];   // (start, stop) indices
     size_t ev_next = 0;
 };
+
+// Largest elements-per-lane-group of the single-RHS element product.
+inline int pf_epl_max(int ntot) { return ntot > 32 ? 2 : ntot > 8 ? 4 : 8; }
 
 // Drop a captured iteration chunk (problem, options or buffers changed).
 void pf_graph_drop(pf_ctx *c);

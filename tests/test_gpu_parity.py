@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from parafem_amd import DeviceArray, mesh as M, problems as P
+from parafem_amd import DeviceArray, PFError, mesh as M, problems as P
 
 pytestmark = pytest.mark.gpu
 
@@ -454,6 +454,100 @@ def test_block_apply_matrix_cores(ctx, nt_case, k):
     assert np.abs(Y - ref).max() <= 1e-13 * scale
     Y32 = ctx.block_apply(X, "mixed")
     assert np.abs(Y32 - ref).max() <= 2e-6 * scale
+
+
+def _shape_problem(case):
+    if case == "hex20":
+        pr = _distorted(3, seed=11)
+        return pr, O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem)
+    pr = P.laplace_cube(6, lognormal_k=True)
+    return pr, O.assemble_laplace(pr.g_coord(), 1.0, 1.0, 1.0, pr.k_elem)
+
+
+@pytest.mark.parametrize("case", ["hex20", "hex8_laplace"])
+def test_launch_shapes_same_bits(ctx, case):
+    """Every single-RHS launch shape the autotuner can pick (elements per
+    lane group): the FP64 PCG equals the oracle bit for bit under each, and
+    each FP32-stored path (mixed, mixed64, split) gives the same iterations
+    and the same bits under every shape."""
+    from parafem_amd import tune as T
+    pr, Ko = _shape_problem(case)
+    cands = T.EPL_CANDIDATES[pr.ntot]
+    say(f"launch shapes {case} neq={pr.neq} epl={cands}")
+    try:
+        ctx.mesh_set(pr.g_coord())
+        ctx.steer_set(pr.g_g, pr.neq)
+        ctx.storkm_set(Ko)
+        r = P.start_r(ctx, pr)
+        dg = O.diag(pr.g_g, Ko, pr.neq)
+        kw = {}
+        if pr.fix_eq is not None and pr.fix_eq.size:
+            store = dg[pr.fix_eq - 1] + P.PENALTY
+            dg[pr.fix_eq - 1] = store
+            kw = {"no_f": pr.fix_eq, "store": store}
+        xo, it, conv = O.pcg(pr.g_g, Ko, r, dg, 1e-10, 3000, stop=O.STOP_RELRES, dot_order=O.DOT_DEVICE, **kw)
+        for e in cands:
+            ctx.tune_set(ebe_epl=e)
+            x, res = ctx.pcg_solve(r, 3000, 1e-10, stop="relres")
+            assert res.iters == it and res.converged == conv, (e, res.iters, it)
+            assert np.array_equal(x, xo), e
+        ctx.storkm_set(Ko, store="fp32")
+        if pr.fix_eq is not None and pr.fix_eq.size:
+            r = P.start_r(ctx, pr)
+        for prec in ("mixed", "mixed64", "split"):
+            first = None
+            for e in cands:
+                ctx.tune_set(ebe_epl=e)
+                x, res = ctx.pcg_solve(r, 5000, 1e-10, stop="relres", precision=prec)
+                assert res.converged, (prec, e)
+                if first is None:
+                    first = (x, res.iters)
+                else:
+                    assert res.iters == first[1] and np.array_equal(x, first[0]), (prec, e)
+    finally:
+        ctx.tune_set()
+
+
+def test_block_shapes_and_autotuner(ctx, tmp_path, monkeypatch):
+    """Every block launch shape (elements per workgroup x row tiles per
+    pass) gives the same bits for the FP32 and FP64 block products; the
+    autotuner picks one of them, caches it, and a cached lookup sets the
+    same shape without timing."""
+    from parafem_amd import tune as T
+    monkeypatch.setenv("PARAFEM_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(T, "_memory", {})
+    pr = _distorted(3, seed=12)
+    Ko = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem)
+    say(f"block shapes neq={pr.neq}")
+    try:
+        ctx.mesh_set(pr.g_coord())
+        ctx.steer_set(pr.g_g, pr.neq)
+        ctx.storkm_set(Ko)
+        X = np.random.default_rng(5).normal(size=(pr.neq, 64))
+        ref32 = ref64 = None
+        for w, rows in T.BLOCK_CANDIDATES:
+            ctx.tune_set(blk_waves=w, blk_rows=rows)
+            Y32, Y64 = ctx.block_apply(X, "mixed"), ctx.block_apply(X, "fp64")
+            if ref32 is None:
+                ref32, ref64 = Y32, Y64
+            assert np.array_equal(Y32, ref32) and np.array_equal(Y64, ref64), (w, rows)
+        ctx.tune_set()
+        got = T.tune_block(ctx, reps=3)
+        assert not got["cached"] and (got["blk_waves"], got["blk_rows"]) in T.BLOCK_CANDIDATES
+        assert ctx.tune_get()["blk_waves"] == got["blk_waves"]
+        monkeypatch.setattr(T, "_memory", {})
+        ctx.tune_set()
+        again = T.tune_block(ctx)
+        assert again["cached"] and ctx.tune_get()["blk_rows"] == got["blk_rows"]
+        r = P.start_r(ctx, pr)
+        e = T.tune_ebe(ctx, r, "fp64", reps=3, warm=1)
+        assert e["ebe_epl"] in T.EPL_CANDIDATES[60] and ctx.tune_get()["ebe_epl"] == e["ebe_epl"]
+        with pytest.raises(PFError):
+            ctx.tune_set(ebe_epl=4)            # beyond the hex20 product's range
+        with pytest.raises(PFError):
+            ctx.tune_set(blk_waves=3)
+    finally:
+        ctx.tune_set()
 
 
 def test_block_solve_64_load_cases(ctx):
