@@ -245,6 +245,20 @@ int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);
  * e_elem: host nels or NULL; sigma: host (6, nels) column-major.  Needs the
  * mesh and an elastic steering (ntot = 3 nod). */
 int pf_stress_centroid(pf_ctx *ctx, const double *x, double e, double v, const double *e_elem, double *sigma);
+/* SCATTER_NODES(npes,nn,nels_pp,g_num_pp,nod,numvar,nodes_pp,node_start,
+ * node_end,element_contribution,nodal_value,flag)
+ * (src/modules/mpi/gather_scatter.f90:1786-1929) at one rank (npes = 1,
+ * nodes_pp = nn, node_start = 1, node_end = nn): nodal_value((n-1)*numvar
+ * + k) is the sum, in ascending element and then element-node order, of
+ * element_contribution((j-1)*numvar + k, iel) over every element node j of
+ * element iel that is node n; flag 1 divides each sum by its number of
+ * terms (the nodal average; 0/0 for a node no element uses, as there).
+ * Node numbers outside 1..nn are skipped.  p121 passes the gathered
+ * displacements with numvar = ndim (p121.f90:132-133).  Host arrays:
+ * g_num_pp (nod x nels), element_contribution (numvar*nod x nels),
+ * nodal_value (numvar*nn). */
+int pf_scatter_nodes(pf_ctx *ctx, int nn, int nod, int nels, const int *g_num_pp, int numvar,
+                     const double *element_contribution, double *nodal_value, int flag);
 
 /* ------------------------------------------------- multi-RHS block solve */
 /* No counterpart in the reference: up to 64 load cases on one mesh solved

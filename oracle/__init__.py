@@ -21,6 +21,10 @@ Pinning status (how each function is checked in ``tests/test_oracle.py``):
   elem_matvec, diag, apply -- numpy and the scipy-assembled global matrix;
   checon -- exact binary cases;
   dot_device, sum_p -- math.fsum within rounding; total load of the fixtures;
+  centroid_stress -- element 1 of the p121 demo against its results file;
+  scatter_nodes -- element counts per node by brute force, exact averages
+      of equal contributions where the count is a power of two, the sums
+      conserved, and the p121 demo's printed displacement field;
   pcg -- the reference's committed fixtures (xx3-tiny: 1640 equations, 79
       iterations, displacements; p121 demo: 98360 equations, total load
       -100, 295 iterations, displacement field) and scipy direct solves.
@@ -58,6 +62,7 @@ def lib():
         i = C.POINTER(C.c_int)
         L.or_centroid_stress.argtypes = [C.c_int, C.c_int, d, i, d, C.c_double, C.c_double, d, d]
         L.or_sample_hex8.argtypes = [d, d]
+        L.or_scatter_nodes.argtypes = [C.c_int, C.c_int, C.c_int, i, C.c_int, d, d, C.c_int]
         L.or_deemat6.argtypes = [d, C.c_double, C.c_double]
         L.or_shape_der.argtypes = [d, d, C.c_int, C.c_int, C.c_int]
         L.or_shape_der.restype = C.c_int
@@ -170,6 +175,18 @@ def centroid_stress(g_coord, g_g, x, e, v, e_elem=None):
     lib().or_centroid_stress(nels, ntot // 3, _d(g), _i(gg), _d(xx), float(e), float(v),
                              _d(ee) if ee is not None else None, _d(out))
     return out.reshape(nels, 6)
+
+
+def scatter_nodes(g_num, contrib, nn, flag=1):
+    """SCATTER_NODES at one rank (gather_scatter.f90:1786-1929): g_num
+    (nels, nod) 1-based, contrib (nels, nod, numvar) -> (nn, numvar)."""
+    g = _i32(g_num)
+    nels, nod = g.shape
+    c = _f64(contrib)
+    numvar = c.shape[2]
+    out = np.zeros(nn * numvar)
+    lib().or_scatter_nodes(int(nn), nod, nels, _i(g), numvar, _d(c), _d(out), int(flag))
+    return out.reshape(nn, numvar)
 
 
 def laplace_kc(coord, kx=1.0, ky=1.0, kz=1.0):

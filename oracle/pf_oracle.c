@@ -260,6 +260,32 @@ void or_centroid_stress(int nels, int nod, const double *g_coord, const int *g_g
     free(bee);
 }
 
+/* SCATTER_NODES (gather_scatter.f90:1786-1929) at one rank: nodes_pp = nn,
+ * node_start = 1, node_end = nn.  temp starts at zero; the loops run
+ * iel = 1..nels, j = 1..nod, k = 1..numvar adding
+ * element_contribution((j-1)*numvar+k, iel) into temp((g-1)*numvar+k) and
+ * counting the terms; flag 1 then divides by the counts.  g_num (nod,nels)
+ * column-major, 1-based; contrib (numvar*nod, nels); nodal (numvar*nn). */
+void or_scatter_nodes(int nn, int nod, int nels, const int *g_num, int numvar, const double *contrib,
+                      double *nodal, int flag)
+{
+    int *cnt = (int *)calloc((size_t)nn * numvar, sizeof(int));
+    for (size_t i = 0; i < (size_t)nn * numvar; ++i) nodal[i] = 0.0;
+    for (int iel = 0; iel < nels; ++iel)
+        for (int j = 0; j < nod; ++j) {
+            int g = g_num[(size_t)iel * nod + j];
+            if (g < 1 || g > nn) continue;
+            for (int k = 0; k < numvar; ++k) {
+                size_t o = (size_t)(g - 1) * numvar + k;
+                nodal[o] = nodal[o] + contrib[((size_t)iel * nod + j) * numvar + k];
+                cnt[o] += 1;
+            }
+        }
+    if (flag == 1)
+        for (size_t i = 0; i < (size_t)nn * numvar; ++i) nodal[i] = nodal[i] / (double)cnt[i];
+    free(cnt);
+}
+
 /* One hex8 Laplace element of p123.f90:70-84:
  * kc{x,y,z} += MATMUL(col,row)*det*w; storkc = kcx*kx + kcy*ky + kcz*kz. */
 void or_laplace_kc(const double *coord, int nod, double kx, double ky, double kz, double *kc)

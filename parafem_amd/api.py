@@ -128,6 +128,7 @@ def lib():
         "pf_block_apply": (i32, [vp, i32, pd, pd, i32]),
         "pf_block_time_ebe": (i32, [vp, i32, pd]),
         "pf_stress_centroid": (i32, [vp, pd, dbl, dbl, pd, pd]),
+        "pf_scatter_nodes": (i32, [vp, i32, i32, i32, pi, i32, pd, pd, i32]),
         "pf_trim": (i32, [vp]),
         "pf_tune_set": (i32, [vp, C.POINTER(Tune)]),
         "pf_tune_get": (i32, [vp, C.POINTER(Tune)]),
@@ -483,6 +484,20 @@ class Context:
         """Free the current problem's device buffers; keep the context."""
         _check(lib().pf_reset(self.h), "pf_reset")
         self.nels = self.ntot = self.neq = self.nod = 0
+
+    def scatter_nodes(self, g_num, contrib, nn: int, flag: int = 1) -> np.ndarray:
+        """SCATTER_NODES at one rank (gather_scatter.f90:1786-1929): g_num
+        (nels, nod) 1-based node numbers, contrib (nels, nod, numvar) ->
+        (nn, numvar) sums, or averages with flag 1."""
+        g = np.ascontiguousarray(g_num, dtype=np.int32)
+        c = np.ascontiguousarray(contrib, dtype=np.float64)
+        nels, nod = g.shape
+        assert c.shape[:2] == (nels, nod)
+        numvar = c.shape[2]
+        out = np.empty(int(nn) * numvar)
+        _check(lib().pf_scatter_nodes(self.h, int(nn), nod, nels, _pi(g), numvar, _pd(c), _pd(out), int(flag)),
+               "pf_scatter_nodes")
+        return out.reshape(int(nn), numvar)
 
     def stress_centroid(self, x, e: float, v: float, e_elem=None) -> np.ndarray:
         """Centroid stresses (nels, 6) of the displacement x (p121.f90:112-123)."""

@@ -11,6 +11,8 @@
 // zeros), no contraction, so the result equals the sequential evaluation.
 #include "pf_internal.h"
 
+#include <vector>
+
 namespace {
 
 // SHAPE_DER at (0,0,0): derivative a of shape function l.
@@ -129,7 +131,76 @@ __global__ __launch_bounds__(128) void k_stress(const double *__restrict__ gc, c
     }
 }
 
+// SCATTER_NODES at one rank: one thread per node adds the contributions of
+// the node's (element, element-node) slots in ascending order, then divides
+// by their number when averaging.
+__global__ __launch_bounds__(256) void k_scatter_nodes(const int *__restrict__ ptr, const long *__restrict__ slot,
+                                                      const double *__restrict__ contrib, double *__restrict__ out,
+                                                      int nn, int numvar, int flag)
+{
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nn) return;
+    const int a = ptr[node], b = ptr[node + 1];
+    for (int k = 0; k < numvar; ++k) {
+        double s = 0.0;
+        for (int q = a; q < b; ++q) s = __dadd_rn(s, contrib[slot[q] * numvar + k]);
+        if (flag == 1) s = s / (double)(b - a);
+        out[(long)node * numvar + k] = s;
+    }
+}
+
 }  // namespace
+
+extern "C" int pf_scatter_nodes(pf_ctx *c, int nn, int nod, int nels, const int *g_num, int numvar,
+                                const double *contrib, double *nodal, int flag)
+{
+    if (!c || nn < 1 || nod < 1 || nels < 0 || numvar < 1 || !nodal || (nels && (!g_num || !contrib)))
+        return PF_ERR_ARG;
+    PF_HIP(hipSetDevice(c->device));
+    // node -> (element, element node) slots, ascending; nodes outside 1..nn
+    // are not this rank's and are skipped
+    const long ns = (long)nels * nod;
+    std::vector<int> ptr((size_t)nn + 1, 0);
+    for (long s = 0; s < ns; ++s)
+        if (g_num[s] >= 1 && g_num[s] <= nn) ++ptr[g_num[s]];
+    for (int i = 0; i < nn; ++i) ptr[i + 1] += ptr[i];
+    std::vector<long> slot((size_t)std::max(ptr[nn], 1));
+    {
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (long s = 0; s < ns; ++s)
+            if (g_num[s] >= 1 && g_num[s] <= nn) slot[fill[g_num[s] - 1]++] = s;
+    }
+    const size_t nslot = slot.size(), nc = (size_t)std::max<long>(ns, 1) * numvar, no = (size_t)nn * numvar;
+    int *dptr = nullptr;
+    long *dslot = nullptr;
+    double *dc = nullptr, *dout = nullptr;
+    int rc = pf_alloc(c, (void **)&dptr, ((size_t)nn + 1) * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&dslot, nslot * sizeof(long));
+    if (!rc) rc = pf_alloc(c, (void **)&dc, nc * sizeof(double));
+    if (!rc) rc = pf_alloc(c, (void **)&dout, no * sizeof(double));
+    if (!rc) {
+        hipError_t he = hipMemcpyAsync(dptr, ptr.data(), ((size_t)nn + 1) * sizeof(int), hipMemcpyHostToDevice,
+                                       c->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(dslot, slot.data(), nslot * sizeof(long), hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess && ns)
+            he = hipMemcpyAsync(dc, contrib, (size_t)ns * numvar * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess) {
+            hipLaunchKernelGGL(k_scatter_nodes, dim3((nn + 255) / 256), dim3(256), 0, c->stream, dptr, dslot, dc,
+                               dout, nn, numvar, flag);
+            he = hipGetLastError();
+        }
+        if (he == hipSuccess) he = hipMemcpyAsync(nodal, dout, no * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+        if (he != hipSuccess) rc = pf_set_hip_error(he, "scatter_nodes", __FILE__, __LINE__);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    pf_free(c, dptr, ((size_t)nn + 1) * sizeof(int));
+    pf_free(c, dslot, nslot * sizeof(long));
+    pf_free(c, dc, nc * sizeof(double));
+    pf_free(c, dout, no * sizeof(double));
+    return rc;
+}
 
 extern "C" int pf_stress_centroid(pf_ctx *c, const double *x, double e, double v, const double *e_elem,
                                   double *sigma)

@@ -57,6 +57,7 @@ MODULE pf_gpu
   PUBLIC :: gather, scatter, scatter_noadd, elem_matvec
   PUBLIC :: dot_product_p, sum_p, checon_par
   PUBLIC :: vec_fill, vec_waxpy, vec_mul, vec_recip
+  PUBLIC :: pf_stress, calc_nodes_pp, scatter_nodes, dismsh_ensi_p
 
   INTERFACE
     INTEGER(c_int) FUNCTION c_create(device, c) BIND(C, name="pf_create")
@@ -223,6 +224,23 @@ MODULE pf_gpu
       REAL(c_double), INTENT(OUT) :: x(*)
       TYPE(pf_pcg_opts), INTENT(IN) :: opts
       TYPE(pf_pcg_result), INTENT(OUT) :: res
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_stress_centroid(c, x, e, v, e_elem, sigma) BIND(C, name="pf_stress_centroid")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), INTENT(IN) :: x(*)
+      REAL(c_double), VALUE :: e, v
+      TYPE(c_ptr), VALUE :: e_elem
+      REAL(c_double), INTENT(OUT) :: sigma(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_scatter_nodes(c, nn, nod, nels, g_num, numvar, contrib, nodal, flag) &
+        BIND(C, name="pf_scatter_nodes")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      INTEGER(c_int), VALUE :: nn, nod, nels, numvar, flag
+      INTEGER(c_int), INTENT(IN) :: g_num(*)
+      REAL(c_double), INTENT(IN) :: contrib(*)
+      REAL(c_double), INTENT(OUT) :: nodal(*)
     END FUNCTION
   END INTERFACE
 
@@ -488,5 +506,63 @@ CONTAINS
     iters = res%iters
     converged = res%converged /= 0
   END SUBROUTINE pf_pcg
+
+  !> Stresses at every element's centroid (p121.f90:112-123 for each element):
+  !> sigma(6,nels) from the displacements x(neq) and DEEMAT(e, v).
+  SUBROUTINE pf_stress(x, e, v, sigma)
+    REAL(iwp), INTENT(IN) :: x(:), e, v
+    REAL(iwp), INTENT(OUT) :: sigma(:, :)
+    CALL pf_check(c_stress_centroid(ctx, x, e, v, c_null_ptr, sigma), 'pf_stress_centroid')
+  END SUBROUTINE pf_stress
+
+  !> CALC_NODES_PP (src/modules/mpi/gather_scatter.f90) at one rank: every
+  !> node is this rank's.
+  SUBROUTINE calc_nodes_pp(nn, npes, numpe, node_end, node_start, nodes_pp)
+    INTEGER, INTENT(IN) :: nn, npes, numpe
+    INTEGER, INTENT(OUT) :: node_end, node_start, nodes_pp
+    IF (npes /= 1 .OR. numpe /= 1) THEN
+      WRITE(*, '(A)') 'pf_gpu: calc_nodes_pp runs on one process'
+      STOP 1
+    END IF
+    node_start = 1
+    node_end = nn
+    nodes_pp = nn
+  END SUBROUTINE calc_nodes_pp
+
+  !> SCATTER_NODES (src/modules/mpi/gather_scatter.f90:1786-1929) with the
+  !> reference's arguments, at one rank: nodal sums, in ascending element
+  !> order, of element_contribution(numvar*nod, nels_pp); flag 1 averages.
+  SUBROUTINE scatter_nodes(npes, nn, nels_pp, g_num_pp, nod, numvar, nodes_pp, node_start, node_end, &
+                           element_contribution, nodal_value, flag)
+    INTEGER, INTENT(IN) :: npes, nn, nels_pp, nod, numvar, nodes_pp, node_start, node_end, flag
+    INTEGER, INTENT(IN) :: g_num_pp(:, :)
+    REAL(iwp), INTENT(IN) :: element_contribution(:, :)
+    REAL(iwp), INTENT(OUT) :: nodal_value(:)
+    IF (npes /= 1 .OR. node_start /= 1 .OR. node_end /= nn .OR. nodes_pp /= nn) THEN
+      WRITE(*, '(A)') 'pf_gpu: scatter_nodes runs on one process (all nodes local)'
+      STOP 1
+    END IF
+    CALL pf_check(c_scatter_nodes(ctx, INT(nn, c_int), INT(nod, c_int), INT(nels_pp, c_int), g_num_pp, &
+                                  INT(numvar, c_int), element_contribution, nodal_value, INT(flag, c_int)), &
+                  'pf_scatter_nodes')
+  END SUBROUTINE scatter_nodes
+
+  !> DISMSH_ENSI_P (src/modules/mpi/output.f90:2983-3111) at one rank: one
+  !> value per line, edit descriptor e12.5.
+  SUBROUTINE dismsh_ensi_p(filnum, iload, nodes_pp, npes, numpe, numvar, stress)
+    INTEGER, INTENT(IN) :: filnum, iload, nodes_pp, npes, numpe, numvar
+    REAL(iwp), INTENT(IN) :: stress(:)
+    INTEGER :: i
+    IF (npes /= 1) THEN
+      WRITE(*, '(A)') 'pf_gpu: dismsh_ensi_p runs on one process'
+      STOP 1
+    END IF
+    IF (numpe == 1) THEN
+      DO i = 1, nodes_pp
+        WRITE(filnum, '(e12.5)') stress(i)
+      END DO
+    END IF
+    IF (iload < 0 .OR. numvar < 0) RETURN   ! unused at one rank, as in the reference
+  END SUBROUTINE dismsh_ensi_p
 
 END MODULE pf_gpu

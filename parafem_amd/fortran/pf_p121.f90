@@ -8,8 +8,11 @@
 ! Reads <deck>.dat/.d/.bnd/.lds in the formats READ_P121, READ_G_NUM_PP,
 ! READ_G_COORD_PP, READ_REST and READ_LOADS read (src/modules/mpi/
 ! input.f90:3234-3396, 935-1080, 288-445, 2570-2632, 2350-2411) and writes
-! <deck>.pf.res (the lines p121.f90:72-74,83,106-109 write) and <deck>.pf.x
-! (xnew_pp as neq raw REAL(8) values, stream access).
+! <deck>.pf.res (the lines p121.f90:72-74,83,106-109,114-121,138-139
+! write), <deck>.pf.x (xnew_pp as neq raw REAL(8) values, stream access) and
+! <deck>.pf.ensi.DISPL-000001 (the nodal displacements as p121.f90:125-137
+! writes them: SCATTER_NODES averages of the gathered element displacements,
+! DISMSH_ENSI_P per component).
 !
 !   dropin : p121's own loop, statement for statement (p121.f90:86-104), with
 !            GATHER / element products / SCATTER / DOT_PRODUCT_P / checon_par
@@ -32,11 +35,14 @@ PROGRAM pf_p121
   CHARACTER(LEN=15) :: element
   INTEGER, ALLOCATABLE :: g_num(:, :), rest(:, :), nf(:, :), g_g_pp(:, :), node(:)
   REAL(iwp), ALLOCATABLE :: coord(:, :), g_coord_pp(:, :, :), val(:, :), r(:), diag(:), x(:)
-  REAL(iwp), ALLOCATABLE :: storkm(:, :, :), timest(:)
+  REAL(iwp), ALLOCATABLE :: storkm(:, :, :), timest(:), sigma(:, :), eld(:, :), disp(:), temp(:)
+  INTEGER :: node_start, node_end, nodes_pp
+  INTEGER(8) :: ta, tst
   INTEGER, ALLOCATABLE :: row(:)
   TYPE(pf_dvec) :: p_pp, r_pp, x_pp, xnew_pp, u_pp, d_pp, diag_precon_pp, pmul_pp, utemp_pp
   INTEGER(8) :: t0, t1, rate
 
+  CALL SYSTEM_CLOCK(tst, rate)
   CALL GET_COMMAND_ARGUMENT(1, argv, nlen)
   IF (nlen == 0) THEN
     WRITE(*, '(A)') 'usage: pf_p121 <deck> [dropin|fused|mixed|km] [relres-tol]'
@@ -222,12 +228,47 @@ PROGRAM pf_p121
   WRITE(11, '(A,I6)') "The number of iterations to convergence was ", iters
   WRITE(11, '(A,F10.4)') "Time to solve equations was  :", REAL(t1 - t0, iwp) / REAL(rate, iwp)
   WRITE(11, '(A,E12.4)') "The central nodal displacement is :", x(1)
-  CLOSE(11)
 
   OPEN(12, FILE=argv(1:nlen)//'.pf.x', STATUS='REPLACE', ACTION='WRITE', ACCESS='STREAM', &
        FORM='UNFORMATTED', IOSTAT=ios)
   WRITE(12) x
   CLOSE(12)
+
+  !------------------------------- stresses at the centroid (p121.f90:112-123)
+  ALLOCATE(sigma(6, nels))
+  CALL pf_stress(x, e, v, sigma)
+  WRITE(11, '(A)') "The Centroid point stresses for element 1 are"
+  WRITE(11, '(A,I5)') "Point ", 1
+  WRITE(11, '(6E12.4)') sigma(:, 1)
+  DEALLOCATE(sigma)
+
+  !------------------------------------ displacements (p121.f90:125-137)
+  ! eld_pp = GATHER(xnew_pp) with xnew(0) = 0, averaged over the elements of
+  ! every node and written one component after the other.
+  ALLOCATE(eld(ntot, nels))
+  DO iel = 1, nels
+    DO i = 1, ntot
+      eld(i, iel) = 0.0_iwp
+      IF (g_g_pp(i, iel) /= 0) eld(i, iel) = x(g_g_pp(i, iel))
+    END DO
+  END DO
+  CALL calc_nodes_pp(nn, 1, 1, node_end, node_start, nodes_pp)
+  OPEN(12, FILE=argv(1:nlen)//'.pf.ensi.DISPL-000001', STATUS='REPLACE', ACTION='WRITE')
+  WRITE(12, '(A)') "Alya Ensight Gold --- Vector per-node variable file"
+  WRITE(12, '(A/A/A)') "part", "     1", "coordinates"
+  ALLOCATE(disp(nodes_pp * ndim), temp(nodes_pp))
+  CALL scatter_nodes(1, nn, nels, g_num, nod, ndim, nodes_pp, node_start, node_end, eld, disp, 1)
+  DO i = 1, ndim
+    DO j = 1, nodes_pp
+      temp(j) = disp(i + ndim * (j - 1))
+    END DO
+    CALL dismsh_ensi_p(12, 1, nodes_pp, 1, 1, 1, temp)
+  END DO
+  CLOSE(12)
+  DEALLOCATE(eld, disp, temp)
+  CALL SYSTEM_CLOCK(ta)
+  WRITE(11, '(A,F10.4)') "This analysis took  :", REAL(ta - tst, iwp) / REAL(rate, iwp)
+  CLOSE(11)
   WRITE(*, '(A,A,A,I0,A,L1)') 'pf_p121 ', TRIM(mode), ': iterations ', iters, ' converged ', converged
 
   IF (mode == 'dropin') THEN

@@ -27,7 +27,8 @@ __all__ = [
     "Mesh", "box_hex20", "box_hex8", "cube_bc20", "load_p121", "abaqus_to_sg",
     "sg_to_abaqus", "base_fixed_rest", "top_pressure_loads", "face_flux_loads",
     "steering", "nodal_loads_to_rhs", "read_deck", "write_p121_deck",
-    "splitmix64_uniform", "lognormal", "element_coords",
+    "splitmix64_uniform", "lognormal", "element_coords", "fortran_e", "write_ensi_vector",
+    "read_ensi_vector",
 ]
 
 
@@ -543,3 +544,48 @@ def write_p121_deck(basename: str, mesh: Mesh, rest, load_node, load_val,
         fh.write("1\n")
         fh.write(f"{mesh.nels:12d}{mesh.nn:12d}{len(rest):12d}{nip:5d}{mesh.nod:5d}{len(load_node):9d}\n")
         fh.write(f"{e:12.4E}{v:12.4E}{tol:12.4E}{limit:8d}\n")
+
+
+def fortran_e(value: float, w: int = 12, d: int = 5) -> str:
+    """``value`` as Fortran's Ew.d edit descriptor writes it (no scale
+    factor): optional minus, ``0.``, d digits rounded to nearest, then
+    ``E+xx`` (``+xxx`` for exponents of three digits), right-justified in w
+    characters (w asterisks when it does not fit)."""
+    v = float(value)
+    if v != v or v in (float("inf"), float("-inf")):
+        body = "NaN" if v != v else ("Infinity" if v > 0 else "-Infinity")
+        return body.rjust(w) if len(body) <= w else "*" * w
+    neg = np.signbit(v)
+    if v == 0.0:
+        digits, e10 = "0" * d, 0
+    else:
+        m, ex = f"{abs(v):.{d - 1}e}".split("e")
+        digits, e10 = m.replace(".", ""), int(ex) + 1
+    exp = f"E{e10:+03d}" if abs(e10) <= 99 else f"{e10:+04d}"
+    body = ("-" if neg else "") + "0." + digits + exp
+    if len(body) > w:
+        body = body.replace("0.", ".", 1)
+    return body.rjust(w) if len(body) <= w else "*" * w
+
+
+ENSI_VECTOR_HEADER = ("Alya Ensight Gold --- Vector per-node variable file", "part", "     1", "coordinates")
+
+
+def write_ensi_vector(path: str, values, d: int = 5) -> None:
+    """A per-node vector in the EnSight Gold variable file p121 writes
+    (``p121.f90:125-137`` with ``DISMSH_ENSI_P``, ``output.f90:2983-3111``):
+    four header lines, then each component for all nodes, one value per line
+    in Fortran's E12.d."""
+    a = np.asarray(values, dtype=np.float64)
+    with open(path, "w") as fh:
+        fh.write("\n".join(ENSI_VECTOR_HEADER) + "\n")
+        for comp in range(a.shape[1]):
+            fh.write("".join(fortran_e(x, 12, d) + "\n" for x in a[:, comp]))
+
+
+def read_ensi_vector(path: str, ndim: int = 3) -> np.ndarray:
+    """(nn, ndim) values of a file write_ensi_vector (or p121) wrote."""
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    vals = np.array([float(t) for t in lines[len(ENSI_VECTOR_HEADER):] if t.strip()])
+    return vals.reshape(ndim, -1).T.copy()

@@ -81,6 +81,16 @@ def test_fortran_dropin_equals_fused_xx3_tiny(tmp_path, golden):
                          dot_order=O.DOT_DEVICE)
     assert neq2 == neq == xa.size and it == ia
     assert np.array_equal(xa, xo)
+    # post-solve outputs (p121.f90:112-139): element 1's centroid stresses
+    # as printed, and the EnSight displacement file line for line
+    sig = O.centroid_stress(gc, g_g, xo, d["e"], d["v"])[0]
+    assert "The Centroid point stresses for element 1 are\nPoint     1\n" in res
+    assert "".join(M.fortran_e(s_, 12, 4) for s_ in sig) in res
+    assert re.search(r"This analysis took  :\s+[0-9.]+", res)
+    disp = O.scatter_nodes(d["g_num"], O.gather(g_g, xo).reshape(d["nels"], d["nod"], 3), d["nn"], 1)
+    lines = open(deck + ".pf.ensi.DISPL-000001").read().split("\n")
+    assert lines[:4] == list(M.ENSI_VECTOR_HEADER)
+    assert lines[4:-1] == [M.fortran_e(v, 12, 5) for v in disp.T.ravel()]
 
 
 @needs_exe

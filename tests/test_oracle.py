@@ -426,9 +426,14 @@ def test_demo_fixture(golden):
     assert conv and abs(it - meta["res_iters"]) <= 3
     assert round(x[0], 4) == meta["res_x1"]
     x, it, conv = O.pcg(g_g, K, r, dg, meta["tol"], meta["res_iters"], nthreads=8)
-    disp = np.zeros((m.nn, 3))
+    direct = np.zeros((m.nn, 3))
     msk = nf > 0
-    disp[msk] = x[nf[msk] - 1]
+    direct[msk] = x[nf[msk] - 1]
+    # the route p121.f90:125-137 takes: eld = GATHER(xnew), SCATTER_NODES
+    # averaging over the elements of each node
+    eld = O.gather(g_g, x).reshape(m.nels, m.nod, 3)
+    disp = O.scatter_nodes(m.g_num, eld, m.nn, flag=1)
+    assert np.all(np.abs(disp - direct) <= 4 * np.spacing(np.abs(direct)))
     ref = np.load(os.path.join(golden, "p121_demo_displ.npz"))["disp"].astype(np.float64)
     unit = 10.0 ** (np.floor(np.log10(np.maximum(np.abs(ref), 1e-30))) - 3)
     assert np.all(np.abs(disp - ref) <= 0.5 * unit * 1.001 + 6e-6 * np.abs(ref).max())
