@@ -174,7 +174,17 @@ def block_bench(args):
     t_block = time.perf_counter() - t0
     log(f"block solve: {t_block:.3f} s, {res.inner_iters} iterations, {res.outer} residual replacements, "
         f"converged {res.converged}/{k}, max relres {res.max_relres:.3e}")
-    ebe_ms = ctx.block_time_ebe(10)
+    # the kernels' own durations inside the iteration: HIP events around
+    # every launch of the same solve run again (the operands' bits move the
+    # matrix cores' clock, so a short or synthetic run would not represent it)
+    ctx.timing_enable(True)
+    ctx.block_solve(F, 1e-10)
+    tk = ctx.timing_get()
+    ctx.timing_enable(False)
+    ebe_ms = tk["ebe"][0] / max(tk["ebe"][1], 1)
+    log("block iteration kernels (ms per launch): " +
+        ", ".join(f"{kk}={v[0] / max(v[1], 1):.4f}" for kk, v in tk.items() if v[1]))
+    btraffic, bsource = profiled_traffic("block", "k_blk_ebe<60, float, 4>") if n == 60 else (None, None)
     flops = 2.0 * pr.ntot * pr.ntot * k * pr.nels
     tflops = flops / (ebe_ms * 1e-3) / 1e12
     kt_bytes = pr.nels * 4096 * 4
@@ -218,7 +228,8 @@ def block_bench(args):
         "roofline": {
             "bound": "mfma", "kernel": "k_blk_ebe<60, float> (v_mfma_f32_16x16x4f32)",
             "achieved": round(tflops, 2), "peak": 157.3, "unit": "TFLOP/s", "frac": round(tflops / 157.3, 4),
-            "traffic": None, "ebe_ms": ebe_ms,
+            "traffic": btraffic, "traffic_source": f"profiles/{bsource}" if bsource else None,
+            "ebe_ms": ebe_ms,
             "hbm_gbs": round((kt_bytes + ut_bytes) / (ebe_ms * 1e-3) / 1e9, 1),
         },
         "cpu_baseline": None,

@@ -333,7 +333,9 @@ int pf_device_info(pf_ctx *ctx, char *arch, int len, int *cus);
  * iteration (0 element products, 1 scatter + p.u, 2 x/r/d update, 3 p
  * update, 4 the mixed solver's true-residual replacement, whose launches do
  * nothing on iterations without a replacement), the summed milliseconds and
- * the launch count since the last enable. */
+ * the launch count since the last enable.  pf_block_solve records its FP32
+ * iteration the same way (0 block element products, 1 block scatter + p.u,
+ * 2 x/r update, 3 p update). */
 #define PF_TIMING_SLOTS 5
 int pf_pcg_timing_enable(pf_ctx *ctx, int on);
 int pf_pcg_timing_get(pf_ctx *ctx, double ms[PF_TIMING_SLOTS], int counts[PF_TIMING_SLOTS]);

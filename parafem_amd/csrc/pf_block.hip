@@ -727,16 +727,30 @@ int pf_block_solve(pf_ctx *c, int k, const double *f, double *x, const pf_block_
         // FP32 block PCG until some column asks for a residual replacement.
         for (;;) {
             for (int it = 0; it < check; ++it) {
+                // with timing on, one event pair per kernel group (pf_pcg_timing_get)
+                hipEvent_t t0, t1;
+                bool tm = pf_timing_begin(c, PF_K_EBE, &t0, &t1);
+                if (tm) PF_HIP(hipEventRecord(t0, c->stream));
                 rc = blk_launch_ebe<float>(c, b->Kt32, b->P, b->Ut32, b->st);
                 if (rc) return rc;
+                if (tm) PF_HIP(hipEventRecord(t1, c->stream));
+                tm = pf_timing_begin(c, PF_K_SCATTER, &t0, &t1);
+                if (tm) PF_HIP(hipEventRecord(t0, c->stream));
                 hipLaunchKernelGGL((k_blk_scatter<float, true>), dim3(G), dim3(256), 0, c->stream, c->csr_ptr,
                                    c->csr_slot, b->Ut32, b->P, b->U, c->neq, b->partials, b->st);
                 hipLaunchKernelGGL(k_blk_fold_alpha, dim3(1), dim3(256), 0, c->stream, b->partials, G, b->st);
+                if (tm) PF_HIP(hipEventRecord(t1, c->stream));
+                tm = pf_timing_begin(c, PF_K_UPDATE, &t0, &t1);
+                if (tm) PF_HIP(hipEventRecord(t0, c->stream));
                 hipLaunchKernelGGL(k_blk_update, dim3(G), dim3(256), 0, c->stream, b->X, b->R, b->P, b->U, b->dinv,
                                    c->neq, b->partials, b->st);
                 hipLaunchKernelGGL(k_blk_fold_update, dim3(1), dim3(256), 0, c->stream, b->partials, G, b->st);
+                if (tm) PF_HIP(hipEventRecord(t1, c->stream));
+                tm = pf_timing_begin(c, PF_K_PUPDATE, &t0, &t1);
+                if (tm) PF_HIP(hipEventRecord(t0, c->stream));
                 hipLaunchKernelGGL(k_blk_pupdate, dim3(G), dim3(256), 0, c->stream, b->P, b->R, b->dinv, c->neq,
                                    b->st);
+                if (tm) PF_HIP(hipEventRecord(t1, c->stream));
             }
             PF_HIP(hipGetLastError());
             PF_HIP(hipMemcpyAsync(b->st_host, b->st, sizeof(pf_blk_state), hipMemcpyDeviceToHost, c->stream));
