@@ -248,24 +248,32 @@ def serve_bench(args):
     for r in closed:
         cache.get(r.n)
     log(f"serve: {args.requests} requests, n in [{args.nmin},{args.nmax}], sizes cached")
+    done = [0]
+
+    def progress(msg):
+        # one line per 25 finished requests, so a long stream shows progress
+        done[0] += 1
+        if done[0] % 25 == 0:
+            log(f"{done[0]} finished; last: {msg}")
+
     srv = S.Server(slots=args.slots, cache=cache, precision=args.serve_precision)
     try:
         # warm-up: compile/capture paths once
         srv.serve(S.make_stream(args.slots, 0.0, seed=5, nmin=args.nmin, nmax=args.nmin + 2))
         t0 = time.perf_counter()
-        res_closed = srv.serve(closed)
+        res_closed = srv.serve(closed, log=progress)
         cap = S.summarize(res_closed, time.perf_counter() - t0)
         log(f"closed loop: {cap}")
         rate = args.rate if args.rate > 0 else 0.8 * cap["solves_per_s"]
         stream = S.make_stream(args.requests, rate, seed=4, nmin=args.nmin, nmax=args.nmax)
         t0 = time.perf_counter()
-        res_mix = srv.serve(stream)
+        res_mix = srv.serve(stream, log=progress)
         mix = S.summarize(res_mix, time.perf_counter() - t0)
         log(f"stream ({args.serve_precision} server, {args.slots} slots): {mix}")
     finally:
         srv.close()
     t0 = time.perf_counter()
-    res_seq = S.serve_sequential(stream, precision="fp64", cache=cache)
+    res_seq = S.serve_sequential(stream, precision="fp64", cache=cache, log=progress)
     seq = S.summarize(res_seq, time.perf_counter() - t0)
     log(f"stream (FP64 one by one): {seq}")
     out = {

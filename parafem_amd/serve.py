@@ -3,7 +3,9 @@
 Each request is a p121-style hex20 elastic cube (n x n x n elements, unit
 edges, base fixed, uniform pressure on the top face) with its own E, nu and
 pressure.  The loop admits arrived requests in arrival order while a context
-("slot") and the HBM budget allow, builds the problem on that slot (mesh and
+("slot") and the HBM budget allow (counting what every context holds,
+cached blocks included, and handing cached blocks back when they are in the
+way), builds the problem on that slot (mesh and
 steering upload, element matrices assembled on the device), and then drives
 every admitted solve from one host thread: each slot always has one chunk of
 PCG iterations (a replayed HIP graph) queued behind a non-blocking state
@@ -130,7 +132,7 @@ class _Slot:
 @dataclass
 class Server:
     slots: int = 8
-    hbm_budget: float = 160e9
+    hbm_budget: float = 240e9
     chunk: int = 32
     precision: str = "split"
     tol: float = 1e-10
@@ -145,6 +147,26 @@ class Server:
     def close(self):
         for s in self._slots:
             s.ctx.close()
+
+    def _device_bytes(self) -> int:
+        """Device memory the slots hold now: live buffers and each context's
+        cache of freed blocks."""
+        return sum(s.ctx.mem_used() for s in self._slots)
+
+    def _fits(self, slot: _Slot, need: int, busy: bool) -> bool:
+        """Whether a request needing `need` bytes can start on `slot`:
+        within the HBM budget after handing back the slot's cached blocks,
+        then those of idle slots, if needed.  With nothing running a request
+        is always admitted."""
+        if self._device_bytes() + need <= self.hbm_budget:
+            return True
+        slot.ctx.trim()
+        if self._device_bytes() + need <= self.hbm_budget:
+            return True
+        for s in self._slots:
+            if s.req is None:
+                s.ctx.trim()
+        return self._device_bytes() + need <= self.hbm_budget or not busy
 
     def _admit(self, s: _Slot, req: Request, now: float):
         c = self.cache.get(req.n)
@@ -165,7 +187,6 @@ class Server:
         queue: deque[Request] = deque()
         free = list(self._slots)
         active: list[_Slot] = []
-        used = 0.0
         out: list[Result] = []
         t0 = time.perf_counter()
         while pending or queue or active:
@@ -174,12 +195,11 @@ class Server:
                 queue.append(pending.popleft())
             while queue and free:
                 need = device_bytes(queue[0].n, self.precision)
-                if active and used + need > self.hbm_budget:
+                if not self._fits(free[-1], need, bool(active)):
                     break
                 s = free.pop()
                 req = queue.popleft()
                 self._admit(s, req, time.perf_counter() - t0)
-                used += s.bytes
                 active.append(s)
             progressed = False
             for s in list(active):
@@ -199,7 +219,6 @@ class Server:
                     if log:
                         log(f"request {r.rid} n={r.n} iters={res.iters} latency={t - r.arrival:.3f}s")
                     s.ctx.reset()
-                    used -= s.bytes
                     s.req = None
                     active.remove(s)
                     free.append(s)
@@ -216,7 +235,8 @@ class Server:
 
 
 def serve_sequential(requests: list[Request], precision: str = "fp64", tol: float = 1e-10,
-                     device: int = 0, keep_x: bool = False, cache: CubeCache | None = None) -> list[Result]:
+                     device: int = 0, keep_x: bool = False, cache: CubeCache | None = None,
+                     log=None) -> list[Result]:
     """The same stream one request at a time (blocking solves, arrival order,
     real time)."""
     cache = cache or CubeCache()
@@ -233,6 +253,8 @@ def serve_sequential(requests: list[Request], precision: str = "fp64", tol: floa
             t = time.perf_counter() - t0
             out.append(Result(r.rid, r.n, x.size, res.iters, bool(res.converged), res.refinements, ta, t,
                               t - r.arrival, x if keep_x else None, float(np.abs(x).sum())))
+            if log:
+                log(f"request {r.rid} n={r.n} iters={res.iters} latency={t - r.arrival:.3f}s")
     finally:
         ctx.close()
     return out
