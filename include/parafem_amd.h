@@ -142,6 +142,18 @@ int pf_mesh_set(pf_ctx *ctx, int nod, int nels, const double *g_coord_pp);
  * MAKE_GGL's 1-rank tables stand for (src/modules/mpi/gather_scatter.f90:
  * 1387-1780).  Integer maps are exact; no equation may be missing. */
 int pf_steer_set(pf_ctx *ctx, int ntot, int nels, int neq, const int *g_g_pp);
+/* REARRANGE + FIND_G3 / FIND_G4 (src/modules/shared/new_library.f90:
+ * 3059-3271) and the gather of g_g_pp (p121.f90:44-47) with the element
+ * steering built on the device: g_num_pp (nod x nels, node numbers 1..nn)
+ * and rest as the driver holds it after READ_REST (nr rows of nodof+1
+ * ints, row after row: the node, then per freedom 0 restrained / nonzero
+ * free; nodes without a row are free).  Equations are numbered in ascending
+ * node, then freedom, order; *neq receives their number and nf (nodof x nn,
+ * nullable) the node freedom table.  Then as pf_steer_set. */
+int pf_steer_build(pf_ctx *ctx, int nod, int nels, int nodof, const int *g_num_pp, int nn, int nr,
+                   const int *rest, int *neq, int *nf);
+/* The device's g_g_pp (ntot x nels) back to the host. */
+int pf_steer_get(pf_ctx *ctx, int *g_g_pp);
 /* Element stiffness of p121 (p121.f90:54-64): SAMPLE, SHAPE_DER, jacobian,
  * DETERMINANT, INVERT, BEEMAT, DEEMAT, K += B^T D B det w over 2x2x2 Gauss
  * points, fused in one kernel that writes storkm_pp to HBM.  e_elem (host,

@@ -130,6 +130,8 @@ def lib():
         "pf_stress_centroid": (i32, [vp, pd, dbl, dbl, pd, pd]),
         "pf_scatter_nodes": (i32, [vp, i32, i32, i32, pi, i32, pd, pd, i32]),
         "pf_trim": (i32, [vp]),
+        "pf_steer_build": (i32, [vp, i32, i32, i32, pi, i32, i32, pi, pi, pi]),
+        "pf_steer_get": (i32, [vp, pi]),
         "pf_tune_set": (i32, [vp, C.POINTER(Tune)]),
         "pf_tune_get": (i32, [vp, C.POINTER(Tune)]),
         "pf_device_info": (i32, [vp, C.c_char_p, i32, pi]),
@@ -306,6 +308,27 @@ class Context:
         nels, ntot = g.shape
         _check(lib().pf_steer_set(self.h, ntot, nels, int(neq), _pi(g)), "pf_steer_set")
         self.nels, self.ntot, self.neq = nels, ntot, int(neq)
+
+    def steer_build(self, g_num: np.ndarray, rest, nn: int, nodof: int = 3):
+        """Steering built on the device from g_num (nels, nod) 1-based node
+        numbers and rest rows (node, nodof freedom flags; 0 = restrained):
+        returns (neq, nf (nn, nodof))."""
+        g = _i32(g_num)
+        nels, nod = g.shape
+        rr = _i32(np.zeros((0, nodof + 1)) if rest is None else rest)
+        assert rr.ndim == 2 and rr.shape[1] == nodof + 1
+        neq = C.c_int()
+        nf = np.empty((int(nn), nodof), dtype=np.int32)
+        _check(lib().pf_steer_build(self.h, nod, nels, nodof, _pi(g), int(nn), rr.shape[0],
+                                    _pi(rr) if rr.size else None, C.byref(neq), _pi(nf)), "pf_steer_build")
+        self.nels, self.ntot, self.neq = nels, nod * nodof, neq.value
+        return neq.value, nf
+
+    def steer_get(self) -> np.ndarray:
+        """The device's g_g_pp as (nels, ntot)."""
+        g = np.empty((self.nels, self.ntot), dtype=np.int32)
+        _check(lib().pf_steer_get(self.h, _pi(g)), "pf_steer_get")
+        return g
 
     def assemble_elastic(self, e: float, v: float, e_elem=None, store: str = "fp64"):
         ee = None if e_elem is None else _f64(e_elem)

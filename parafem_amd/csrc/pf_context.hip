@@ -174,6 +174,60 @@ static void reset_problem(pf_ctx *c)
     c->nnz = 0;
 }
 
+// A new steering of ntot x nels slots and neq equations: the previous
+// problem's steering, maps and vectors freed (matrices too when ntot
+// changes), g_g, csr_ptr, utemp and the PCG vectors allocated, vectors
+// zeroed.  The caller fills g_g and builds the transposed steering.
+int pf_steer_install(pf_ctx *c, int ntot, int nels, int neq)
+{
+    if (ntot != 60 && ntot != 24 && ntot != 8) return PF_ERR_ARG;   // hex20/hex8 elastic, hex8 Laplace
+    if (c->nels && c->nels != nels) return PF_ERR_ARG;             // pf_mesh_set first for a new mesh
+    const size_t nslot = (size_t)ntot * nels;
+    if (nslot >= (size_t)INT32_MAX) return PF_ERR_ARG;
+    PF_HIP(hipSetDevice(c->device));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    if (c->ntot && c->ntot != ntot) free_matrices(c);
+    free_problem_vectors(c);
+    free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
+    free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
+    free_arr(c, c->csr_slot, (size_t)c->nnz);
+    free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
+    free_arr(c, c->utemp32, (size_t)c->ntot * c->nels);
+    c->ntot = ntot;
+    c->nels = nels;
+    c->neq = neq;
+    c->nnz = 0;
+    int rc = pf_alloc(c, (void **)&c->g_g, nslot * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&c->csr_ptr, ((size_t)neq + 1) * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&c->utemp, nslot * sizeof(double));
+    const size_t nv = (size_t)neq + 1;
+    double **vecs[] = {&c->x, &c->r, &c->p, &c->d, &c->u, &c->dinv, &c->diag, &c->diag0, &c->f};
+    for (double **v : vecs)
+        if (!rc) rc = pf_alloc(c, (void **)v, nv * sizeof(double));
+    if (!rc)
+        for (double **v : vecs)
+            if (hipMemsetAsync(*v, 0, nv * sizeof(double), c->stream) != hipSuccess) rc = PF_ERR_HIP;
+    if (rc) pf_steer_drop(c);
+    return rc;
+}
+
+// Forget the steering (after a failed set-up): maps, vectors and matrices
+// freed, the mesh kept.
+void pf_steer_drop(pf_ctx *c)
+{
+    (void)hipStreamSynchronize(c->stream);
+    free_problem_vectors(c);
+    free_matrices(c);
+    free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
+    free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
+    free_arr(c, c->csr_slot, (size_t)c->nnz);
+    free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
+    free_arr(c, c->utemp32, (size_t)c->ntot * c->nels);
+    c->ntot = c->neq = 0;
+    c->nnz = 0;
+    if (!c->g_coord) c->nels = 0;
+}
+
 extern "C" {
 
 int pf_version(void) { return 200; }
@@ -385,58 +439,17 @@ int pf_mesh_set(pf_ctx *c, int nod, int nels, const double *g_coord)
 int pf_steer_set(pf_ctx *c, int ntot, int nels, int neq, const int *g_g)
 {
     if (!c || !g_g || nels <= 0 || neq <= 0) return PF_ERR_ARG;
-    if (ntot != 60 && ntot != 24 && ntot != 8) return PF_ERR_ARG;   // hex20/hex8 elastic, hex8 Laplace
-    if (c->nels && c->nels != nels) return PF_ERR_ARG;             // pf_mesh_set first for a new mesh
-    const size_t nslot = (size_t)ntot * nels;
-    if (nslot >= (size_t)INT32_MAX) return PF_ERR_ARG;
-    // Transposed steering: for each equation the element slots that touch it
-    // in ascending slot (= ascending element) order, by a counting sort.
-    std::vector<int> cnt((size_t)neq + 1, 0);
-    for (size_t k = 0; k < nslot; ++k) {
-        int g = g_g[k];
-        if (g < 0 || g > neq) return PF_ERR_ARG;
-        if (g) cnt[g]++;
-    }
-    std::vector<int> ptr((size_t)neq + 1, 0);
-    for (int q = 0; q < neq; ++q) {
-        if (cnt[q + 1] == 0) return PF_ERR_ARG;       // equation touched by no element
-        ptr[q + 1] = ptr[q] + cnt[q + 1];
-    }
-    const int64_t nnz = ptr[neq];
-    std::vector<int> slot((size_t)std::max<int64_t>(nnz, 1));
-    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-    for (size_t k = 0; k < nslot; ++k) {
-        int g = g_g[k];
-        if (g) slot[fill[g - 1]++] = (int)k;
-    }
-    PF_HIP(hipSetDevice(c->device));
-    PF_HIP(hipStreamSynchronize(c->stream));
-    if (c->ntot && c->ntot != ntot) free_matrices(c);
-    free_problem_vectors(c);
-    free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
-    free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
-    free_arr(c, c->csr_slot, (size_t)c->nnz);
-    free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
-    free_arr(c, c->utemp32, (size_t)c->ntot * c->nels);
-    c->ntot = ntot;
-    c->nels = nels;
-    c->neq = neq;
-    c->nnz = nnz;
-    int rc = pf_alloc(c, (void **)&c->g_g, nslot * sizeof(int));
-    if (!rc) rc = pf_alloc(c, (void **)&c->csr_ptr, ((size_t)neq + 1) * sizeof(int));
-    if (!rc) rc = pf_alloc(c, (void **)&c->csr_slot, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int));
-    if (!rc) rc = pf_alloc(c, (void **)&c->utemp, nslot * sizeof(double));
-    const size_t nv = (size_t)neq + 1;
-    double **vecs[] = {&c->x, &c->r, &c->p, &c->d, &c->u, &c->dinv, &c->diag, &c->diag0, &c->f};
-    for (double **v : vecs)
-        if (!rc) rc = pf_alloc(c, (void **)v, nv * sizeof(double));
+    int rc = pf_steer_install(c, ntot, nels, neq);
     if (rc) return rc;
-    PF_HIP(hipMemcpyAsync(c->g_g, g_g, nslot * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    PF_HIP(hipMemcpyAsync(c->csr_ptr, ptr.data(), nv * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    PF_HIP(hipMemcpyAsync(c->csr_slot, slot.data(), (size_t)nnz * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    for (double **v : vecs) PF_HIP(hipMemsetAsync(*v, 0, nv * sizeof(double), c->stream));
-    PF_HIP(hipStreamSynchronize(c->stream));
-    return PF_OK;
+    const size_t nslot = (size_t)ntot * nels;
+    hipError_t he = hipMemcpyAsync(c->g_g, g_g, nslot * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (he != hipSuccess) {
+        pf_steer_drop(c);
+        return pf_set_hip_error(he, "steering upload", __FILE__, __LINE__);
+    }
+    rc = pf_csr_build(c);
+    if (rc) pf_steer_drop(c);
+    return rc;
 }
 
 static int alloc_k64(pf_ctx *c, int store, bool shared = false)

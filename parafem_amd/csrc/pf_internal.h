@@ -177,6 +177,11 @@ struct pf_ctx {
     size_t ev_next = 0;
 };
 
+// Steering set-up (pf_context.hip, pf_setup.hip).
+int pf_steer_install(pf_ctx *c, int ntot, int nels, int neq);
+void pf_steer_drop(pf_ctx *c);
+int pf_csr_build(pf_ctx *c);
+
 // Largest elements-per-lane-group of the single-RHS element product.
 inline int pf_epl_max(int ntot) { return ntot > 32 ? 2 : ntot > 8 ? 4 : 8; }
 

@@ -5,9 +5,9 @@ edges, base fixed, uniform pressure on the top face) with its own E, nu and
 pressure.  The loop admits arrived requests in arrival order while a context
 ("slot") and the HBM budget allow (counting what every context holds,
 cached blocks included, and handing cached blocks back when they are in the
-way), builds the problem on that slot (mesh and
-steering upload, element matrices assembled on the device), and then drives
-every admitted solve from one host thread: each slot always has one chunk of
+way), builds the problem on that slot (mesh upload, steering and maps built
+on the device, element matrices assembled there), and then drives every
+admitted solve from one host thread: each slot always has one chunk of
 PCG iterations (a replayed HIP graph) queued behind a non-blocking state
 snapshot, so the GPU runs the solves of all slots concurrently on their own
 streams while the host only polls.  A finished solve's displacements are read
@@ -72,8 +72,9 @@ def make_stream(count: int, rate: float, seed: int = 4, nmin: int = 10, nmax: in
 
 
 class CubeCache:
-    """Host-side mesh, steering and unit load of the n^3 cube, kept for the
-    most recently used sizes."""
+    """Host-side mesh (coordinates, g_num, rest) and unit load of the n^3
+    cube, kept for the most recently used sizes; the steering itself is built
+    on the device at admission."""
 
     def __init__(self, keep: int = 24):
         self.keep = keep
@@ -86,10 +87,10 @@ class CubeCache:
         m = M.box_hex20(n, n, n, 1.0, 1.0, 1.0)
         rest = M.base_fixed_rest(m)
         nf, neq = M.steering(m.nn, 3, rest)
-        g_g = M.g_g_from(nf, m.g_num)
         nodes, vals = M.top_pressure_loads(m, -1.0)
         r_unit = M.nodal_loads_to_rhs(nf, nodes, vals, neq)
-        entry = {"g_coord": m.g_coord(), "g_g": g_g, "neq": neq, "nels": m.nels, "r_unit": r_unit}
+        entry = {"g_coord": m.g_coord(), "g_num": m.g_num, "rest": rest, "nn": m.nn, "neq": neq,
+                 "nels": m.nels, "r_unit": r_unit}
         self._d[n] = entry
         while len(self._d) > self.keep:
             self._d.popitem(last=False)
@@ -112,7 +113,7 @@ def solve_alone(ctx: Context, cache: CubeCache, req: Request, precision: str, to
     """One request by itself on `ctx` (blocking): (x, PcgResult)."""
     c = cache.get(req.n)
     ctx.mesh_set(c["g_coord"])
-    ctx.steer_set(c["g_g"], c["neq"])
+    ctx.steer_build(c["g_num"], c["rest"], c["nn"])
     ctx.assemble_elastic(req.e, req.v, None, store="fp64" if precision == "fp64" else "fp32")
     x, res = ctx.pcg_solve(c["r_unit"] * req.pressure, limit, tol, stop="relres", precision=precision,
                            check_every=64)
@@ -172,7 +173,7 @@ class Server:
         c = self.cache.get(req.n)
         ctx = s.ctx
         ctx.mesh_set(c["g_coord"])
-        ctx.steer_set(c["g_g"], c["neq"])
+        ctx.steer_build(c["g_num"], c["rest"], c["nn"])      # steering and maps built on the device
         ctx.assemble_elastic(req.e, req.v, None, store="fp64" if self.precision == "fp64" else "fp32")
         ctx.pcg_begin(c["r_unit"] * req.pressure, self.limit, self.tol, stop="relres", precision=self.precision)
         s.req, s.t_admit, s.bytes, s.chunks = req, now, device_bytes(req.n, self.precision), 0

@@ -609,6 +609,57 @@ def test_scatter_nodes_bitexact(ctx):
     assert np.array_equal(a, b, equal_nan=True)
 
 
+def test_steering_built_on_device(ctx):
+    """pf_steer_build: the node freedom table (ascending node, then
+    freedom; a later rest row for a node overrides an earlier one), g_g and
+    neq equal the host construction; the device-built transposed steering
+    gives the PCG bit for bit what pf_steer_set's does."""
+    pr = _distorted(3, seed=15)
+    m = pr.mesh
+    rest = M.base_fixed_rest(m)
+    rest = np.concatenate([rest, [[m.nn, 1, 0, 1], [m.nn, 0, 1, 1]]]).astype(np.int32)   # last row wins
+    nf_h, neq_h = M.steering(m.nn, 3, rest)
+    g_h = M.g_g_from(nf_h, m.g_num)
+    say(f"steer_build nn={m.nn} nels={m.nels} neq={neq_h}")
+    ctx.mesh_set(pr.g_coord())
+    neq, nf = ctx.steer_build(m.g_num, rest, m.nn)
+    assert neq == neq_h and np.array_equal(nf, nf_h)
+    assert np.array_equal(ctx.steer_get(), g_h)
+    r = M.nodal_loads_to_rhs(nf_h, *M.top_pressure_loads(m, -1.0), neq_h)
+    ctx.assemble_elastic(pr.e, pr.v, pr.e_elem)
+    xa, ra = ctx.pcg_solve(r, 3000, 1e-10, stop="relres")
+    ctx.steer_set(g_h, neq_h)
+    ctx.assemble_elastic(pr.e, pr.v, pr.e_elem)
+    xb, rb = ctx.pcg_solve(r, 3000, 1e-10, stop="relres")
+    assert ra.iters == rb.iters and np.array_equal(xa, xb)
+    Ko = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem)
+    xo, it, _ = O.pcg(g_h, Ko, r, O.diag(g_h, Ko, neq_h), 1e-10, 3000, stop=O.STOP_RELRES, dot_order=O.DOT_DEVICE)
+    assert it == ra.iters and np.array_equal(xa, xo)
+
+
+def test_steering_rejects_bad_maps(ctx):
+    """An equation number outside 0..neq, an equation no element touches,
+    or a node number outside 1..nn is refused; the context then takes a
+    valid steering again."""
+    pr = P.elastic_cube(2)
+    say("bad steering maps")
+    ctx.mesh_set(pr.g_coord())
+    bad = pr.g_g.copy()
+    bad[0, 0] = pr.neq + 1
+    with pytest.raises(PFError):
+        ctx.steer_set(bad, pr.neq)
+    with pytest.raises(PFError):
+        ctx.steer_set(pr.g_g, pr.neq + 1)             # equation neq+1 has no slot
+    gn = pr.mesh.g_num.copy()
+    gn[0, 0] = pr.mesh.nn + 1
+    with pytest.raises(PFError):
+        ctx.steer_build(gn, None, pr.mesh.nn)
+    ctx.mesh_set(pr.g_coord())
+    r = P.setup(ctx, pr)
+    x, res = ctx.pcg_solve(r, 500, 1e-8, stop="relres")
+    assert res.converged
+
+
 def test_context_reuse_and_trim():
     """A context solving problems one after another reuses its freed device
     blocks (same result as a fresh context, bit for bit); pf_trim gives the
