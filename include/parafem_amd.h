@@ -163,6 +163,15 @@ int pf_assemble_elastic(pf_ctx *ctx, double e, double v, const double *e_elem, i
  * storkc = kx*kcx + ky*kcy + kz*kcz; k_elem (host, nullable) scales all three
  * per element. */
 int pf_assemble_laplace(pf_ctx *ctx, double kx, double ky, double kz, const double *k_elem, int store);
+/* p124's element loop (src/programs/5th_ed/p124/p124.f90:73-86) on 8-node
+ * bricks, one freedom per node: conductivity kc (kay = diag(kx,ky,kz)) and
+ * consistent capacity pm (SHAPE_FUN, rho*cp) per Gauss point, then
+ * storka_pp = pm + kc*theta*dtim, the PCG's matrices (stored as `store`),
+ * and storkb_pp = pm - kc*(1-theta)*dtim (FP64, for pf_apply_raw).
+ * prop: (5, np_types) column-major kx, ky, kz, rho, cp per type
+ * (READ_MATERIAL); etype: nels type numbers 1..np_types. */
+int pf_assemble_transient(pf_ctx *ctx, double theta, double dtim, int np_types, const double *prop,
+                          const int *etype, int store);
 /* Upload / download storkm_pp(ntot,ntot,nels) (host, FP64).  Upload lets a
  * caller keep its own assembly (the driver's loop A) and use the device PCG. */
 int pf_storkm_set(pf_ctx *ctx, const double *storkm_pp, int store);
@@ -248,6 +257,11 @@ int pf_pcg_query(pf_ctx *ctx, pf_pcg_result *res, int *ready);
 /* y = K x over the assembled operator in FP64 (host vectors, neq), with the
  * p123 fixed-freedom rows applied.  Used to report true residuals. */
 int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);
+/* The element products of p124's time loop without the fixed-freedom rows:
+ * y = SCATTER(storka_pp . GATHER(x)) (which = 0, the PCG's matrices, in
+ * FP64) or storkb_pp (which = 1) (p124.f90:158-162, 171-174, 191-194).
+ * Host vectors, neq. */
+int pf_apply_raw(pf_ctx *ctx, int which, const double *x_host, double *y_host);
 
 /* ------------------------------------------------------- post-solve */
 /* Stresses at the centroid of every element, p121.f90:112-123 (which

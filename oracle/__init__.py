@@ -22,6 +22,12 @@ Pinning status (how each function is checked in ``tests/test_oracle.py``):
   checon -- exact binary cases;
   dot_device, sum_p -- math.fsum within rounding; total load of the fixtures;
   centroid_stress -- element 1 of the p121 demo against its results file;
+  shape_fun_hex8 -- bit for bit against the reference's SHAPE_FUN in
+      oracle/_ref; partition of unity and the nodal delta property;
+  transient_km, assemble_transient -- kc equals the p123 element (another
+      summation order) to rounding, pm's entries sum to rho*cp*volume and
+      its closed form on a box, storka/storkb combine them as stated, and
+      the p124 demo's temperature history (.res) and fields (NDTTR);
   scatter_nodes -- element counts per node by brute force, exact averages
       of equal contributions where the count is a power of two, the sums
       conserved, and the p121 demo's printed displacement field;
@@ -63,6 +69,9 @@ def lib():
         L.or_centroid_stress.argtypes = [C.c_int, C.c_int, d, i, d, C.c_double, C.c_double, d, d]
         L.or_sample_hex8.argtypes = [d, d]
         L.or_scatter_nodes.argtypes = [C.c_int, C.c_int, C.c_int, i, C.c_int, d, d, C.c_int]
+        L.or_shape_fun_hex8.argtypes = [d, d, C.c_int, C.c_int]
+        L.or_transient_km.argtypes = [d, d, C.c_double, C.c_double, C.c_double, C.c_double, d, d]
+        L.or_assemble_transient.argtypes = [C.c_int, d, d, i, C.c_double, C.c_double, d, d, C.c_int]
         L.or_deemat6.argtypes = [d, C.c_double, C.c_double]
         L.or_shape_der.argtypes = [d, d, C.c_int, C.c_int, C.c_int]
         L.or_shape_der.restype = C.c_int
@@ -187,6 +196,35 @@ def scatter_nodes(g_num, contrib, nn, flag=1):
     out = np.zeros(nn * numvar)
     lib().or_scatter_nodes(int(nn), nod, nels, _i(g), numvar, _d(c), _d(out), int(flag))
     return out.reshape(nn, numvar)
+
+
+def shape_fun_hex8(points: np.ndarray, i: int):
+    """SHAPE_FUN for the 8-node brick at Gauss point i (1-based)."""
+    p = np.asfortranarray(points, dtype=np.float64)
+    out = np.zeros(8)
+    lib().or_shape_fun_hex8(_d(out), _d(p), p.shape[0], int(i))
+    return out
+
+
+def transient_km(coord, kay, rho, cp, theta, dtim):
+    """p124's storka, storkb of one 8-node brick (coord (8,3))."""
+    c = _f64(np.asarray(coord).T)
+    k3 = _f64(kay)
+    ka, kb = np.zeros(64), np.zeros(64)
+    lib().or_transient_km(_d(c), _d(k3), float(rho), float(cp), float(theta), float(dtim), _d(ka), _d(kb))
+    return ka.reshape(8, 8).T.copy(), kb.reshape(8, 8).T.copy()
+
+
+def assemble_transient(g_coord, prop, etype, theta, dtim, nthreads=0):
+    """p124's storka_pp and storkb_pp as (nels, 8, 8) in storkm order."""
+    g = _f64(g_coord)
+    nels = g.shape[0]
+    pr = np.ascontiguousarray(prop, dtype=np.float64)          # (np_types, 5)
+    et = _i32(etype)
+    ka, kb = np.zeros((nels, 8, 8)), np.zeros((nels, 8, 8))
+    lib().or_assemble_transient(nels, _d(g), _d(pr), _i(et), float(theta), float(dtim), _d(ka), _d(kb),
+                                int(nthreads))
+    return ka, kb
 
 
 def laplace_kc(coord, kx=1.0, ky=1.0, kz=1.0):

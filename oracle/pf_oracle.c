@@ -308,6 +308,64 @@ void or_laplace_kc(const double *coord, int nod, double kx, double ky, double kz
     free(kcx);
 }
 
+/* SHAPE_FUN(fun, points, i) for the 8-node brick (new_library.f90:396-420):
+ * fun = 0.125 * (1 -/+ xi) (1 -/+ eta) (1 -/+ zeta) per node, evaluated left
+ * to right as written there.  points (nip,3) column-major, i 1-based. */
+void or_shape_fun_hex8(double *fun, const double *points, int nip, int i)
+{
+    const double xi = points[IDX2(i - 1, 0, nip)], eta = points[IDX2(i - 1, 1, nip)];
+    const double zeta = points[IDX2(i - 1, 2, nip)];
+    const double etam = 1.0 - eta, xim = 1.0 - xi, zetam = 1.0 - zeta;
+    const double etap = eta + 1.0, xip = xi + 1.0, zetap = zeta + 1.0;
+    fun[0] = 0.125 * xim * etam * zetam;
+    fun[1] = 0.125 * xim * etam * zetap;
+    fun[2] = 0.125 * xip * etam * zetap;
+    fun[3] = 0.125 * xip * etam * zetam;
+    fun[4] = 0.125 * xim * etap * zetam;
+    fun[5] = 0.125 * xim * etap * zetap;
+    fun[6] = 0.125 * xip * etap * zetap;
+    fun[7] = 0.125 * xip * etap * zetam;
+}
+
+/* One element of p124's loop (p124.f90:73-86), 8-node brick, nodof = 1:
+ * per Gauss point kc = kc + MATMUL(MATMUL(TRANSPOSE(deriv),kay),deriv)*det
+ * *weights(i) and pm = pm + MATMUL(TRANSPOSE(funny),funny)*det*weights(i)
+ * *rho*cp; then ka = pm + kc*theta*dtim, kb = pm - kc*(1-theta)*dtim.
+ * kay = diag(kay3); every MATMUL sums its terms in index order. */
+void or_transient_km(const double *coord, const double *kay3, double rho, double cp, double theta, double dtim,
+                     double *ka, double *kb)
+{
+    double points[24], weights[8], der[24], deriv[24], fun[8], kay[9], t1[24], kc[64], pm[64];
+    for (int k = 0; k < 9; ++k) kay[k] = 0.0;
+    for (int k = 0; k < 3; ++k) kay[IDX2(k, k, 3)] = kay3[k];
+    for (int k = 0; k < 64; ++k) kc[k] = pm[k] = 0.0;
+    or_sample_hex8(points, weights);
+    for (int gp = 1; gp <= 8; ++gp) {
+        or_shape_der(der, points, 8, gp, 8);
+        or_shape_fun_hex8(fun, points, 8, gp);
+        double det = jac_deriv(der, coord, 8, deriv);
+        for (int a = 0; a < 8; ++a)            /* t1 = MATMUL(TRANSPOSE(deriv), kay): (8,3) */
+            for (int c = 0; c < 3; ++c) {
+                double s = 0.0;
+                for (int m = 0; m < 3; ++m) s = s + deriv[IDX2(m, a, 3)] * kay[IDX2(m, c, 3)];
+                t1[IDX2(a, c, 8)] = s;
+            }
+        for (int b = 0; b < 8; ++b)
+            for (int a = 0; a < 8; ++a) {
+                double s = 0.0;                /* MATMUL(t1, deriv) */
+                for (int c = 0; c < 3; ++c) s = s + t1[IDX2(a, c, 8)] * deriv[IDX2(c, b, 3)];
+                kc[IDX2(a, b, 8)] = kc[IDX2(a, b, 8)] + s * det * weights[gp - 1];
+                double f = 0.0;                /* MATMUL(TRANSPOSE(funny), funny) */
+                f = f + fun[a] * fun[b];
+                pm[IDX2(a, b, 8)] = pm[IDX2(a, b, 8)] + f * det * weights[gp - 1] * rho * cp;
+            }
+    }
+    for (int k = 0; k < 64; ++k) {
+        ka[k] = pm[k] + kc[k] * theta * dtim;
+        kb[k] = pm[k] - kc[k] * (1.0 - theta) * dtim;
+    }
+}
+
 static int omp_threads(int nthreads)
 {
 #ifdef _OPENMP
@@ -332,6 +390,19 @@ void or_assemble_elastic(int nels, int nod, const double *g_coord, const double 
         if (e_elem) or_deemat6(dee, e_elem[iel], v);
         else memcpy(dee, dee0, sizeof dee);
         or_elastic_km(g_coord + (size_t)iel * nod * 3, nod, dee, storkm + (size_t)iel * ntot * ntot);
+    }
+}
+
+/* p124's loop over all elements: prop (5, np_types) = kx, ky, kz, rho, cp,
+ * etype 1-based; storka and storkb (8,8,nels) column-major. */
+void or_assemble_transient(int nels, const double *g_coord, const double *prop, const int *etype, double theta,
+                           double dtim, double *storka, double *storkb, int nthreads)
+{
+#pragma omp parallel for schedule(static) num_threads(omp_threads(nthreads))
+    for (long iel = 0; iel < nels; ++iel) {
+        const double *p = prop + 5 * (size_t)(etype[iel] - 1);
+        or_transient_km(g_coord + (size_t)iel * 24, p, p[3], p[4], theta, dtim, storka + (size_t)iel * 64,
+                        storkb + (size_t)iel * 64);
     }
 }
 

@@ -131,6 +131,8 @@ def lib():
         "pf_scatter_nodes": (i32, [vp, i32, i32, i32, pi, i32, pd, pd, i32]),
         "pf_trim": (i32, [vp]),
         "pf_steer_build": (i32, [vp, i32, i32, i32, pi, i32, i32, pi, pi, pi]),
+        "pf_assemble_transient": (i32, [vp, dbl, dbl, i32, pd, pi, i32]),
+        "pf_apply_raw": (i32, [vp, i32, pd, pd]),
         "pf_steer_get": (i32, [vp, pi]),
         "pf_tune_set": (i32, [vp, C.POINTER(Tune)]),
         "pf_tune_get": (i32, [vp, C.POINTER(Tune)]),
@@ -308,6 +310,22 @@ class Context:
         nels, ntot = g.shape
         _check(lib().pf_steer_set(self.h, ntot, nels, int(neq), _pi(g)), "pf_steer_set")
         self.nels, self.ntot, self.neq = nels, ntot, int(neq)
+
+    def assemble_transient(self, theta: float, dtim: float, prop, etype, store: str = "fp64"):
+        """p124's element matrices: prop (np_types, 5) kx, ky, kz, rho, cp;
+        etype (nels,) 1-based types."""
+        pr = np.ascontiguousarray(prop, dtype=np.float64)
+        et = _i32(etype)
+        assert pr.ndim == 2 and pr.shape[1] == 5 and et.size == self.nels
+        _check(lib().pf_assemble_transient(self.h, float(theta), float(dtim), pr.shape[0], _pd(pr), _pi(et),
+                                           STORE[store]), "pf_assemble_transient")
+
+    def apply_raw(self, which: int, x) -> np.ndarray:
+        """storka (which 0) or storkb (which 1) times x, no fixed rows."""
+        xx = _f64(x)
+        y = np.empty(self.neq)
+        _check(lib().pf_apply_raw(self.h, int(which), _pd(xx), _pd(y)), "pf_apply_raw")
+        return y
 
     def steer_build(self, g_num: np.ndarray, rest, nn: int, nodof: int = 3):
         """Steering built on the device from g_num (nels, nod) 1-based node

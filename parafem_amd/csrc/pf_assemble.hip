@@ -243,6 +243,56 @@ __global__ __launch_bounds__(64 * WAVES) void k_assemble_laplace8(
     K[pf_idx_pair(e, 8, a, b)] = cx * (kx * s) + cy * (ky * s) + cz * (kz * s);
 }
 
+// p124's element loop (src/programs/5th_ed/p124/p124.f90:73-86) on an
+// 8-node brick: per Gauss point kc += MATMUL(MATMUL(TRANSPOSE(deriv),kay),
+// deriv)*det*weights(i) with kay = diag(kx,ky,kz), pm += MATMUL(TRANSPOSE(
+// funny),funny)*det*weights(i)*rho*cp with funny = SHAPE_FUN
+// (new_library.f90:416-420); then storka = pm + kc*theta*dtim and storkb =
+// pm - kc*(1-theta)*dtim.  prop(5,np_types) = kx, ky, kz, rho, cp per
+// element type (READ_MATERIAL).  Every product and sum in the reference's
+// order: MATMUL(TRANSPOSE(deriv),kay) has one nonzero term per entry (the
+// others add exact zeros), the weights of SAMPLE are 1.
+__global__ __launch_bounds__(64 * WAVES) void k_assemble_transient8(
+    const double *__restrict__ g_coord, const double *__restrict__ prop, const int *__restrict__ etype,
+    double theta, double dtim, double *__restrict__ Ka, double *__restrict__ Kb, int nels)
+{
+    __shared__ ElemLds<8> lds[WAVES];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long e = (long)blockIdx.x * WAVES + wave;
+    const bool valid = e < nels;
+    ElemLds<8> &L = lds[wave];
+    element_geometry<8>(L, g_coord + (valid ? e : 0) * 24, lane, valid);
+    if (!valid) return;
+    const double *pr = prop + 5 * (long)(etype[e] - 1);
+    const double kay[3] = {pr[0], pr[1], pr[2]}, rho = pr[3], cp = pr[4];
+    const int a = lane & 7, b = lane >> 3;   // row a, column b
+    // SHAPE_FUN, 8-node brick: node -> (xi, eta, zeta) factors, - = (1 - s),
+    // + = (s + 1), each 0.125 * f_xi * f_eta * f_zeta
+    constexpr int sx[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    constexpr int sy[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    constexpr int sz[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    constexpr int gs[8][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {1, -1, -1},
+                              {-1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {-1, -1, -1}};
+    const double r3 = 1.0 / sqrt(3.0);
+    double kc = 0.0, pm = 0.0;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+        const double xi = gs[g][0] > 0 ? r3 : -r3, eta = gs[g][1] > 0 ? r3 : -r3, zeta = gs[g][2] > 0 ? r3 : -r3;
+        const double xim = 1.0 - xi, etam = 1.0 - eta, zetam = 1.0 - zeta;
+        const double xip = xi + 1.0, etap = eta + 1.0, zetap = zeta + 1.0;
+        const double fa = 0.125 * (sx[a] < 0 ? xim : xip) * (sy[a] < 0 ? etam : etap) * (sz[a] < 0 ? zetam : zetap);
+        const double fb = 0.125 * (sx[b] < 0 ? xim : xip) * (sy[b] < 0 ? etam : etap) * (sz[b] < 0 ? zetam : zetap);
+        const double det = L.dw[g];
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) t = t + (L.deriv[g][m][a] * kay[m]) * L.deriv[g][m][b];
+        kc = kc + t * det * 1.0;
+        pm = pm + fa * fb * det * 1.0 * rho * cp;
+    }
+    Ka[pf_idx_pair(e, 8, a, b)] = pm + kc * theta * dtim;
+    Kb[pf_idx_pair(e, 8, a, b)] = pm - kc * (1.0 - theta) * dtim;
+}
+
 // hi = fl32(K), lo = fl32(K - hi) in the packed symmetric layout (upper
 // triangle): hi + lo carries ~48 significant bits of K.
 __global__ void k_split_fp32(const double *__restrict__ K, float *__restrict__ hi, float *__restrict__ lo,
@@ -342,6 +392,16 @@ int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const
     const int grid = (c->nels + WAVES - 1) / WAVES;
     hipLaunchKernelGGL(k_assemble_laplace8, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord, k_elem, kx,
                        ky, kz, c->K64, c->nels);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+int pf_launch_assemble_transient(pf_ctx *c, double theta, double dtim, const double *prop, const int *etype)
+{
+    if (c->nod != 8) return PF_ERR_ARG;
+    const int grid = (c->nels + WAVES - 1) / WAVES;
+    hipLaunchKernelGGL(k_assemble_transient8, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord, prop, etype,
+                       theta, dtim, c->K64, c->K64b, c->nels);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }

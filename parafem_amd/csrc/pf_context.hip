@@ -150,6 +150,7 @@ static void free_matrices(pf_ctx *c)
     pf_block_free(c);
     const size_t ns = (size_t)pf_sym_per(c->ntot) * c->nels;
     free_arr(c, c->K64, pf_k64_count(c));
+    free_arr(c, c->K64b, (size_t)c->ntot * c->ntot * c->nels);
     free_arr(c, c->K32hi, ns);
     free_arr(c, c->K32lo, ns);
     c->have_K = false;
@@ -538,6 +539,36 @@ int pf_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double
         (void)hipStreamSynchronize(c->stream);
         pf_free(c, kk, sizeof(double) * c->nels);
     }
+    if (rc) return rc;
+    return finish_matrices(c);
+}
+
+int pf_assemble_transient(pf_ctx *c, double theta, double dtim, int np_types, const double *prop,
+                          const int *etype, int store)
+{
+    if (!c || !prop || !etype || np_types < 1) return PF_ERR_ARG;
+    if (!c->g_coord || !c->g_g) return PF_ERR_STATE;
+    if (c->ntot != c->nod || c->nod != 8) return PF_ERR_ARG;
+    for (int e = 0; e < c->nels; ++e)
+        if (etype[e] < 1 || etype[e] > np_types) return PF_ERR_ARG;
+    PF_HIP(hipSetDevice(c->device));
+    int rc = alloc_k64(c, store);
+    if (!rc) rc = pf_alloc(c, (void **)&c->K64b, (size_t)64 * c->nels * sizeof(double));
+    if (rc) return rc;
+    double *dprop = nullptr;
+    int *dtype = nullptr;
+    rc = pf_alloc(c, (void **)&dprop, sizeof(double) * 5 * np_types);
+    if (!rc) rc = pf_alloc(c, (void **)&dtype, sizeof(int) * c->nels);
+    if (!rc) {
+        hipError_t he = hipMemcpyAsync(dprop, prop, sizeof(double) * 5 * np_types, hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(dtype, etype, sizeof(int) * c->nels, hipMemcpyHostToDevice, c->stream);
+        if (he != hipSuccess) rc = pf_set_hip_error(he, "transient upload", __FILE__, __LINE__);
+    }
+    if (!rc) rc = pf_launch_assemble_transient(c, theta, dtim, dprop, dtype);
+    (void)hipStreamSynchronize(c->stream);
+    pf_free(c, dprop, sizeof(double) * 5 * np_types);
+    pf_free(c, dtype, sizeof(int) * c->nels);
     if (rc) return rc;
     return finish_matrices(c);
 }

@@ -701,22 +701,23 @@ __global__ void k_scatter_fix(const int *__restrict__ ptr, const int *__restrict
 }
 
 template <int NT, int KIND, bool GATHER, typename OUT, int EPL>
-void launch_ebe_epl(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate)
+void launch_ebe_epl(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *K64)
 {
     constexpr int EPB = Lanes<NT, KIND>::EPW * EPL * (PF_BLOCK / 64);
     const long nlb = (c->nels + EPB - 1) / EPB;
     hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, false, EPL>), dim3((int)nlb), dim3(PF_BLOCK), 0, c->stream,
-                       c->K64, c->K32hi, c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
+                       K64, c->K32hi, c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
 }
 
 template <int NT, int KIND, bool GATHER, typename OUT>
-int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate)
+int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *Kalt)
 {
+    const double *K64 = Kalt ? Kalt : c->K64;
     if (gate == RUN_REFINE) {
         // usually does nothing: dispatch few workgroups that loop
         const long nlb = (c->nels + Lanes<NT, KIND>::EPB - 1) / Lanes<NT, KIND>::EPB;
         const int grid = (int)std::min<long>(nlb, 2048);
-        hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, true>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, c->K64,
+        hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, true>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, K64,
                            c->K32hi, c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
     } else {
         // the tuned shape of the iteration's product (an out-of-range value,
@@ -724,13 +725,13 @@ int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, in
         constexpr int E0 = Lanes<NT, KIND>::EPL;
         const int e = GATHER && c->tune_epl <= pf_epl_max(NT) ? c->tune_epl : 0;
         if constexpr (GATHER) {
-            if (e == 1) launch_ebe_epl<NT, KIND, GATHER, OUT, 1>(c, src, out, st, gate);
-            else if (e == 2) launch_ebe_epl<NT, KIND, GATHER, OUT, 2>(c, src, out, st, gate);
-            else if (e == 4) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 24 ? 4 : E0)>(c, src, out, st, gate);
-            else if (e == 8) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 8 ? 8 : E0)>(c, src, out, st, gate);
-            else launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate);
+            if (e == 1) launch_ebe_epl<NT, KIND, GATHER, OUT, 1>(c, src, out, st, gate, K64);
+            else if (e == 2) launch_ebe_epl<NT, KIND, GATHER, OUT, 2>(c, src, out, st, gate, K64);
+            else if (e == 4) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 24 ? 4 : E0)>(c, src, out, st, gate, K64);
+            else if (e == 8) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 8 ? 8 : E0)>(c, src, out, st, gate, K64);
+            else launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate, K64);
         } else {
-            launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate);
+            launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate, K64);
         }
     }
     PF_HIP(hipGetLastError());
@@ -738,12 +739,12 @@ int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, in
 }
 
 template <int KIND, bool GATHER, typename OUT>
-int launch_ebe(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate)
+int launch_ebe(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *Kalt = nullptr)
 {
     switch (c->ntot) {
-    case 60: return launch_ebe_nt<60, KIND, GATHER, OUT>(c, src, out, st, gate);
-    case 24: return launch_ebe_nt<24, KIND, GATHER, OUT>(c, src, out, st, gate);
-    case 8: return launch_ebe_nt<8, KIND, GATHER, OUT>(c, src, out, st, gate);
+    case 60: return launch_ebe_nt<60, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt);
+    case 24: return launch_ebe_nt<24, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt);
+    case 8: return launch_ebe_nt<8, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt);
     default: return PF_ERR_ARG;
     }
 }
@@ -1126,10 +1127,10 @@ int pf_pcg_solve(pf_ctx *c, const double *r0, double *x, const pf_pcg_opts *opts
     return R->status;
 }
 
-int pf_apply(pf_ctx *c, const double *x_host, double *y_host)
+// y = K x through gather / element products / ascending scatter in FP64 (the
+// stored matrices, or K64b), with or without the p123 fixed-freedom rows.
+static int apply_host(pf_ctx *c, const double *x_host, double *y_host, const double *Kalt, bool fixed_rows)
 {
-    if (!c || !x_host || !y_host) return PF_ERR_ARG;
-    if (!c->have_K) return PF_ERR_STATE;
     PF_HIP(hipSetDevice(c->device));
     const size_t nv = (size_t)c->neq + 1, ns = (size_t)c->ntot * c->nels;
     double *xe = nullptr, *y = nullptr, *ut = nullptr;
@@ -1142,10 +1143,13 @@ int pf_apply(pf_ctx *c, const double *x_host, double *y_host)
         if (e == hipSuccess) e = hipMemcpyAsync(xe + 1, x_host, c->neq * sizeof(double), hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) rc = pf_set_hip_error(e, "pf_apply upload", __FILE__, __LINE__);
     }
-    if (!rc) rc = launch_ebe_exact<true>(c, xe, ut);
+    if (!rc) {
+        if (Kalt) rc = launch_ebe<KIND_F64, true, double>(c, xe, ut, nullptr, RUN_ALWAYS, Kalt);
+        else rc = launch_ebe_exact<true>(c, xe, ut);
+    }
     if (!rc) {
         hipLaunchKernelGGL(k_scatter_fix, dim3(pf_grid_for(c->neq)), dim3(256), 0, c->stream, c->csr_ptr,
-                           c->csr_slot, ut, c->fixstore, xe, y, c->neq);
+                           c->csr_slot, ut, fixed_rows ? c->fixstore : (const double *)nullptr, xe, y, c->neq);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(y_host, y, c->neq * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1156,6 +1160,20 @@ int pf_apply(pf_ctx *c, const double *x_host, double *y_host)
     pf_free(c, y, nv * sizeof(double));
     pf_free(c, ut, ns * sizeof(double));
     return rc;
+}
+
+int pf_apply(pf_ctx *c, const double *x_host, double *y_host)
+{
+    if (!c || !x_host || !y_host) return PF_ERR_ARG;
+    if (!c->have_K) return PF_ERR_STATE;
+    return apply_host(c, x_host, y_host, nullptr, true);
+}
+
+int pf_apply_raw(pf_ctx *c, int which, const double *x_host, double *y_host)
+{
+    if (!c || !x_host || !y_host || (which != 0 && which != 1)) return PF_ERR_ARG;
+    if (!c->have_K || (which == 1 && !c->K64b)) return PF_ERR_STATE;
+    return apply_host(c, x_host, y_host, which == 1 ? c->K64b : nullptr, false);
 }
 
 }  // extern "C"

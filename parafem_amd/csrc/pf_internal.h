@@ -134,6 +134,7 @@ struct pf_ctx {
     // element matrices (internal layouts above)
     int store = PF_STORE_FP64;
     double *K64 = nullptr;             // FP64 column pairs (PF_STORE_FP64)
+    double *K64b = nullptr;            // p124's second matrix set (storkb_pp), FP64 column pairs
     bool k_shared = false;             // K64 holds one matrix used by every element
     float *K32hi = nullptr;            // packed symmetric FP32, hi = fl32(K)
     float *K32lo = nullptr;            // packed symmetric FP32, lo = fl32(K - hi)
@@ -176,6 +177,9 @@ struct pf_ctx {
     std::vector<std::pair<int, int>> ev_used[PF_NKERNELS];   // (start, stop) indices
     size_t ev_next = 0;
 };
+
+// p124's transient element matrices into K64 (storka) and K64b (storkb).
+int pf_launch_assemble_transient(pf_ctx *c, double theta, double dtim, const double *prop, const int *etype);
 
 // Steering set-up (pf_context.hip, pf_setup.hip).
 int pf_steer_install(pf_ctx *c, int ntot, int nels, int neq);

@@ -58,6 +58,7 @@ MODULE pf_gpu
   PUBLIC :: dot_product_p, sum_p, checon_par
   PUBLIC :: vec_fill, vec_waxpy, vec_mul, vec_recip
   PUBLIC :: pf_stress, calc_nodes_pp, scatter_nodes, dismsh_ensi_p
+  PUBLIC :: pf_assemble_transient, pf_apply_raw
 
   INTERFACE
     INTEGER(c_int) FUNCTION c_create(device, c) BIND(C, name="pf_create")
@@ -232,6 +233,22 @@ MODULE pf_gpu
       REAL(c_double), VALUE :: e, v
       TYPE(c_ptr), VALUE :: e_elem
       REAL(c_double), INTENT(OUT) :: sigma(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_assemble_transient(c, theta, dtim, np_types, prop, etype, store) &
+        BIND(C, name="pf_assemble_transient")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      REAL(c_double), VALUE :: theta, dtim
+      INTEGER(c_int), VALUE :: np_types, store
+      REAL(c_double), INTENT(IN) :: prop(*)
+      INTEGER(c_int), INTENT(IN) :: etype(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_apply_raw(c, which, x, y) BIND(C, name="pf_apply_raw")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c
+      INTEGER(c_int), VALUE :: which
+      REAL(c_double), INTENT(IN) :: x(*)
+      REAL(c_double), INTENT(OUT) :: y(*)
     END FUNCTION
     INTEGER(c_int) FUNCTION c_scatter_nodes(c, nn, nod, nels, g_num, numvar, contrib, nodal, flag) &
         BIND(C, name="pf_scatter_nodes")
@@ -506,6 +523,25 @@ CONTAINS
     iters = res%iters
     converged = res%converged /= 0
   END SUBROUTINE pf_pcg
+
+  !> p124's element loop (p124.f90:73-86) on the device: storka_pp (the
+  !> PCG's matrices, stored as `store`) and storkb_pp from prop(5,np_types)
+  !> (kx, ky, kz, rho, cp) and the element types etype_pp.
+  SUBROUTINE pf_assemble_transient(theta, dtim, prop, etype_pp, store)
+    REAL(iwp), INTENT(IN) :: theta, dtim, prop(:, :)
+    INTEGER, INTENT(IN) :: etype_pp(:), store
+    CALL pf_check(c_assemble_transient(ctx, theta, dtim, INT(SIZE(prop, 2), c_int), prop, etype_pp, &
+                                       INT(store, c_int)), 'pf_assemble_transient')
+  END SUBROUTINE pf_assemble_transient
+
+  !> u = SCATTER(storka_pp . GATHER(x)) (which = 0) or storkb_pp (which = 1),
+  !> without fixed-freedom rows (p124.f90:158-162, 171-174).
+  SUBROUTINE pf_apply_raw(which, x, u)
+    INTEGER, INTENT(IN) :: which
+    REAL(iwp), INTENT(IN) :: x(:)
+    REAL(iwp), INTENT(OUT) :: u(:)
+    CALL pf_check(c_apply_raw(ctx, INT(which, c_int), x, u), 'pf_apply_raw')
+  END SUBROUTINE pf_apply_raw
 
   !> Stresses at every element's centroid (p121.f90:112-123 for each element):
   !> sigma(6,nels) from the displacements x(neq) and DEEMAT(e, v).

@@ -28,7 +28,7 @@ __all__ = [
     "sg_to_abaqus", "base_fixed_rest", "top_pressure_loads", "face_flux_loads",
     "steering", "nodal_loads_to_rhs", "read_deck", "write_p121_deck",
     "splitmix64_uniform", "lognormal", "element_coords", "fortran_e", "write_ensi_vector",
-    "read_ensi_vector",
+    "read_ensi_vector", "write_p124_deck", "p124_box_rest",
 ]
 
 
@@ -467,15 +467,20 @@ def read_deck(basename: str, program: str = "p121") -> dict:
         keys = ["element", "meshgen", "partitioner", "nels", "nn", "nr", "nip",
                 "nod", "loaded_nodes", "fixed_freedoms", "kx", "ky", "kz", "tol",
                 "limit", "nres"]
+    elif program == "p124":
+        # READ_P124 (input.f90:3997-4169)
+        keys = ["element", "meshgen", "partitioner", "np_types", "nels", "nn", "nr", "nip",
+                "nod", "loaded_nodes", "fixed_freedoms", "val0", "dtim", "nstep", "npri",
+                "theta", "tol", "limit", "nres"]
     else:
         raise ValueError(program)
     for k, t in zip(keys, toks):
         out[k] = t
+    ints = ("meshgen", "partitioner", "np_types", "nels", "nn", "nr", "nip", "nod", "loaded_nodes",
+            "fixed_freedoms", "limit", "nres", "nstep", "npri")
     for k in keys[1:]:
         v = out[k]
-        out[k] = float(v) if ("." in v or "E" in v.upper()) and k not in (
-            "meshgen", "partitioner", "nels", "nn", "nr", "nip", "nod",
-            "loaded_nodes", "fixed_freedoms", "limit", "nres") else int(float(v))
+        out[k] = int(float(v)) if k in ints else float(v)
     nod, nels, nn = out["nod"], out["nels"], out["nn"]
     coords = np.zeros((nn, 3))
     g_num = np.zeros((nels, nod), dtype=np.int32)
@@ -493,6 +498,7 @@ def read_deck(basename: str, program: str = "p121") -> dict:
     i += 1
     arr = np.array(" ".join(lines[i:i + nels]).split(), dtype=np.int64).reshape(nels, nod + 5)
     g_num[:] = arr[:, 4:4 + nod]
+    out["etype"] = arr[:, 4 + nod].astype(np.int32)
     if out["meshgen"] == 2:
         g_num = abaqus_to_sg(g_num)
     out["g_num"] = g_num
@@ -509,12 +515,20 @@ def read_deck(basename: str, program: str = "p121") -> dict:
         a = a[: nl * (nodof + 1)].reshape(nl, nodof + 1)
         out["load_node"] = a[:, 0].astype(np.int32)
         out["load_val"] = a[:, 1:]
-    if program == "p123" and out.get("fixed_freedoms", 0) > 0:
+    if program in ("p123", "p124") and out.get("fixed_freedoms", 0) > 0:
         a = np.array(open(basename + ".fix").read().split(), dtype=np.float64)
         a = a.reshape(out["fixed_freedoms"], 3)
         out["fix_node"] = a[:, 0].astype(np.int32)
         out["fix_sense"] = a[:, 1].astype(np.int32)
         out["fix_val"] = a[:, 2]
+    if program == "p124":
+        # READ_MATERIAL (input.f90:3067-3102): keyword nmats nvals, a skipped
+        # line, then per type its number and nvals values
+        with open(basename + ".mat") as fh:
+            mlines = fh.read().splitlines()
+        nmats, nvals = int(mlines[0].split()[1]), int(mlines[0].split()[2])
+        vals = np.array(" ".join(mlines[2:]).split(), dtype=np.float64)[: nmats * (nvals + 1)]
+        out["prop"] = vals.reshape(nmats, nvals + 1)[:, 1:]
     return out
 
 
@@ -544,6 +558,62 @@ def write_p121_deck(basename: str, mesh: Mesh, rest, load_node, load_val,
         fh.write("1\n")
         fh.write(f"{mesh.nels:12d}{mesh.nn:12d}{len(rest):12d}{nip:5d}{mesh.nod:5d}{len(load_node):9d}\n")
         fh.write(f"{e:12.4E}{v:12.4E}{tol:12.4E}{limit:8d}\n")
+
+
+def p124_box_rest(mesh: Mesh) -> np.ndarray:
+    """The restraint of the p124 demo cube: temperature held (freedom 0) on
+    the faces x = max, y = max and z = 0."""
+    c = mesh.coords
+    on = (np.isclose(c[:, 0], c[:, 0].max()) | np.isclose(c[:, 1], c[:, 1].max())
+          | np.isclose(c[:, 2], 0.0))
+    nodes = np.nonzero(on)[0] + 1
+    return np.stack([nodes, np.zeros_like(nodes)], axis=1).astype(np.int32)
+
+
+def write_p124_deck(basename: str, mesh: Mesh, rest, prop, etype, val0: float, dtim: float, nstep: int,
+                    npri: int, theta: float, tol: float, limit: int, nres: int, nip: int = 8,
+                    abaqus: bool = True, fix=None, loads=None) -> None:
+    """A p124 deck in the formats READ_P124, READ_ELEMENTS, READ_G_COORD_PP,
+    READ_REST, READ_MATERIAL, READ_FIXED and READ_LOADS read (input.f90:
+    3997-4169, 1434-1583, 288-445, 2570-2632, 3067-3102, 2483-2564,
+    2350-2411).  fix: (node, sense, value) rows; loads: (node, value) rows."""
+    os.makedirs(os.path.dirname(os.path.abspath(basename)), exist_ok=True)
+    g = sg_to_abaqus(mesh.g_num) if abaqus else mesh.g_num
+    et = np.asarray(etype, dtype=np.int64)
+    with open(basename + ".d", "w") as fh:
+        fh.write("*THREE_DIMENSIONAL\n*NODES\n")
+        for i, (x, y, z) in enumerate(mesh.coords, 1):
+            fh.write(f"{i:12d}{x:14.6E}{y:14.6E}{z:14.6E}\n")
+        fh.write("*ELEMENTS\n")
+        for i, row in enumerate(g, 1):
+            fh.write(f"{i:12d} 3 {mesh.nod} 1 " + "".join(f"{n:12d}" for n in row) + f" {et[i - 1]}\n")
+    with open(basename + ".bnd", "w") as fh:
+        for r in rest:
+            fh.write(f"{r[0]:12d}" + "".join(f"{x:6d}" for x in r[1:]) + "\n")
+    pr = np.atleast_2d(np.asarray(prop, dtype=np.float64))
+    with open(basename + ".mat", "w") as fh:
+        fh.write(f"*MATERIAL {pr.shape[0]:4d} {pr.shape[1]:4d}\n<material>\n")
+        for i, row in enumerate(pr, 1):
+            fh.write(f"{i:2d}" + "".join(f"{x:12.4E}" for x in row) + "\n")
+    nfix = 0 if fix is None else len(fix)
+    nld = 0 if loads is None else len(loads)
+    if nfix:
+        with open(basename + ".fix", "w") as fh:
+            for n, sense, v in fix:
+                fh.write(f"{int(n):12d}{int(sense):6d}{v:16.8E}\n")
+    if nld:
+        with open(basename + ".lds", "w") as fh:
+            for n, v in loads:
+                fh.write(f"{int(n):12d}{v:16.8E}\n")
+    with open(basename + ".dat", "w") as fh:
+        fh.write("'hexahedron'\n")
+        fh.write("2\n" if abaqus else "1\n")
+        fh.write("1\n")
+        fh.write(f"{pr.shape[0]}\n")
+        fh.write(f"{mesh.nels:12d}{mesh.nn:12d}{len(rest):12d}{nip:5d}{mesh.nod:5d}{nld:9d}{nfix:9d}\n")
+        fh.write(f"{val0:12.4E}\n")
+        fh.write(f"{dtim:12.4E}{nstep:8d}{npri:8d}{theta:12.4E}\n")
+        fh.write(f"{tol:12.4E}{limit:8d}{nres:8d}\n")
 
 
 def fortran_e(value: float, w: int = 12, d: int = 5) -> str:

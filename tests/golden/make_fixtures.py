@@ -11,6 +11,12 @@ Writes (data only, no reference source):
   p121_demo_displ.npz   -- the nodal displacement field of
                            p121_demo.ensi.DISPL-000001 (x, y, z per node, as
                            printed with 4-5 significant digits).
+  p124_demo.json        -- control data of examples/5th_ed/p124/demo/p124_demo
+                           (.dat, .mat), a SHA-256 of the parsed deck arrays
+                           (g_num after ABAQUS2SG, coordinates, rest, element
+                           types) and the temperature history its .res prints;
+  p124_demo_ttr.npz     -- the nodal temperatures of p124_demo.ensi.NDTTR-*
+                           at steps 10, 80 and 150 (as printed, 5 digits).
 The xx3_tiny/ directory is a verbatim copy of examples/dev/xx3/demo/xx3-tiny.*
 (the deck and its .res/.dis results).
 """
@@ -38,6 +44,53 @@ def deck_digest(d):
         h.update(str(a.dtype).encode())
         h.update(a.tobytes())
     return h.hexdigest()
+
+
+REF124 = "/root/reference/parafem/examples/5th_ed/p124/demo/p124_demo"
+P124_STEPS = (10, 80, 150)
+
+
+def p124_digest(d):
+    h = hashlib.sha256()
+    for k in ("g_num", "coords", "rest", "etype"):
+        a = np.ascontiguousarray(d[k])
+        if a.dtype.kind == "f":
+            a = a + 0.0
+        h.update(k.encode())
+        h.update(str(a.dtype).encode())
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def p124_main():
+    d = M.read_deck(REF124, "p124")
+    res = open(REF124 + ".res").read().splitlines()
+    hist = []
+    for line in res:
+        t = line.split()
+        if len(t) in (2, 3) and t[0][0] in "0123456789" and "E" in t[0]:
+            hist.append([float(t[0]), float(t[1]), int(t[2]) if len(t) == 3 else None])
+    fields = {}
+    for step in P124_STEPS:
+        lines = open(f"{REF124}.ensi.NDTTR-{step:06d}").read().split("\n")[4:]
+        fields[f"step{step}"] = np.array([float(x) for x in lines if x.strip()], dtype=np.float32)
+    keys = ("element", "meshgen", "partitioner", "np_types", "nels", "nn", "nr", "nip", "nod",
+            "loaded_nodes", "fixed_freedoms", "val0", "dtim", "nstep", "npri", "theta", "tol", "limit", "nres")
+    out = {
+        "source": "examples/5th_ed/p124/demo/p124_demo.{dat,d,bnd,mat,res,ensi.NDTTR-*}",
+        "control": {k: d[k] for k in keys},
+        "prop": d["prop"].tolist(),
+        "res_processes": int(res[0].split()[4]),
+        "res_neq": int(res[1].split()[7]),
+        "history": hist,
+        "mg": {"nxe": 25, "nye": 25, "nze": 25, "aa": 0.04, "bb": 0.04, "cc": 0.04},
+        "deck_sha256": p124_digest(d),
+        "ttr_steps": list(P124_STEPS),
+    }
+    with open(os.path.join(HERE, "p124_demo.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    np.savez_compressed(os.path.join(HERE, "p124_demo_ttr.npz"), **fields)
+    print(json.dumps({k: v for k, v in out.items() if k != "history"}, indent=1))
 
 
 def main():
@@ -71,3 +124,4 @@ def main():
 
 if __name__ == "__main__":
     main()
+    p124_main()

@@ -22,6 +22,7 @@ from parafem_amd import mesh as M, problems as P
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "parafem_amd", "bin", "pf_p121")
+EXE124 = os.path.join(ROOT, "parafem_amd", "bin", "pf_p124")
 
 needs_exe = pytest.mark.skipif(not os.path.exists(EXE), reason="pf_p121 not built (no amdflang)")
 
@@ -127,3 +128,46 @@ def test_fortran_generated_abaqus_deck_and_mixed(tmp_path):
     xo, it, _ = O.pcg(g_g, full, r, O.diag(g_g, Ko, neq), d["tol"], d["limit"], dot_order=O.DOT_DEVICE)
     assert ik == it and np.array_equal(xk, xo)
     assert np.abs(xk - xa).max() <= 1e-6 * np.abs(xa).max()
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_fortran_p124_demo(tmp_path, golden):
+    """pf_p124 on the p124 demo deck (regenerated exactly from its
+    generator): the temperature history prints as p124_demo.res prints it
+    (same digits, same iteration counts); the last temperatures are the
+    oracle's bit for bit (device dot order) and every NDTTR file equals the
+    oracle's nodal averages line for line; the fields agree with the
+    demo's printed ones."""
+    sys.path.insert(0, os.path.dirname(__file__))
+    import oracle as O
+    from test_p124 import _demo_deck, oracle_p124
+    meta, base = _demo_deck(golden, tmp_path)
+    say("pf_p124 demo deck")
+    r = subprocess.run([EXE124, os.path.basename(base)], cwd=os.path.dirname(base), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, f"pf_p124 failed ({r.returncode}):\n{r.stdout}\n{r.stderr}"
+    res = open(base + ".pf.res").read().splitlines()
+    ref = open(os.path.join(golden, "p124_demo.json")).read()
+    hist_lines = [ln for ln in res if re.match(r"^\s+\d\.\d{4}E[+-]\d\d\s+\S+", ln)]
+    pub = __import__("json").loads(ref)["history"]
+    assert len(hist_lines) == len(pub)
+    for ln, (t, v, it) in zip(hist_lines, pub):
+        expect = M.fortran_e(t, 12, 4) + M.fortran_e(v, 12, 4) + (f"{it:10d}" if it is not None else "")
+        assert ln == expect, (ln, expect)
+    assert re.search(r"There are\s+17576 nodes\s+1951 restrained and\s+15625 equations", res[1])
+    d = M.read_deck(base, "p124")
+    keep = tuple(__import__("json").loads(ref)["ttr_steps"])
+    hist, fields, xo, (nf, neq, g_g) = oracle_p124(d, O.DOT_DEVICE, keep)
+    x = np.fromfile(base + ".pf.x", dtype=np.float64)
+    assert np.array_equal(x, xo)
+    ttr = np.load(os.path.join(golden, "p124_demo_ttr.npz"))
+    for step in keep:
+        eld = O.gather(g_g, fields[step]).reshape(d["nels"], 8, 1)
+        nodal = O.scatter_nodes(d["g_num"], eld, d["nn"], 1)[:, 0]
+        lines = open(f"{base}.pf.ensi.NDTTR-{step:06d}").read().split("\n")
+        assert lines[:4] == ["Alya Ensight Gold --- Scalar per-node variable file", "part", "    1", "coordinates"]
+        assert lines[4:-1] == [M.fortran_e(v, 12, 5) for v in nodal], step
+        refv = ttr[f"step{step}"].astype(np.float64)
+        unit = 10.0 ** (np.floor(np.log10(np.maximum(np.abs(refv), 1e-30))) - 4)
+        assert np.all(np.abs(nodal - refv) <= unit + 1e-4 * np.abs(refv).max()), step

@@ -660,6 +660,29 @@ def test_steering_rejects_bad_maps(ctx):
     assert res.converged
 
 
+def test_transient_matrices_and_products_bitexact(ctx):
+    """p124's element loop on the device (two element types on a
+    distorted mesh): storka and storkb equal the oracle bit for bit, and the
+    raw products with each equal the oracle's gather/product/scatter."""
+    m = M.box_hex8(4, 3, 3, 0.1, 0.2, 0.15)
+    rng = np.random.default_rng(17)
+    m.coords = m.coords + rng.uniform(-0.01, 0.01, m.coords.shape)
+    rest = M.p124_box_rest(m)
+    nf, neq = M.steering(m.nn, 1, rest)
+    g_g = M.g_g_from(nf, m.g_num)
+    prop = np.array([[1.0, 2.0, 0.5, 1.5, 0.8], [3.0, 0.7, 1.1, 0.9, 2.0]])
+    etype = (np.arange(m.nels) % 2 + 1).astype(np.int32)
+    say(f"transient matrices nels={m.nels} neq={neq}")
+    ctx.mesh_set(m.g_coord())
+    ctx.steer_set(g_g, neq)
+    ctx.assemble_transient(0.5, 0.01, prop, etype)
+    KA, KB = O.assemble_transient(m.g_coord(), prop, etype, 0.5, 0.01)
+    assert np.array_equal(ctx.storkm_get(), KA)
+    x = rng.normal(size=neq)
+    assert np.array_equal(ctx.apply_raw(0, x), O.apply(g_g, KA, x))
+    assert np.array_equal(ctx.apply_raw(1, x), O.apply(g_g, KB, x))
+
+
 def test_context_reuse_and_trim():
     """A context solving problems one after another reuses its freed device
     blocks (same result as a fresh context, bit for bit); pf_trim gives the

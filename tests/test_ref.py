@@ -47,6 +47,19 @@ def test_shape_der_bitexact(nod):
             assert np.array_equal(der.T, O.shape_der(P_, i, nod)), (nod, i)
 
 
+def test_shape_fun_hex8_bitexact():
+    """SHAPE_FUN of the 8-node brick (p124's capacity matrix) at the Gauss
+    points and at random points."""
+    pts, _ = ref_sample()
+    extra = np.random.default_rng(5).uniform(-1, 1, size=(6, 3))
+    for P_ in (pts, extra):
+        colmaj = np.ascontiguousarray(P_.T)
+        for i in range(1, P_.shape[0] + 1):
+            fun = np.zeros(8)
+            R.ref_shape_fun(8, P_.shape[0], d_p(colmaj), i, d_p(fun))
+            assert np.array_equal(fun, O.shape_fun_hex8(P_, i)), i
+
+
 @pytest.mark.parametrize("e,v", [(100.0, 0.3), (1.0, 0.0), (7.5, 0.45), (210e3, 0.2)])
 def test_deemat_bitexact(e, v):
     dee = np.zeros((6, 6))
