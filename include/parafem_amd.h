@@ -263,6 +263,25 @@ int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);
  * Host vectors, neq. */
 int pf_apply_raw(pf_ctx *ctx, int which, const double *x_host, double *y_host);
 
+/* ------------------------------------------------------ p122 plasticity */
+/* p122's viscoplastic Mohr-Coulomb stress update and body loads
+ * (src/programs/5th_ed/p122/p122.f90:164-199) on 8-node bricks (ntot 24),
+ * one element per thread: from the displacement increment x_dev (device,
+ * neq; p122's loads_pp after the solve), per Gauss point eps = B eld - evpt,
+ * stress = D eps + tensor, INVAR / MOCOUF / MOCOUQ / FORMM
+ * (src/modules/shared/new_library.f90:1813-1900, 2364-2460, 85-198);
+ * updates evpt_dev and (final_pass != 0, p122's plastic_converged .OR.
+ * plasiters == plasits) tensor_dev, both (6, 8, nels) device arrays, and
+ * writes each element's body loads into utemp_dev (24, nels) for SCATTER.
+ * e, v: DEEMAT; phi, c, psi in degrees / stress units; dt the viscoplastic
+ * time step.  Trigonometric functions are the device library's. */
+int pf_plastic_update(pf_ctx *ctx, const double *x_dev, double *evpt_dev, double *tensor_dev, double *utemp_dev,
+                      double e, double v, double phi, double c, double psi, double dt, int final_pass);
+/* p122's CG statement for the fixed freedoms set by pf_fixed_set
+ * (p122.f90:141-146): u(j) = p(j)*store(j) (mode 1, first plastic
+ * iteration) or u(j) = 0 (mode 0, later ones).  Device vectors, neq. */
+int pf_fixed_rows(pf_ctx *ctx, double *u_dev, const double *p_dev, int mode);
+
 /* ------------------------------------------------------- post-solve */
 /* Stresses at the centroid of every element, p121.f90:112-123 (which
  * prints element 1's): eld = GATHER(xnew), SHAPE_DER at points = 0,

@@ -22,6 +22,10 @@ Pinning status (how each function is checked in ``tests/test_oracle.py``):
   checon -- exact binary cases;
   dot_device, sum_p -- math.fsum within rounding; total load of the fixtures;
   centroid_stress -- element 1 of the p121 demo against its results file;
+  plastic_update, mocouf, mocouq, formm6 -- closed forms (Tresca limit
+      of MOCOUF at phi = 0, FORMM's sum m1+m2+m3 acting on the stress,
+      MOCOUQ against finite differences of the potential), an elastic
+      state that stays elastic, and the p122 demo's history (.res);
   shape_fun_hex8 -- bit for bit against the reference's SHAPE_FUN in
       oracle/_ref; partition of unity and the nodal delta property;
   transient_km, assemble_transient -- kc equals the p123 element (another
@@ -70,6 +74,11 @@ def lib():
         L.or_sample_hex8.argtypes = [d, d]
         L.or_scatter_nodes.argtypes = [C.c_int, C.c_int, C.c_int, i, C.c_int, d, d, C.c_int]
         L.or_shape_fun_hex8.argtypes = [d, d, C.c_int, C.c_int]
+        L.or_plastic_update.argtypes = [C.c_int, d, i, d, d, d, d] + [C.c_double] * 6 + [C.c_int]
+        L.or_mocouf.restype = C.c_double
+        L.or_mocouf.argtypes = [C.c_double] * 5
+        L.or_mocouq.argtypes = [C.c_double, C.c_double, C.c_double, d]
+        L.or_formm6.argtypes = [d, d]
         L.or_transient_km.argtypes = [d, d, C.c_double, C.c_double, C.c_double, C.c_double, d, d]
         L.or_assemble_transient.argtypes = [C.c_int, d, d, i, C.c_double, C.c_double, d, d, C.c_int]
         L.or_deemat6.argtypes = [d, C.c_double, C.c_double]
@@ -225,6 +234,35 @@ def assemble_transient(g_coord, prop, etype, theta, dtim, nthreads=0):
     lib().or_assemble_transient(nels, _d(g), _d(pr), _i(et), float(theta), float(dtim), _d(ka), _d(kb),
                                 int(nthreads))
     return ka, kb
+
+
+def plastic_update(g_coord, g_g, x, evpt, tensor, e, v, phi, c, psi, dt, final):
+    """p122's stress update (p122.f90:164-199): updates evpt and tensor
+    ((nels, 8, 6) arrays, in place) and returns bload (nels, 24)."""
+    g = _f64(g_coord)
+    gg = _i32(g_g)
+    nels = gg.shape[0]
+    assert evpt.dtype == np.float64 and evpt.flags.c_contiguous and tensor.flags.c_contiguous
+    out = np.zeros((nels, 24))
+    lib().or_plastic_update(nels, _d(g), _i(gg), _d(_f64(x)), _d(evpt), _d(tensor), _d(out), float(e), float(v),
+                            float(phi), float(c), float(psi), float(dt), int(bool(final)))
+    return out
+
+
+def mocouf(phi, c, sigm, dsbar, theta):
+    return lib().or_mocouf(float(phi), float(c), float(sigm), float(dsbar), float(theta))
+
+
+def mocouq(psi, dsbar, theta):
+    dq = np.zeros(3)
+    lib().or_mocouq(float(psi), float(dsbar), float(theta), _d(dq))
+    return dq
+
+
+def formm6(stress):
+    m = np.zeros(108)
+    lib().or_formm6(_d(_f64(stress)), _d(m))
+    return [m[36 * k: 36 * (k + 1)].reshape(6, 6).T.copy() for k in range(3)]
 
 
 def laplace_kc(coord, kx=1.0, ky=1.0, kz=1.0):

@@ -366,6 +366,161 @@ void or_transient_km(const double *coord, const double *kay3, double rho, double
     }
 }
 
+/* INVAR for nst = 6 (new_library.f90:1813-1900). */
+static void or_invar6(const double *s, double *sigm, double *dsbar, double *theta)
+{
+    double sq3 = sqrt(3.0);
+    double s1 = s[0], s2 = s[1], s3 = s[2], s4 = s[3], s5 = s[4], s6 = s[5];
+    *sigm = (s1 + s2 + s3) / 3.0;
+    double d2 = ((s1 - s2) * (s1 - s2) + (s2 - s3) * (s2 - s3) + (s3 - s1) * (s3 - s1)) / 6.0 + s4 * s4 + s5 * s5 +
+                s6 * s6;
+    double ds1 = s1 - *sigm, ds2 = s2 - *sigm, ds3 = s3 - *sigm;
+    double d3 = ds1 * ds2 * ds3 - ds1 * s5 * s5 - ds2 * s6 * s6 - ds3 * s4 * s4 + 2.0 * s4 * s5 * s6;
+    *dsbar = sq3 * sqrt(d2);
+    if (*dsbar < 1.e-10) {
+        *theta = 0.0;
+    } else {
+        double r = sqrt(d2);
+        double sine = -(3.0 * sq3 * d3 / (2.0 * (r * r * r)));
+        if (sine > 1.0) sine = 1.0;
+        if (sine < -1.0) sine = -1.0;
+        *theta = asin(sine) / 3.0;
+    }
+}
+
+/* MOCOUF (new_library.f90:2364-2382): the Mohr-Coulomb yield function. */
+double or_mocouf(double phi, double c, double sigm, double dsbar, double theta)
+{
+    double phir = phi * 4.0 * atan(1.0) / 180.0;
+    double snph = sin(phir), csph = cos(phir), csth = cos(theta), snth = sin(theta);
+    return snph * sigm + dsbar * (csth / sqrt(3.0) - snth * snph / 3.0) - c * csph;
+}
+
+/* MOCOUQ (new_library.f90:2423-2460): derivatives of the plastic potential. */
+void or_mocouq(double psi, double dsbar, double theta, double *dq)
+{
+    double psir = psi * 4.0 * atan(1.0) / 180.0;
+    double snth = sin(theta), snps = sin(psir), sq3 = sqrt(3.0);
+    dq[0] = snps;
+    if (fabs(snth) > 0.49) {
+        double c1 = 1.0;
+        if (snth < 0.0) c1 = -1.0;
+        dq[1] = (sq3 * 0.5 - c1 * snps * 0.5 / sq3) * sq3 * 0.5 / dsbar;
+        dq[2] = 0.0;
+    } else {
+        double csth = cos(theta), cs3th = cos(3.0 * theta), tn3th = tan(3.0 * theta);
+        double tnth = snth / csth;
+        dq[1] = sq3 * csth / dsbar * ((1.0 + tnth * tn3th) + snps * (tn3th - tnth) / sq3) * 0.5;
+        dq[2] = 0.5 * 3.0 * (sq3 * snth + snps * csth) / (cs3th * dsbar * dsbar);
+    }
+}
+
+/* FORMM for nst = 6 (new_library.f90:85-198); m (3,6,6): m1, m2, m3 with
+ * entry (i,j) at i + 6 j. */
+void or_formm6(const double *s, double *m)
+{
+    double *m1 = m, *m2 = m + 36, *m3 = m + 72;
+    double sx = s[0], sy = s[1], sz = s[2], txy = s[3], tyz = s[4], tzx = s[5];
+    double sigm = (sx + sy + sz) / 3.0;
+    double dx = sx - sigm, dy = sy - sigm, dz = sz - sigm;
+#define A(mm, i, j) mm[((i) - 1) + 6 * ((j) - 1)]
+    for (int k = 0; k < 108; ++k) m[k] = 0.0;
+    for (int i = 1; i <= 3; ++i)
+        for (int j = 1; j <= 3; ++j) A(m1, i, j) = 1.0 / (3.0 * sigm);
+    for (int i = 1; i <= 3; ++i) {
+        A(m2, i, i) = 2.0;
+        A(m2, i + 3, i + 3) = 6.0;
+    }
+    A(m2, 1, 2) = -1.0; A(m2, 1, 3) = -1.0; A(m2, 2, 3) = -1.0;
+    A(m3, 1, 1) = dx; A(m3, 1, 2) = dz; A(m3, 1, 3) = dy; A(m3, 1, 4) = txy; A(m3, 1, 5) = -2.0 * tyz;
+    A(m3, 1, 6) = tzx;
+    A(m3, 2, 2) = dy; A(m3, 2, 3) = dx; A(m3, 2, 4) = txy; A(m3, 2, 5) = tyz; A(m3, 2, 6) = -2.0 * tzx;
+    A(m3, 3, 3) = dz; A(m3, 3, 4) = -2.0 * txy; A(m3, 3, 5) = tyz; A(m3, 3, 6) = tzx;
+    A(m3, 4, 4) = -3.0 * dz; A(m3, 4, 5) = 3.0 * tzx; A(m3, 4, 6) = 3.0 * tyz;
+    A(m3, 5, 5) = -3.0 * dx; A(m3, 5, 6) = 3.0 * txy;
+    A(m3, 6, 6) = -3.0 * dy;
+    for (int i = 1; i <= 6; ++i)
+        for (int j = i + 1; j <= 6; ++j) {
+            A(m1, j, i) = A(m1, i, j);
+            A(m2, j, i) = A(m2, i, j);
+            A(m3, j, i) = A(m3, i, j);
+        }
+    for (int k = 0; k < 108; ++k) m[k] = m[k] / 3.0;
+#undef A
+}
+
+/* p122's stress update over all elements (p122.f90:164-199), 8-node bricks:
+ * x (neq) the displacement increment, evpt and tensor (6,8,nels), bload
+ * (24,nels) out; final = plastic_converged .OR. plasiters == plasits. */
+void or_plastic_update(int nels, const double *g_coord, const int *g_g, const double *x, double *evpt,
+                       double *tensor, double *bload, double e, double v, double phi, double c, double psi,
+                       double dt, int final_pass)
+{
+    double points[24], weights[8], der[24], deriv[24], dee[36], bee[144];
+    or_sample_hex8(points, weights);
+    or_deemat6(dee, e, v);
+    for (int iel = 0; iel < nels; ++iel) {
+        double eld[24], bl[24];
+        for (int i = 0; i < 24; ++i) {
+            int g = g_g[(size_t)iel * 24 + i];
+            eld[i] = g ? x[g - 1] : 0.0;
+            bl[i] = 0.0;
+        }
+        for (int ip = 1; ip <= 8; ++ip) {
+            or_shape_der(der, points, 8, ip, 8);
+            double det = jac_deriv(der, g_coord + (size_t)iel * 24, 8, deriv);
+            or_beemat6(bee, deriv, 8);
+            double eps[6], sigma[6], stress[6], devp[6] = {0, 0, 0, 0, 0, 0};
+            double *ev = evpt + ((size_t)iel * 8 + ip - 1) * 6, *te = tensor + ((size_t)iel * 8 + ip - 1) * 6;
+            for (int r = 0; r < 6; ++r) {
+                double s = 0.0;
+                for (int k = 0; k < 24; ++k) s = s + bee[IDX2(r, k, 6)] * eld[k];
+                eps[r] = s;
+            }
+            for (int r = 0; r < 6; ++r) eps[r] = eps[r] - ev[r];
+            for (int r = 0; r < 6; ++r) {
+                double s = 0.0;
+                for (int t = 0; t < 6; ++t) s = s + dee[IDX2(r, t, 6)] * eps[t];
+                sigma[r] = s;
+            }
+            for (int r = 0; r < 6; ++r) stress[r] = sigma[r] + te[r];
+            double sigm, dsbar, theta;
+            or_invar6(stress, &sigm, &dsbar, &theta);
+            double f = or_mocouf(phi, c, sigm, dsbar, theta);
+            if (final_pass) {
+                for (int r = 0; r < 6; ++r) devp[r] = stress[r];
+            } else if (f >= 0.0) {
+                double dq[3], m[108], evp[6];
+                or_mocouq(psi, dsbar, theta, dq);
+                or_formm6(stress, m);
+                for (int a = 0; a < 6; ++a) {
+                    double s = 0.0;
+                    for (int b = 0; b < 6; ++b) {
+                        double flow = f * (m[a + 6 * b] * dq[0] + m[36 + a + 6 * b] * dq[1] + m[72 + a + 6 * b] * dq[2]);
+                        s = s + flow * stress[b];
+                    }
+                    evp[a] = s * dt;
+                }
+                for (int a = 0; a < 6; ++a) ev[a] = ev[a] + evp[a];
+                for (int a = 0; a < 6; ++a) {
+                    double s = 0.0;
+                    for (int t = 0; t < 6; ++t) s = s + dee[IDX2(a, t, 6)] * evp[t];
+                    devp[a] = s;
+                }
+            }
+            if (f >= 0.0)
+                for (int k = 0; k < 24; ++k) {
+                    double s = 0.0;
+                    for (int r = 0; r < 6; ++r) s = s + bee[IDX2(r, k, 6)] * devp[r];
+                    bl[k] = bl[k] + s * det * weights[ip - 1];
+                }
+            if (final_pass)
+                for (int r = 0; r < 6; ++r) te[r] = stress[r];
+        }
+        for (int i = 0; i < 24; ++i) bload[(size_t)iel * 24 + i] = bl[i];
+    }
+}
+
 static int omp_threads(int nthreads)
 {
 #ifdef _OPENMP

@@ -133,6 +133,8 @@ def lib():
         "pf_steer_build": (i32, [vp, i32, i32, i32, pi, i32, i32, pi, pi, pi]),
         "pf_assemble_transient": (i32, [vp, dbl, dbl, i32, pd, pi, i32]),
         "pf_apply_raw": (i32, [vp, i32, pd, pd]),
+        "pf_plastic_update": (i32, [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, dbl, dbl, i32]),
+        "pf_fixed_rows": (i32, [vp, vp, vp, i32]),
         "pf_steer_get": (i32, [vp, pi]),
         "pf_tune_set": (i32, [vp, C.POINTER(Tune)]),
         "pf_tune_get": (i32, [vp, C.POINTER(Tune)]),
@@ -319,6 +321,17 @@ class Context:
         assert pr.ndim == 2 and pr.shape[1] == 5 and et.size == self.nels
         _check(lib().pf_assemble_transient(self.h, float(theta), float(dtim), pr.shape[0], _pd(pr), _pi(et),
                                            STORE[store]), "pf_assemble_transient")
+
+    def plastic_update(self, x: "DeviceArray", evpt: "DeviceArray", tensor: "DeviceArray", utemp: "DeviceArray",
+                       e: float, v: float, phi: float, c: float, psi: float, dt: float, final: bool):
+        """p122's stress update on device arrays (x: neq; evpt, tensor:
+        6*8*nels; utemp: 24*nels out)."""
+        _check(lib().pf_plastic_update(self.h, x.ptr, evpt.ptr, tensor.ptr, utemp.ptr, float(e), float(v),
+                                       float(phi), float(c), float(psi), float(dt), int(bool(final))),
+               "pf_plastic_update")
+
+    def fixed_rows(self, u: "DeviceArray", p: "DeviceArray | None", mode: int):
+        _check(lib().pf_fixed_rows(self.h, u.ptr, None if p is None else p.ptr, int(mode)), "pf_fixed_rows")
 
     def apply_raw(self, which: int, x) -> np.ndarray:
         """storka (which 0) or storkb (which 1) times x, no fixed rows."""

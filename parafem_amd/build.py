@@ -21,6 +21,7 @@ LIB = os.path.join(HERE, "libparafem_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 FSRC = os.path.join(HERE, "fortran")
 P121 = os.path.join(HERE, "bin", "pf_p121")
+P122 = os.path.join(HERE, "bin", "pf_p122")
 P124 = os.path.join(HERE, "bin", "pf_p124")
 FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 
@@ -47,7 +48,7 @@ def _compile(src: str) -> str:
 
 
 def build_fortran(verbose: bool = False) -> str | None:
-    """Compile the Fortran binding module and the p121 and p124 drivers against the
+    """Compile the Fortran binding module and the p121, p122 and p124 drivers against the
     library (rpath to this directory).  Skipped, returning None, when no
     Fortran compiler is installed."""
     if not os.path.exists(FLANG):
@@ -55,7 +56,7 @@ def build_fortran(verbose: bool = False) -> str | None:
     moddir = os.path.join(OBJDIR, "fmod")
     os.makedirs(moddir, exist_ok=True)
     os.makedirs(os.path.dirname(P121), exist_ok=True)
-    for exe, src in ((P121, "pf_p121.f90"), (P124, "pf_p124.f90")):
+    for exe, src in ((P121, "pf_p121.f90"), (P122, "pf_p122.f90"), (P124, "pf_p124.f90")):
         tmp = exe + ".tmp"
         cmd = [FLANG, "-O2", "-J", moddir, os.path.join(FSRC, "pf_gpu.f90"),
                os.path.join(FSRC, src), "-L", HERE, "-lparafem_amd",

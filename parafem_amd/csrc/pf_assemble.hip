@@ -19,57 +19,11 @@
 // a sequential evaluation of p121's MATMUL(MATMUL(TRANSPOSE(bee),dee),bee)
 // *det*weights(i) gives, bit for bit.
 #include "pf_internal.h"
+#include "pf_element.h"
 
 namespace {
 
 constexpr int WAVES = 4;   // elements per block
-
-// SHAPE_DER for hex8 (nod = 8) and hex20 (nod = 20) at (xi, eta, zeta):
-// derivative k (0..2) of shape function l (0-based).
-template <int NOD>
-__device__ __forceinline__ void shape_der3(double xi, double eta, double zeta, int l, double out[3])
-{
-    if constexpr (NOD == 8) {
-        const double etam = 1.0 - eta, xim = 1.0 - xi, zetam = 1.0 - zeta;
-        const double etap = eta + 1.0, xip = xi + 1.0, zetap = zeta + 1.0;
-        // Table of SHAPE_DER case(8): der(k,l) = s * f1 * f2.
-        switch (l) {
-        case 0: out[0] = -0.125 * etam * zetam; out[1] = -0.125 * xim * zetam; out[2] = -0.125 * xim * etam; break;
-        case 1: out[0] = -0.125 * etam * zetap; out[1] = -0.125 * xim * zetap; out[2] = 0.125 * xim * etam; break;
-        case 2: out[0] = 0.125 * etam * zetap;  out[1] = -0.125 * xip * zetap; out[2] = 0.125 * xip * etam; break;
-        case 3: out[0] = 0.125 * etam * zetam;  out[1] = -0.125 * xip * zetam; out[2] = -0.125 * xip * etam; break;
-        case 4: out[0] = -0.125 * etap * zetam; out[1] = 0.125 * xim * zetam;  out[2] = -0.125 * xim * etap; break;
-        case 5: out[0] = -0.125 * etap * zetap; out[1] = 0.125 * xim * zetap;  out[2] = 0.125 * xim * etap; break;
-        case 6: out[0] = 0.125 * etap * zetap;  out[1] = 0.125 * xip * zetap;  out[2] = 0.125 * xip * etap; break;
-        default: out[0] = 0.125 * etap * zetam; out[1] = 0.125 * xip * zetam; out[2] = -0.125 * xip * etap; break;
-        }
-    } else {
-        // Node sign tables of SHAPE_DER case(20).
-        constexpr int xii[20] = {-1, -1, -1, 0, 1, 1, 1, 0, -1, -1, 1, 1, -1, -1, -1, 0, 1, 1, 1, 0};
-        constexpr int etai[20] = {-1, -1, -1, -1, -1, -1, -1, -1, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1};
-        constexpr int zetai[20] = {-1, 0, 1, 1, 1, 0, -1, -1, -1, 1, 1, -1, -1, 0, 1, 1, 1, 0, -1, -1};
-        const double sx = xii[l], sy = etai[l], sz = zetai[l];
-        const double xi0 = xi * sx, eta0 = eta * sy, zeta0 = zeta * sz;
-        const int n = l + 1;
-        if (n == 4 || n == 8 || n == 16 || n == 20) {
-            out[0] = -.5 * xi * (1. + eta0) * (1. + zeta0);
-            out[1] = .25 * sy * (1. - xi * xi) * (1. + zeta0);
-            out[2] = .25 * sz * (1. - xi * xi) * (1. + eta0);
-        } else if (n >= 9 && n <= 12) {
-            out[0] = .25 * sx * (1. - eta * eta) * (1. + zeta0);
-            out[1] = -.5 * eta * (1. + xi0) * (1. + zeta0);
-            out[2] = .25 * sz * (1. + xi0) * (1. - eta * eta);
-        } else if (n == 2 || n == 6 || n == 14 || n == 18) {
-            out[0] = .25 * sx * (1. + eta0) * (1. - zeta * zeta);
-            out[1] = .25 * sy * (1. + xi0) * (1. - zeta * zeta);
-            out[2] = -.5 * zeta * (1. + xi0) * (1. + eta0);
-        } else {
-            out[0] = .125 * sx * (1. + eta0) * (1. + zeta0) * (2. * xi0 + eta0 + zeta0 - 1.);
-            out[1] = .125 * sy * (1. + xi0) * (1. + zeta0) * (xi0 + 2. * eta0 + zeta0 - 1.);
-            out[2] = .125 * sz * (1. + xi0) * (1. + eta0) * (xi0 + eta0 + 2. * zeta0 - 1.);
-        }
-    }
-}
 
 // Per-wave LDS scratch for one element.
 template <int NOD>

@@ -58,7 +58,7 @@ MODULE pf_gpu
   PUBLIC :: dot_product_p, sum_p, checon_par
   PUBLIC :: vec_fill, vec_waxpy, vec_mul, vec_recip
   PUBLIC :: pf_stress, calc_nodes_pp, scatter_nodes, dismsh_ensi_p
-  PUBLIC :: pf_assemble_transient, pf_apply_raw
+  PUBLIC :: pf_assemble_transient, pf_apply_raw, pf_plastic_update, pf_fixed_rows
 
   INTERFACE
     INTEGER(c_int) FUNCTION c_create(device, c) BIND(C, name="pf_create")
@@ -242,6 +242,18 @@ MODULE pf_gpu
       INTEGER(c_int), VALUE :: np_types, store
       REAL(c_double), INTENT(IN) :: prop(*)
       INTEGER(c_int), INTENT(IN) :: etype(*)
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_plastic_update(c, x, evpt, tensor, utemp, e, v, phi, coh, psi, dt, final) &
+        BIND(C, name="pf_plastic_update")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: c, x, evpt, tensor, utemp
+      REAL(c_double), VALUE :: e, v, phi, coh, psi, dt
+      INTEGER(c_int), VALUE :: final
+    END FUNCTION
+    INTEGER(c_int) FUNCTION c_fixed_rows(c, u, p, mode) BIND(C, name="pf_fixed_rows")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: c, u, p
+      INTEGER(c_int), VALUE :: mode
     END FUNCTION
     INTEGER(c_int) FUNCTION c_apply_raw(c, which, x, y) BIND(C, name="pf_apply_raw")
       IMPORT :: c_int, c_ptr, c_double
@@ -542,6 +554,34 @@ CONTAINS
     REAL(iwp), INTENT(OUT) :: u(:)
     CALL pf_check(c_apply_raw(ctx, INT(which, c_int), x, u), 'pf_apply_raw')
   END SUBROUTINE pf_apply_raw
+
+  !> p122's stress update and body loads (p122.f90:164-199) on the device:
+  !> from loads_pp (the displacement increment), evpt_pp and tensor_pp
+  !> (nst*nip*nels_pp) are updated and each element's body loads written to
+  !> utemp_pp for SCATTER.  final: plastic_converged .OR. plasiters==plasits.
+  SUBROUTINE pf_plastic_update(loads_pp, evpt_pp, tensor_pp, utemp_pp, e, v, phi, c, psi, dt, final)
+    TYPE(pf_dvec), INTENT(IN) :: loads_pp
+    TYPE(pf_dvec), INTENT(INOUT) :: evpt_pp, tensor_pp, utemp_pp
+    REAL(iwp), INTENT(IN) :: e, v, phi, c, psi, dt
+    LOGICAL, INTENT(IN) :: final
+    INTEGER(c_int) :: f
+    f = 0
+    IF (final) f = 1
+    CALL pf_check(c_plastic_update(ctx, loads_pp%ptr, evpt_pp%ptr, tensor_pp%ptr, utemp_pp%ptr, e, v, phi, c, &
+                                   psi, dt, f), 'pf_plastic_update')
+  END SUBROUTINE pf_plastic_update
+
+  !> The fixed-freedom rows of p122's CG product (p122.f90:141-146):
+  !> u(j) = p(j)*store (first plastic iteration) or 0.
+  SUBROUTINE pf_fixed_rows(u_pp, p_pp, first)
+    TYPE(pf_dvec), INTENT(INOUT) :: u_pp
+    TYPE(pf_dvec), INTENT(IN) :: p_pp
+    LOGICAL, INTENT(IN) :: first
+    INTEGER(c_int) :: mode
+    mode = 0
+    IF (first) mode = 1
+    CALL pf_check(c_fixed_rows(ctx, u_pp%ptr, p_pp%ptr, mode), 'pf_fixed_rows')
+  END SUBROUTINE pf_fixed_rows
 
   !> Stresses at every element's centroid (p121.f90:112-123 for each element):
   !> sigma(6,nels) from the displacements x(neq) and DEEMAT(e, v).

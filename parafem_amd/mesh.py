@@ -467,6 +467,10 @@ def read_deck(basename: str, program: str = "p121") -> dict:
         keys = ["element", "meshgen", "partitioner", "nels", "nn", "nr", "nip",
                 "nod", "loaded_nodes", "fixed_freedoms", "kx", "ky", "kz", "tol",
                 "limit", "nres"]
+    elif program == "p122":
+        # READ_P122 (input.f90:3402-3582), then READ_QINC's incs values
+        keys = ["element", "meshgen", "partitioner", "nels", "nn", "nr", "nip", "nod", "fixed_freedoms",
+                "loaded_nodes", "phi", "c", "psi", "e", "v", "incs", "plasits", "cjits", "plastol", "cjtol"]
     elif program == "p124":
         # READ_P124 (input.f90:3997-4169)
         keys = ["element", "meshgen", "partitioner", "np_types", "nels", "nn", "nr", "nip",
@@ -477,10 +481,12 @@ def read_deck(basename: str, program: str = "p121") -> dict:
     for k, t in zip(keys, toks):
         out[k] = t
     ints = ("meshgen", "partitioner", "np_types", "nels", "nn", "nr", "nip", "nod", "loaded_nodes",
-            "fixed_freedoms", "limit", "nres", "nstep", "npri")
+            "fixed_freedoms", "limit", "nres", "nstep", "npri", "incs", "plasits", "cjits")
     for k in keys[1:]:
         v = out[k]
         out[k] = int(float(v)) if k in ints else float(v)
+    if program == "p122":
+        out["qinc"] = np.array([float(t) for t in toks[len(keys):len(keys) + out["incs"]]])
     nod, nels, nn = out["nod"], out["nels"], out["nn"]
     coords = np.zeros((nn, 3))
     g_num = np.zeros((nels, nod), dtype=np.int32)
@@ -503,7 +509,7 @@ def read_deck(basename: str, program: str = "p121") -> dict:
         g_num = abaqus_to_sg(g_num)
     out["g_num"] = g_num
     out["coords"] = coords
-    nodof = 3 if program == "p121" else 1
+    nodof = 3 if program in ("p121", "p122") else 1
     if out["nr"] > 0:
         rest = np.array(open(basename + ".bnd").read().split(), dtype=np.int64)
         out["rest"] = rest.reshape(out["nr"], nodof + 1).astype(np.int32)
@@ -515,7 +521,7 @@ def read_deck(basename: str, program: str = "p121") -> dict:
         a = a[: nl * (nodof + 1)].reshape(nl, nodof + 1)
         out["load_node"] = a[:, 0].astype(np.int32)
         out["load_val"] = a[:, 1:]
-    if program in ("p123", "p124") and out.get("fixed_freedoms", 0) > 0:
+    if program in ("p122", "p123", "p124") and out.get("fixed_freedoms", 0) > 0:
         a = np.array(open(basename + ".fix").read().split(), dtype=np.float64)
         a = a.reshape(out["fixed_freedoms"], 3)
         out["fix_node"] = a[:, 0].astype(np.int32)

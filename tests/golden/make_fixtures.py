@@ -17,6 +17,12 @@ Writes (data only, no reference source):
                            types) and the temperature history its .res prints;
   p124_demo_ttr.npz     -- the nodal temperatures of p124_demo.ensi.NDTTR-*
                            at steps 10, 80 and 150 (as printed, 5 digits).
+  p122_demo.json        -- control data and load increments of
+                           examples/5th_ed/p122/demo/p122_demo.dat, its
+                           restraints (.bnd) and fixed freedoms (.fix), and
+                           the per-increment results its .res prints;
+  p122_demo_mesh.npz    -- the demo's nodal coordinates and element nodes
+                           (.d; a graded mesh no generator here reproduces).
 The xx3_tiny/ directory is a verbatim copy of examples/dev/xx3/demo/xx3-tiny.*
 (the deck and its .res/.dis results).
 """
@@ -93,6 +99,41 @@ def p124_main():
     print(json.dumps({k: v for k, v in out.items() if k != "history"}, indent=1))
 
 
+REF122 = "/root/reference/parafem/examples/5th_ed/p122/demo/p122_demo"
+
+
+def p122_main():
+    d = M.read_deck(REF122, "p122")
+    res = open(REF122 + ".res").read().splitlines()
+    incs = []
+    for k, line in enumerate(res):
+        if line.startswith("Load Increment"):
+            disp = float(res[k + 1].split()[-1])
+            sig = [float(t) for t in res[k + 3].split()]
+            cj = int(res[k + 4].split()[-1])
+            pl = int(res[k + 5].split()[-1])
+            incs.append({"increment": int(line.split()[-1]), "displacement": disp, "sigma_zxy": sig,
+                         "cj_iterations": cj, "plastic_iterations": pl})
+    keys = ("element", "meshgen", "partitioner", "nels", "nn", "nr", "nip", "nod", "fixed_freedoms",
+            "loaded_nodes", "phi", "c", "psi", "e", "v", "incs", "plasits", "cjits", "plastol", "cjtol")
+    out = {
+        "source": "examples/5th_ed/p122/demo/p122_demo.{dat,d,bnd,fix,res}",
+        "control": {k: d[k] for k in keys},
+        "qinc": d["qinc"].tolist(),
+        "rest": d["rest"].tolist(),
+        "fix": [[int(n), int(sn), float(v)] for n, sn, v in zip(d["fix_node"], d["fix_sense"], d["fix_val"])],
+        "res_processes": int(res[0].split()[4]),
+        "res_neq": int(res[1].split()[7]),
+        "res_dt": float(res[3].split()[-1]),
+        "increments": incs,
+    }
+    with open(os.path.join(HERE, "p122_demo.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    np.savez_compressed(os.path.join(HERE, "p122_demo_mesh.npz"), coords=d["coords"], g_num=d["g_num"],
+                        etype=d["etype"])
+    print(json.dumps({k: v for k, v in out.items() if k not in ("rest",)}, indent=1)[:2000])
+
+
 def main():
     d = M.read_deck(REF)
     res = open(REF + ".res").read().splitlines()
@@ -125,3 +166,4 @@ def main():
 if __name__ == "__main__":
     main()
     p124_main()
+    p122_main()

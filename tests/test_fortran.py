@@ -23,6 +23,7 @@ from parafem_amd import mesh as M, problems as P
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "parafem_amd", "bin", "pf_p121")
 EXE124 = os.path.join(ROOT, "parafem_amd", "bin", "pf_p124")
+EXE122 = os.path.join(ROOT, "parafem_amd", "bin", "pf_p122")
 
 needs_exe = pytest.mark.skipif(not os.path.exists(EXE), reason="pf_p121 not built (no amdflang)")
 
@@ -171,3 +172,40 @@ def test_fortran_p124_demo(tmp_path, golden):
         refv = ttr[f"step{step}"].astype(np.float64)
         unit = 10.0 ** (np.floor(np.log10(np.maximum(np.abs(refv), 1e-30))) - 4)
         assert np.all(np.abs(nodal - refv) <= unit + 1e-4 * np.abs(refv).max()), step
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_fortran_p122_demo(tmp_path, golden):
+    """pf_p122 on the p122 demo deck (1152 bricks, 19 prescribed
+    displacements, 10 increments): the critical time step, per increment the
+    displacement and the three stresses print as p122_demo.res prints them,
+    with the same CG and plastic iteration counts; the total displacements
+    agree with the oracle's run in the device's dot order to 1e-9."""
+    sys.path.insert(0, os.path.dirname(__file__))
+    import json
+    import oracle as O
+    from test_p122 import demo, oracle_p122
+    meta, base = demo(golden, tmp_path)
+    say("pf_p122 demo deck")
+    r = subprocess.run([EXE122, os.path.basename(base)], cwd=os.path.dirname(base), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, f"pf_p122 failed ({r.returncode}):\n{r.stdout}\n{r.stderr}"
+    res = open(base + ".pf.res").read().splitlines()
+    assert f"The critical timestep is   {M.fortran_e(meta['res_dt'], 12, 4)}" in res
+    incs = []
+    for k, line in enumerate(res):
+        if line.startswith("Load Increment"):
+            incs.append((res[k + 1], res[k + 3], int(res[k + 4].split()[-1]), int(res[k + 5].split()[-1])))
+    pub = meta["increments"]
+    assert len(incs) == len(pub)
+    for (ld, ls, cj, pl), inc in zip(incs, pub):
+        assert ld == "The displacement is  " + M.fortran_e(inc["displacement"], 12, 4), ld
+        assert ls == "".join(M.fortran_e(v, 12, 4) for v in inc["sigma_zxy"]), ls
+        assert (cj, pl) == (inc["cj_iterations"], inc["plastic_iterations"]), (inc["increment"], cj, pl)
+    d = M.read_deck(base, "p122")
+    _, totd, _ = oracle_p122(d, dot=lambda a, b: O.dot_device(a, b))
+    x = np.fromfile(base + ".pf.x", dtype=np.float64)
+    assert np.abs(x - totd).max() <= 1e-9 * np.abs(totd).max()
+    lines = open(base + ".pf.ensi.DISPL-000010").read().split("\n")
+    assert lines[:4] == ["Alya Ensight Gold --- Vector per-node variable file", "part", "    1", "coordinates"]

@@ -683,6 +683,49 @@ def test_transient_matrices_and_products_bitexact(ctx):
     assert np.array_equal(ctx.apply_raw(1, x), O.apply(g_g, KB, x))
 
 
+def test_plastic_update_matches_oracle(ctx):
+    """p122's Gauss-point stress update on the device against the oracle
+    on a distorted brick mesh loaded past yield (prestressed Gauss points,
+    friction and dilation on): body loads, viscoplastic strains and the
+    final stresses agree to rounding (the device's trigonometric functions
+    are not the host's: 1e-11 of each array's scale); pf_fixed_rows sets
+    the fixed rows as p122's CG does."""
+    m = M.box_hex8(3, 2, 2, 0.5, 0.5, 0.5)
+    rng = np.random.default_rng(19)
+    m.coords = m.coords + rng.uniform(-0.05, 0.05, m.coords.shape)
+    nf, neq = M.steering(m.nn, 3, M.base_fixed_rest(m))
+    g_g = M.g_g_from(nf, m.g_num)
+    gc = m.g_coord()
+    say(f"plastic update nels={m.nels} neq={neq}")
+    ctx.mesh_set(gc)
+    ctx.steer_set(g_g, neq)
+    x = 1e-3 * rng.normal(size=neq)
+    evpt = 1e-5 * rng.normal(size=(m.nels, 8, 6))
+    tensor = rng.normal(size=(m.nels, 8, 6)) * 20.0
+    args = (1e4, 0.3, 20.0, 5.0, 5.0, 5e-4)
+    dx, dev, dte = DeviceArray.from_host(ctx, x), DeviceArray.from_host(ctx, evpt), DeviceArray.from_host(ctx, tensor)
+    dut = DeviceArray(ctx, m.nels * 24)
+    for final in (False, True):
+        ctx.plastic_update(dx, dev, dte, dut, *args, final)
+        bl = O.plastic_update(gc, g_g, x, evpt, tensor, *args, final)
+        got = dut.download().reshape(m.nels, 24)
+        assert np.abs(bl).max() > 0
+        assert np.abs(got - bl).max() <= 1e-11 * np.abs(bl).max(), final
+        assert np.abs(dev.download().reshape(evpt.shape) - evpt).max() <= 1e-11 * np.abs(evpt).max()
+        assert np.abs(dte.download().reshape(tensor.shape) - tensor).max() <= 1e-11 * np.abs(tensor).max()
+    # fixed rows: u(j) = p(j)*store, then 0
+    no_f = np.array([3, 7, 11], dtype=np.int32)
+    ctx.assemble_elastic(1e4, 0.3)
+    store = ctx.fixed_set(no_f, 1e20)
+    p = rng.normal(size=neq)
+    du, dp = DeviceArray.from_host(ctx, np.ones(neq)), DeviceArray.from_host(ctx, p)
+    ctx.fixed_rows(du, dp, 1)
+    u = du.download()
+    assert np.array_equal(u[no_f - 1], p[no_f - 1] * store) and np.array_equal(np.delete(u, no_f - 1), np.ones(neq - 3))
+    ctx.fixed_rows(du, None, 0)
+    assert not du.download()[no_f - 1].any()
+
+
 def test_context_reuse_and_trim():
     """A context solving problems one after another reuses its freed device
     blocks (same result as a fresh context, bit for bit); pf_trim gives the
