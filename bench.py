@@ -57,6 +57,13 @@ def iteration_bytes(nels, ntot, neq, store, precision="mixed"):
     return nels * ebe_bytes(ntot, store, precision) + 15 * neq * 8
 
 
+def ebe_kernel_name(ntot, precision, epl):
+    """The iteration's element-product kernel as rocprofv3 names it."""
+    kind, out = {"fp64": (0, "double"), "mixed": (1, "float"), "split": (2, "double"),
+                 "mixed64": (3, "double")}[precision]
+    return f"k_ebe<{ntot}, {kind}, true, {out}, false, {epl}>"
+
+
 def time_iterations(ctx, pr, precision, steps, warmup):
     """Iterations/s with tol = 0 (no early exit), then a per-kernel pass with
     HIP events on the context's stream."""
@@ -393,6 +400,7 @@ def main():
             "setup_assembly_s": round(t_setup, 3),
             "device_gib": round(ctx.mem_used() / 2**30, 2),
             "tile_shape": {"ebe_epl": shape["ebe_epl"], "source": "autotuner"},
+            "kernel": ebe_kernel_name(pr.ntot, prec, shape["ebe_epl"]),
         }
         log(f"{prec}: {paths[prec]['iter_per_s']} iter/s, element products {ebe_ms:.3f} ms "
             f"({achieved:.0f} GB/s), kernels " +
@@ -418,7 +426,7 @@ def main():
 
     head = paths[args.precision]
     mixed = args.precision == "mixed"
-    kname = f"k_ebe<{pr.ntot}, {1 if mixed else 0}, true, {'float' if mixed else 'double'}>"
+    kname = head["kernel"]
     traffic, tsource = profiled_traffic(args.workload, kname) if args.n == (100 if args.workload == "hex20" else 200) \
         else (None, None)
     out = {
