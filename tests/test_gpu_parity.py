@@ -421,6 +421,60 @@ def test_serving_mix_equals_alone():
         assert np.abs(a.x - c.x).max() <= 1e-8 * np.abs(c.x).max()
 
 
+def _ragged_laplace():
+    """A 4^3 conduction cube with the 8 bricks around the node at (2,2,-2)
+    repeated twice and 3 of them a third time, each copy with its own
+    conductivity: that node's equation has 27 element slots and its
+    neighbours up to 19, past the 8 a brick mesh gives."""
+    pr = P.laplace_cube(4, lognormal_k=True)
+    m = pr.mesh
+    centre = int(np.nonzero(np.all(np.abs(m.coords - [2.0, 2.0, -2.0]) < 1e-12, axis=1))[0][0]) + 1
+    els = np.nonzero((m.g_num == centre).any(axis=1))[0]
+    assert els.size == 8
+    dup = np.concatenate([els, els, els[:3]])
+    m.g_num = np.ascontiguousarray(np.vstack([m.g_num, m.g_num[dup]]))
+    pr.g_g = np.ascontiguousarray(np.vstack([pr.g_g, pr.g_g[dup]]))
+    pr.k_elem = np.concatenate([pr.k_elem, np.random.default_rng(21).uniform(0.5, 2.0, dup.size)])
+    counts = np.bincount(pr.g_g.ravel(), minlength=pr.neq + 1)[1:]
+    assert counts.max() == 27 and (counts > 8).sum() >= 9
+    return pr
+
+
+def test_ragged_slot_lists_bitexact(ctx):
+    """Equations with more element slots than the unrolled first 8 of the
+    scatter: device assembly, SCATTER, the FP64 PCG and the block product
+    against the oracle (bit-exact; the block product within its tolerances),
+    the mixed path within 1e-8 of the FP64 solution."""
+    pr = _ragged_laplace()
+    say(f"ragged slot lists nels={pr.nels} neq={pr.neq}")
+    Ko = O.assemble_laplace(pr.g_coord(), 1.0, 1.0, 1.0, pr.k_elem)
+    r = P.setup(ctx, pr)
+    assert np.array_equal(ctx.storkm_get(), Ko)
+    dg = O.diag(pr.g_g, Ko, pr.neq)
+    store = dg[pr.fix_eq - 1] + P.PENALTY
+    dg[pr.fix_eq - 1] = store
+    x, res = ctx.pcg_solve(r, 3000, 1e-10, stop="relres")
+    xo, it, conv = O.pcg(pr.g_g, Ko, r, dg, 1e-10, 3000, stop=O.STOP_RELRES, no_f=pr.fix_eq,
+                         store=store, dot_order=O.DOT_DEVICE)
+    assert conv and res.iters == it and np.array_equal(x, xo)
+    rng = np.random.default_rng(22)
+    ut_o = rng.normal(size=(pr.nels, pr.ntot))
+    ut = DeviceArray.from_host(ctx, ut_o.reshape(-1))
+    u0 = rng.normal(size=pr.neq)
+    du = DeviceArray.from_host(ctx, u0)
+    ctx.scatter(du, ut)
+    assert np.array_equal(du.download(), u0 + O.scatter(pr.g_g, ut_o, pr.neq))
+    X = rng.normal(size=(pr.neq, 16))
+    ref = np.stack([O.apply(pr.g_g, Ko, X[:, c]) for c in range(16)], axis=1)
+    scale = np.abs(ref).max()
+    assert np.abs(ctx.block_apply(X, "fp64") - ref).max() <= 1e-13 * scale
+    assert np.abs(ctx.block_apply(X, "mixed") - ref).max() <= 2e-6 * scale
+    ctx.storkm_set(Ko, store="fp32")
+    r = P.start_r(ctx, pr)
+    xm, rm = ctx.pcg_solve(r, 5000, 1e-10, stop="relres", precision="mixed")
+    assert rm.converged and np.abs(xm - xo).max() <= 1e-8 * np.abs(xo).max()
+
+
 @pytest.mark.parametrize("nt_case,k", [("hex20", 64), ("hex20", 37), ("hex8_elastic", 16), ("hex8_laplace", 64)])
 def test_block_apply_matrix_cores(ctx, nt_case, k):
     """The multi-RHS element product on the matrix cores (FP64 and FP32
