@@ -142,10 +142,16 @@ def cpu_baseline(workload: str, n_sample: int, nels_full: int, budget_s: float =
               dot_order=O.DOT_THREADS)
         return time.perf_counter() - t
 
-    t1 = run(2)
-    per = max(run(4) - t1, 1e-6) / 2
-    n = max(4, int(budget_s / per))
-    t = run(n + 2) - t1
+    run(2)                                   # thread pool and caches warm
+    k = 4
+    while True:                              # a pilot run of at least 0.5 s
+        t = run(k)
+        if t >= 0.5 or k >= 4096:
+            break
+        k *= 2
+    n = max(4, min(int(budget_s * k / t), 100000))
+    log(f"cpu baseline: {k} iterations in {t:.2f} s, timing {n}")
+    t = run(n)
     rate = n / t
     return {
         "value": rate * pr.nels / nels_full,
