@@ -140,6 +140,7 @@ static void free_problem_vectors(pf_ctx *c)
     free_arr(c, c->d, nv); free_arr(c, c->u, nv); free_arr(c, c->dinv, nv);
     free_arr(c, c->diag, nv); free_arr(c, c->diag0, nv); free_arr(c, c->f, nv);
     free_arr(c, c->fixstore, nv);
+    free_arr(c, c->p32, nv);
     c->nfix = 0;
     c->pcg_ready = false;
 }

@@ -20,7 +20,7 @@
 //                 sums of (r,d), (r,r), block maxima of |xnew|, |xnew-x|;
 //   k_fold_update (one block) beta, the stopping rule, the iteration count
 //                 and the freeze flag;
-//   k_pupdate     p = d + p*beta.
+//   k_pupdate     p = d + p*beta (and, for the mixed solver, p32 = fl32(p)).
 // The mixed solver adds a true-residual replacement (reliable update) on
 // the iterations its fold kernel flags: k_ebe over hi+lo in FP64 with x,
 // k_scatter_res (r = f - K x, d = dinv*r) and k_fold_refine.
@@ -162,7 +162,9 @@ __device__ __forceinline__ double bpermute64(int addr, double v)
 //   KIND_F64: full K_e in FP64 column pairs, FP64 fused-multiply-add chain in
 //             ascending column order -> out (double);
 //   KIND_F32: packed symmetric FP32 hi, FP32 arithmetic on fl32(p) -> out
-//             (float);
+//             (float); fl32(p) is gathered from src32 when given (the
+//             mixed solver keeps it beside p for hex8 meshes: half the
+//             gathered bytes);
 //   KIND_ACC: packed symmetric (double)hi + (double)lo, FP64 arithmetic ->
 //             out (double);
 //   KIND_WIDE: packed symmetric (double)hi, FP64 arithmetic on the FP64 p ->
@@ -174,8 +176,9 @@ __device__ __forceinline__ double bpermute64(int addr, double v)
 template <int NT, int KIND, bool GATHER, typename OUT, bool LOOP = false, int EPLT = Lanes<NT, KIND>::EPL>
 __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
                                                  const float *__restrict__ Klo, const int *__restrict__ gg,
-                                                 const double *__restrict__ src, OUT *__restrict__ out, int nels,
-                                                 long kstride, const pf_state *__restrict__ st, int gate, long nlb)
+                                                 const double *__restrict__ src, const float *__restrict__ src32,
+                                                 OUT *__restrict__ out, int nels, long kstride,
+                                                 const pf_state *__restrict__ st, int gate, long nlb)
 {
     if (skip(st, gate)) return;
     using L = Lanes<NT, KIND>;
@@ -240,7 +243,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
         float pv[EPL];
 #pragma unroll
         for (int t = 0; t < EPL; ++t) {
-            pv[t] = (float)(GATHER ? src[g[t]] : src[ec[t] * NT + i]);
+            if constexpr (GATHER)
+                pv[t] = src32 ? src32[g[t]] : (float)src[g[t]];
+            else
+                pv[t] = (float)src[ec[t] * NT + i];
             float *pg = reinterpret_cast<float *>(&ps[wave][t][0]) + 2 * gb;
             if (li < NT) {
                 pg[li] = pv[t];
@@ -537,19 +543,25 @@ __global__ __launch_bounds__(PF_BLOCK) void k_fold_refine(const double *partials
     }
 }
 
-__global__ __launch_bounds__(PF_BLOCK) void k_pupdate(double *__restrict__ p, const double *__restrict__ d, int neq,
+// p = d + p*beta, and p32 = fl32(p) when the FP32 products gather it.
+__global__ __launch_bounds__(PF_BLOCK) void k_pupdate(double *__restrict__ p, float *__restrict__ p32,
+                                                     const double *__restrict__ d, int neq,
                                                      const pf_state *__restrict__ st)
 {
     if (skip(st, RUN_LIVE)) return;
     const double beta = st->beta;
     const int stride = gridDim.x * PF_BLOCK;
-    for (int q = blockIdx.x * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride)
-        p[q] = __dadd_rn(d[q], __dmul_rn(p[q], beta));
+    for (int q = blockIdx.x * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
+        const double pq = __dadd_rn(d[q], __dmul_rn(p[q], beta));
+        p[q] = pq;
+        if (p32) p32[q] = (float)pq;
+    }
 }
 
 // PCG start: dinv = 1/diag, d = dinv*r, p = d, x = 0, f = r; partial (r,d), (r,r).
 __global__ __launch_bounds__(PF_BLOCK) void k_pcg_init(double *__restrict__ x, const double *__restrict__ r,
                                                       double *__restrict__ d, double *__restrict__ p,
+                                                      float *__restrict__ p32,
                                                       double *__restrict__ dinv, const double *__restrict__ diag,
                                                       double *__restrict__ f, int neq, int keep_dinv,
                                                       double *partials)
@@ -563,6 +575,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_pcg_init(double *__restrict__ x, c
         dinv[q] = di;
         d[q] = dq;
         p[q] = dq;
+        if (p32) p32[q] = (float)dq;
         x[q] = 0.0;
         f[q] = rq;
         v[0] = __dadd_rn(v[0], __dmul_rn(rq, dq));
@@ -570,6 +583,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_pcg_init(double *__restrict__ x, c
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         x[0] = 0.0; d[0] = 0.0; p[0] = 0.0; dinv[0] = 0.0; f[0] = 0.0;
+        if (p32) p32[0] = 0.0f;
     }
     constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
     constexpr bool mx[2] = {false, false};
@@ -701,16 +715,19 @@ __global__ void k_scatter_fix(const int *__restrict__ ptr, const int *__restrict
 }
 
 template <int NT, int KIND, bool GATHER, typename OUT, int EPL>
-void launch_ebe_epl(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *K64)
+void launch_ebe_epl(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *K64,
+                    const float *src32)
 {
     constexpr int EPB = Lanes<NT, KIND>::EPW * EPL * (PF_BLOCK / 64);
     const long nlb = (c->nels + EPB - 1) / EPB;
     hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, false, EPL>), dim3((int)nlb), dim3(PF_BLOCK), 0, c->stream,
-                       K64, c->K32hi, c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
+                       K64, c->K32hi, c->K32lo, c->g_g, src, src32, out, c->nels, pf_k64_estride(c), st, gate,
+                       nlb);
 }
 
 template <int NT, int KIND, bool GATHER, typename OUT>
-int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *Kalt)
+int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *Kalt,
+                  const float *src32)
 {
     const double *K64 = Kalt ? Kalt : c->K64;
     if (gate == RUN_REFINE) {
@@ -718,20 +735,21 @@ int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, in
         const long nlb = (c->nels + Lanes<NT, KIND>::EPB - 1) / Lanes<NT, KIND>::EPB;
         const int grid = (int)std::min<long>(nlb, 2048);
         hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, true>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, K64,
-                           c->K32hi, c->K32lo, c->g_g, src, out, c->nels, pf_k64_estride(c), st, gate, nlb);
+                           c->K32hi, c->K32lo, c->g_g, src, src32, out, c->nels, pf_k64_estride(c), st, gate,
+                           nlb);
     } else {
         // the tuned shape of the iteration's product (an out-of-range value,
         // e.g. from before the mesh changed, keeps the built-in one)
         constexpr int E0 = Lanes<NT, KIND>::EPL;
         const int e = GATHER && c->tune_epl <= pf_epl_max(NT) ? c->tune_epl : 0;
         if constexpr (GATHER) {
-            if (e == 1) launch_ebe_epl<NT, KIND, GATHER, OUT, 1>(c, src, out, st, gate, K64);
-            else if (e == 2) launch_ebe_epl<NT, KIND, GATHER, OUT, 2>(c, src, out, st, gate, K64);
-            else if (e == 4) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 24 ? 4 : E0)>(c, src, out, st, gate, K64);
-            else if (e == 8) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 8 ? 8 : E0)>(c, src, out, st, gate, K64);
-            else launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate, K64);
+            if (e == 1) launch_ebe_epl<NT, KIND, GATHER, OUT, 1>(c, src, out, st, gate, K64, src32);
+            else if (e == 2) launch_ebe_epl<NT, KIND, GATHER, OUT, 2>(c, src, out, st, gate, K64, src32);
+            else if (e == 4) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 24 ? 4 : E0)>(c, src, out, st, gate, K64, src32);
+            else if (e == 8) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 8 ? 8 : E0)>(c, src, out, st, gate, K64, src32);
+            else launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate, K64, src32);
         } else {
-            launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate, K64);
+            launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate, K64, src32);
         }
     }
     PF_HIP(hipGetLastError());
@@ -739,12 +757,13 @@ int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, in
 }
 
 template <int KIND, bool GATHER, typename OUT>
-int launch_ebe(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *Kalt = nullptr)
+int launch_ebe(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *Kalt = nullptr,
+               const float *src32 = nullptr)
 {
     switch (c->ntot) {
-    case 60: return launch_ebe_nt<60, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt);
-    case 24: return launch_ebe_nt<24, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt);
-    case 8: return launch_ebe_nt<8, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt);
+    case 60: return launch_ebe_nt<60, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt, src32);
+    case 24: return launch_ebe_nt<24, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt, src32);
+    case 8: return launch_ebe_nt<8, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt, src32);
     default: return PF_ERR_ARG;
     }
 }
@@ -891,6 +910,13 @@ int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opt
         return PF_ERR_ARG;
     }
     PF_HIP(hipSetDevice(c->device));
+    // fl32(p) beside p where the gathered vector is a large share of an
+    // element's bytes (hex8 Laplace: 64 of 256; hex20: 480 of 8160)
+    if (opts->precision == PF_PREC_MIXED && c->ntot <= 24 && !c->p32) {
+        pf_graph_drop(c);
+        const int rc = pf_alloc(c, (void **)&c->p32, ((size_t)c->neq + 1) * sizeof(float));
+        if (rc) return rc;
+    }
     c->opts = *opts;
     PF_HIP(hipMemcpyAsync(c->r + 1, r0, sizeof(double) * c->neq,
                           r0_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
@@ -905,7 +931,8 @@ int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opt
     *c->state_host = s;
     PF_HIP(hipMemcpyAsync(c->state, c->state_host, sizeof(pf_state), hipMemcpyHostToDevice, c->stream));
     const int G = pf_grid_for(c->neq);
-    hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, c->dinv, c->diag,
+    float *p32 = opts->precision == PF_PREC_MIXED && c->ntot <= 24 ? c->p32 : nullptr;
+    hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, p32, c->dinv, c->diag,
                        c->f, c->neq, c->precon_given ? 1 : 0, c->partials);
     hipLaunchKernelGGL(k_fold_init, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
     PF_HIP(hipGetLastError());
@@ -925,13 +952,14 @@ int pf_pcg_run(pf_ctx *c, int n)
     const bool mixed = c->opts.precision == PF_PREC_MIXED;
     const bool wide = c->opts.precision == PF_PREC_MIXED64;
     const bool split = c->opts.precision == PF_PREC_SPLIT;
+    float *p32 = mixed && c->ntot <= 24 ? c->p32 : nullptr;
     hipEvent_t t0, t1;
     for (int it = 0; it < n; ++it) {
         int rc;
         bool tm = pf_timing_begin(c, PF_K_EBE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
         if (mixed)
-            rc = launch_ebe<KIND_F32, true, float>(c, c->p, c->utemp32, c->state, RUN_LIVE);
+            rc = launch_ebe<KIND_F32, true, float>(c, c->p, c->utemp32, c->state, RUN_LIVE, nullptr, p32);
         else if (wide)
             rc = launch_ebe<KIND_WIDE, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE);
         else if (split)
@@ -979,7 +1007,7 @@ int pf_pcg_run(pf_ctx *c, int n)
         }
         tm = pf_timing_begin(c, PF_K_PUPDATE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
-        hipLaunchKernelGGL(k_pupdate, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->p, c->d, c->neq, c->state);
+        hipLaunchKernelGGL(k_pupdate, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->p, p32, c->d, c->neq, c->state);
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         PF_HIP(hipGetLastError());
     }

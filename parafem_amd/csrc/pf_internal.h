@@ -148,6 +148,7 @@ struct pf_ctx {
     double *x = nullptr, *r = nullptr, *p = nullptr, *d = nullptr, *u = nullptr;
     double *dinv = nullptr, *diag = nullptr, *f = nullptr;
     double *diag0 = nullptr;           // element diagonal before penalty rows
+    float *p32 = nullptr;              // fl32(p): the FP32 products' gather source (mixed path)
     double *utemp = nullptr;           // (ntot,nels) element products, FP64
     float *utemp32 = nullptr;          // (ntot,nels) element products, FP32
     double *partials = nullptr;        // PF_NQ * PF_REDUCE_MAX_GRID (+ spare)
