@@ -277,6 +277,18 @@ int pf_apply_raw(pf_ctx *ctx, int which, const double *x_host, double *y_host);
  * time step.  Trigonometric functions are the device library's. */
 int pf_plastic_update(pf_ctx *ctx, const double *x_dev, double *evpt_dev, double *tensor_dev, double *utemp_dev,
                       double e, double v, double phi, double c, double psi, double dt, int final_pass);
+/* p122's warm-started CG (p122.f90:141-162) as one call on device vectors
+ * (neq): x = x0 and r = r0, which the caller forms as loads - K x0 with
+ * GATHER / element products / SCATTER as p122.f90:141-145 does; d = dinv*r,
+ * p = d; then the iterations of pf_pcg_solve (checon_par at opts->tol,
+ * opts->limit = cjits) with the freedoms of pf_fixed_set giving
+ * u(j) = p(j)*store(j) (fixed_mode 1, p122's first plastic iteration) or
+ * u(j) = 0 (fixed_mode 0, the later ones).  xnew receives the last iterate
+ * (p122's xnew_pp; checon_par leaves x_pp equal to it).  PF_PREC_FP64 or
+ * PF_PREC_SPLIT; the same arithmetic, in the same order, as the statement
+ * form on the primitives above. */
+int pf_pcg_solve_warm(pf_ctx *ctx, const double *r0_dev, const double *x0_dev, double *xnew_dev, int fixed_mode,
+                      const pf_pcg_opts *opts, pf_pcg_result *res);
 /* p122's CG statement for the fixed freedoms set by pf_fixed_set
  * (p122.f90:141-146): u(j) = p(j)*store(j) (mode 1, first plastic
  * iteration) or u(j) = 0 (mode 0, later ones).  Device vectors, neq. */

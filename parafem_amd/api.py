@@ -135,6 +135,7 @@ def lib():
         "pf_apply_raw": (i32, [vp, i32, pd, pd]),
         "pf_plastic_update": (i32, [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, dbl, dbl, i32]),
         "pf_fixed_rows": (i32, [vp, vp, vp, i32]),
+        "pf_pcg_solve_warm": (i32, [vp, vp, vp, vp, i32, C.POINTER(PcgOpts), C.POINTER(PcgResult)]),
         "pf_steer_get": (i32, [vp, pi]),
         "pf_tune_set": (i32, [vp, C.POINTER(Tune)]),
         "pf_tune_get": (i32, [vp, C.POINTER(Tune)]),
@@ -495,6 +496,18 @@ class Context:
     def _res(res):
         return SolveResult(res.iters, bool(res.converged), res.crit, res.rr0, res.rr,
                            res.refinements, res.status)
+
+    def pcg_solve_warm(self, r0: "DeviceArray", x0: "DeviceArray", xnew: "DeviceArray", fixed_mode: int,
+                       limit: int, tol: float, stop: str = "checon", precision: str = "fp64",
+                       check_every: int = 0) -> "SolveResult":
+        """p122's warm-started CG on device vectors: x = x0, r = r0 (the
+        caller's loads - K x0), fixed rows u = p*store (fixed_mode 1) or 0
+        (fixed_mode 0); xnew receives the last iterate."""
+        o = self._opts(limit, tol, stop, precision, check_every)
+        res = PcgResult()
+        rc = lib().pf_pcg_solve_warm(self.h, r0.ptr, x0.ptr, xnew.ptr, int(fixed_mode), C.byref(o), C.byref(res))
+        _check(rc, "pf_pcg_solve_warm", ok=(PF_OK, PF_NOT_CONVERGED, PF_ERR_BREAKDOWN))
+        return self._res(res)
 
     def pcg_begin(self, r0, limit: int, tol: float, stop: str = "checon", precision: str = "fp64",
                   refine_every: int = 0, refine_delta: float = 0.0, on_device: bool = False):

@@ -357,13 +357,15 @@ __device__ __forceinline__ double slot_sum(const int *__restrict__ slot, const U
     return s;
 }
 
-// SCATTER fused with p.u.  u and p have the leading 0 slot.
+// SCATTER fused with p.u.  u and p have the leading 0 slot.  A fixed
+// freedom's u is p*store (p123; p122's first plastic iteration) or, with
+// fix_zero, 0 (p122's later plastic iterations).
 template <typename UT>
 __global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__ ptr, const int *__restrict__ slot,
                                                         const UT *__restrict__ utemp,
                                                         const double *__restrict__ fixstore,
                                                         const double *__restrict__ p, double *__restrict__ u,
-                                                        int neq, double *partials, const pf_state *st)
+                                                        int neq, int fix_zero, double *partials, const pf_state *st)
 {
     if (skip(st, RUN_LIVE)) return;
     double acc[1] = {0.0};
@@ -373,7 +375,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__
         const double pq = p[q + 1];
         if (fixstore) {
             const double fs = fixstore[q + 1];
-            if (fs != 0.0) s = __dmul_rn(pq, fs);
+            if (fs != 0.0) s = fix_zero ? 0.0 : __dmul_rn(pq, fs);
         }
         u[q + 1] = s;
         acc[0] = __dadd_rn(acc[0], __dmul_rn(pq, s));
@@ -558,13 +560,14 @@ __global__ __launch_bounds__(PF_BLOCK) void k_pupdate(double *__restrict__ p, fl
     }
 }
 
-// PCG start: dinv = 1/diag, d = dinv*r, p = d, x = 0, f = r; partial (r,d), (r,r).
+// PCG start: dinv = 1/diag, d = dinv*r, p = d, x = x0 (or 0), f = r; partial
+// (r,d), (r,r).
 __global__ __launch_bounds__(PF_BLOCK) void k_pcg_init(double *__restrict__ x, const double *__restrict__ r,
                                                       double *__restrict__ d, double *__restrict__ p,
                                                       float *__restrict__ p32,
                                                       double *__restrict__ dinv, const double *__restrict__ diag,
-                                                      double *__restrict__ f, int neq, int keep_dinv,
-                                                      double *partials)
+                                                      double *__restrict__ f, const double *__restrict__ x0, int neq,
+                                                      int keep_dinv, double *partials)
 {
     double v[2] = {0.0, 0.0};
     const int stride = gridDim.x * PF_BLOCK;
@@ -576,7 +579,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_pcg_init(double *__restrict__ x, c
         d[q] = dq;
         p[q] = dq;
         if (p32) p32[q] = (float)dq;
-        x[q] = 0.0;
+        x[q] = x0 ? x0[q - 1] : 0.0;
         f[q] = rq;
         v[0] = __dadd_rn(v[0], __dmul_rn(rq, dq));
         v[1] = __dadd_rn(v[1], __dmul_rn(rq, rq));
@@ -893,7 +896,8 @@ int pf_checon_par(pf_ctx *c, const double *xnew, double tol, int *converged, dou
     return rc;
 }
 
-int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opts *opts)
+// pf_pcg_begin, with x starting at x0 (device, neq) when x0 is given.
+static int pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const double *x0, const pf_pcg_opts *opts)
 {
     if (!c || !r0 || !opts) return PF_ERR_ARG;
     if (!c->have_K) return PF_ERR_STATE;
@@ -933,7 +937,7 @@ int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opt
     const int G = pf_grid_for(c->neq);
     float *p32 = opts->precision == PF_PREC_MIXED && c->ntot <= 24 ? c->p32 : nullptr;
     hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, p32, c->dinv, c->diag,
-                       c->f, c->neq, c->precon_given ? 1 : 0, c->partials);
+                       c->f, x0, c->neq, c->precon_given ? 1 : 0, c->partials);
     hipLaunchKernelGGL(k_fold_init, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
     PF_HIP(hipGetLastError());
     // The host mirror is read back by pf_pcg_poll; wait so it is not reused
@@ -942,6 +946,11 @@ int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opt
     c->pcg_ready = true;
     c->query_pending = false;
     return PF_OK;
+}
+
+int pf_pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const pf_pcg_opts *opts)
+{
+    return pcg_begin(c, r0, r0_on_device, nullptr, opts);
 }
 
 int pf_pcg_run(pf_ctx *c, int n)
@@ -972,10 +981,12 @@ int pf_pcg_run(pf_ctx *c, int n)
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
         if (mixed)
             hipLaunchKernelGGL(k_scatter_pu<float>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, c->csr_slot,
-                               c->utemp32, c->fixstore, c->p, c->u, c->neq, c->partials, c->state);
+                               c->utemp32, c->fixstore, c->p, c->u, c->neq, c->fix_zero ? 1 : 0, c->partials,
+                               c->state);
         else
             hipLaunchKernelGGL(k_scatter_pu<double>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr,
-                               c->csr_slot, c->utemp, c->fixstore, c->p, c->u, c->neq, c->partials, c->state);
+                               c->csr_slot, c->utemp, c->fixstore, c->p, c->u, c->neq, c->fix_zero ? 1 : 0,
+                               c->partials, c->state);
         hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         tm = pf_timing_begin(c, PF_K_UPDATE, &t0, &t1);
@@ -1132,15 +1143,11 @@ int pf_pcg_r(pf_ctx *c, double *r, int on_device)
     return PF_OK;
 }
 
-int pf_pcg_solve(pf_ctx *c, const double *r0, double *x, const pf_pcg_opts *opts, pf_pcg_result *res)
+// Iterations in chunks of check_every (default 16) until the state freezes.
+static int pcg_finish(pf_ctx *c, const pf_pcg_opts *opts, pf_pcg_result *R)
 {
-    if (!c || !r0 || !x || !opts) return PF_ERR_ARG;
-    int rc = pf_pcg_begin(c, r0, 0, opts);
-    if (rc) return rc;
-    pf_pcg_result tmp{};
-    pf_pcg_result *R = res ? res : &tmp;
     const int chunk = opts->check_every > 0 ? opts->check_every : 16;
-    rc = pf_pcg_poll(c, R);
+    int rc = pf_pcg_poll(c, R);
     while (!rc && !c->state_host->frozen) {
         int n = chunk;
         const int left = opts->limit - R->iters;
@@ -1149,8 +1156,39 @@ int pf_pcg_solve(pf_ctx *c, const double *r0, double *x, const pf_pcg_opts *opts
         rc = pf_pcg_run(c, n);
         if (!rc) rc = pf_pcg_poll(c, R);
     }
+    return rc;
+}
+
+int pf_pcg_solve(pf_ctx *c, const double *r0, double *x, const pf_pcg_opts *opts, pf_pcg_result *res)
+{
+    if (!c || !r0 || !x || !opts) return PF_ERR_ARG;
+    int rc = pf_pcg_begin(c, r0, 0, opts);
+    if (rc) return rc;
+    pf_pcg_result tmp{};
+    pf_pcg_result *R = res ? res : &tmp;
+    rc = pcg_finish(c, opts, R);
     if (rc) return rc;
     rc = pf_pcg_x(c, x, 0);
+    if (rc) return rc;
+    return R->status;
+}
+
+int pf_pcg_solve_warm(pf_ctx *c, const double *r0, const double *x0, double *xnew, int fixed_mode,
+                      const pf_pcg_opts *opts, pf_pcg_result *res)
+{
+    if (!c || !r0 || !x0 || !xnew || !opts || (fixed_mode != 0 && fixed_mode != 1)) return PF_ERR_ARG;
+    // no residual replacement: its true residual would apply the fixed rows
+    // that the caller's r0 = loads - K x0 leaves out
+    if (opts->precision != PF_PREC_FP64 && opts->precision != PF_PREC_SPLIT) return PF_ERR_ARG;
+    int rc = pcg_begin(c, r0, 1, x0, opts);
+    if (rc) return rc;
+    pf_pcg_result tmp{};
+    pf_pcg_result *R = res ? res : &tmp;
+    c->fix_zero = fixed_mode == 0;   // read by the launches of pf_pcg_run below
+    rc = pcg_finish(c, opts, R);
+    c->fix_zero = false;
+    if (rc) return rc;
+    rc = pf_pcg_x(c, xnew, 1);
     if (rc) return rc;
     return R->status;
 }

@@ -142,6 +142,7 @@ struct pf_ctx {
     // p123 fixed freedoms: fixstore[eq] = store value, 0 for free equations
     double *fixstore = nullptr;
     int nfix = 0;
+    bool fix_zero = false;             // fixed rows give u = 0 instead of p*store (p122, later plastic iterations)
 
     // PCG vectors: length neq+1, entry 0 is a permanent zero so that a
     // steering value of 0 gathers 0 without a branch.

@@ -58,7 +58,7 @@ MODULE pf_gpu
   PUBLIC :: dot_product_p, sum_p, checon_par
   PUBLIC :: vec_fill, vec_waxpy, vec_mul, vec_recip
   PUBLIC :: pf_stress, calc_nodes_pp, scatter_nodes, dismsh_ensi_p
-  PUBLIC :: pf_assemble_transient, pf_apply_raw, pf_plastic_update, pf_fixed_rows
+  PUBLIC :: pf_assemble_transient, pf_apply_raw, pf_plastic_update, pf_fixed_rows, pf_cg_warm
 
   INTERFACE
     INTEGER(c_int) FUNCTION c_create(device, c) BIND(C, name="pf_create")
@@ -250,6 +250,13 @@ MODULE pf_gpu
       REAL(c_double), VALUE :: e, v, phi, coh, psi, dt
       INTEGER(c_int), VALUE :: final
     END FUNCTION
+    INTEGER(c_int) FUNCTION c_pcg_solve_warm(c, r0, x0, xnew, mode, opts, res) BIND(C, name="pf_pcg_solve_warm")
+      IMPORT :: c_int, c_ptr, pf_pcg_opts, pf_pcg_result
+      TYPE(c_ptr), VALUE :: c, r0, x0, xnew
+      INTEGER(c_int), VALUE :: mode
+      TYPE(pf_pcg_opts), INTENT(IN) :: opts
+      TYPE(pf_pcg_result), INTENT(OUT) :: res
+    END FUNCTION c_pcg_solve_warm
     INTEGER(c_int) FUNCTION c_fixed_rows(c, u, p, mode) BIND(C, name="pf_fixed_rows")
       IMPORT :: c_int, c_ptr
       TYPE(c_ptr), VALUE :: c, u, p
@@ -582,6 +589,36 @@ CONTAINS
     IF (first) mode = 1
     CALL pf_check(c_fixed_rows(ctx, u_pp%ptr, p_pp%ptr, mode), 'pf_fixed_rows')
   END SUBROUTINE pf_fixed_rows
+
+  !> p122's conjugate_gradients loop (p122.f90:147-162) as one call: from
+  !> x_pp and r_pp (= loads - K x_pp, formed by the caller as p122.f90:141-145
+  !> does), with the fixed rows of pf_fixed as pf_fixed_rows(first) sets
+  !> them, to checon_par at cjtol or cjits iterations; xnew_pp receives the
+  !> last iterate (and x_pp, as checon_par leaves it), cjiters the count.
+  SUBROUTINE pf_cg_warm(r_pp, x_pp, xnew_pp, first, cjits, cjtol, cjiters)
+    TYPE(pf_dvec), INTENT(IN) :: r_pp
+    TYPE(pf_dvec), INTENT(INOUT) :: x_pp, xnew_pp
+    LOGICAL, INTENT(IN) :: first
+    INTEGER, INTENT(IN) :: cjits
+    REAL(iwp), INTENT(IN) :: cjtol
+    INTEGER, INTENT(OUT) :: cjiters
+    TYPE(pf_pcg_opts) :: o
+    TYPE(pf_pcg_result) :: res
+    INTEGER(c_int) :: rc, mode
+    mode = 0
+    IF (first) mode = 1
+    o%limit = cjits
+    o%tol = cjtol
+    o%stop_mode = PF_STOP_CHECON
+    o%precision = PF_PREC_FP64
+    o%check_every = 0
+    o%refine_every = 0
+    o%refine_delta = 0.0_iwp
+    rc = c_pcg_solve_warm(ctx, r_pp%ptr, x_pp%ptr, xnew_pp%ptr, mode, o, res)
+    IF (rc /= PF_OK .AND. rc /= PF_NOT_CONVERGED) CALL pf_check(rc, 'pf_pcg_solve_warm')
+    CALL vec_waxpy(x_pp, xnew_pp, xnew_pp, 0.0_iwp)
+    cjiters = res%iters
+  END SUBROUTINE pf_cg_warm
 
   !> Stresses at every element's centroid (p121.f90:112-123 for each element):
   !> sigma(6,nels) from the displacements x(neq) and DEEMAT(e, v).

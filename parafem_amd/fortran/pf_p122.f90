@@ -2,10 +2,10 @@
 ! pf_p122: the p122 program flow (src/programs/5th_ed/p122/p122.f90) on one
 ! MI355X through the pf_gpu module -- three-dimensional strain of an
 ! elastic-viscoplastic (Mohr-Coulomb) solid of 8-node bricks, load
-! increments, plastic iterations, each a warm-started Jacobi PCG written
-! statement for statement with GATHER / element products / SCATTER /
-! DOT_PRODUCT_P / checon_par on device vectors, and the Gauss-point stress
-! update on the device.
+! increments, plastic iterations, each a warm-started Jacobi PCG (r formed
+! with GATHER / element products / SCATTER on device vectors, the CG loop as
+! one device call doing p122's statements in their order), and the
+! Gauss-point stress update on the device.
 !
 !   pf_p122 <deck>
 !
@@ -25,16 +25,16 @@ PROGRAM pf_p122
   INTEGER :: nels, nn, nr, nip, nod, fixed_freedoms, loaded_nodes, meshgen, partitioner, incs, plasits, cjits
   INTEGER :: ntot, neq, iel, i, j, k, iy, plasiters, cjiters, cjtot, nlen, ios
   INTEGER :: node_start, node_end, nodes_pp
-  REAL(iwp) :: phi, c, psi, e, v, plastol, cjtol, dt, snph, pi, up, alpha, beta, xyz(3)
-  LOGICAL :: plastic_converged, cj_converged, final
+  REAL(iwp) :: phi, c, psi, e, v, plastol, cjtol, dt, snph, pi, xyz(3)
+  LOGICAL :: plastic_converged, final
   CHARACTER(LEN=256) :: argv, line
   CHARACTER(LEN=15) :: element
   CHARACTER(LEN=6) :: ch
   INTEGER, ALLOCATABLE :: g_num(:, :), rest(:, :), nf(:, :), g_g_pp(:, :), row(:), nodef(:), sense(:), no_f(:)
   INTEGER, ALLOCATABLE :: node(:)
   REAL(iwp), ALLOCATABLE :: coord(:, :), g_coord_pp(:, :, :), qinc(:), valf(:), store(:), val(:, :)
-  REAL(iwp), ALLOCATABLE :: diag(:), h(:), ld0(:), totd(:), tensor(:), eld(:, :), disp(:), temp(:)
-  TYPE(pf_dvec) :: loads_pp, bdylds_pp, oldis_pp, x_pp, xnew_pp, r_pp, d_pp, p_pp, u_pp, diag_precon_pp
+  REAL(iwp), ALLOCATABLE :: h(:), ld0(:), totd(:), tensor(:), eld(:, :), disp(:), temp(:)
+  TYPE(pf_dvec) :: loads_pp, bdylds_pp, oldis_pp, x_pp, xnew_pp, r_pp
   TYPE(pf_dvec) :: ld0_pp, totd_pp, pmul_pp, utemp_pp, evpt_pp, tensor_pp
   INTEGER(8) :: t0, t1, rate
 
@@ -130,8 +130,7 @@ PROGRAM pf_p122
   CALL pf_mesh(g_coord_pp)
   CALL pf_steer(g_g_pp, neq)
   CALL pf_assemble_elastic(e, v, PF_STORE_FP64)
-  ALLOCATE(diag(neq), h(neq), ld0(neq), totd(neq))
-  CALL pf_diag(diag)
+  ALLOCATE(h(neq), ld0(neq), totd(neq))
 
   !---------------------------------------------------------------- loads
   ld0 = 0.0_iwp
@@ -151,7 +150,8 @@ PROGRAM pf_p122
 
   !------------------------------------------------------- fixed freedoms
   ! READ_FIXED node, sense, value; FIND_NO: equation of freedom `sense` of
-  ! the node; penalty on the diagonal, store = the new diagonal.
+  ! the node; penalty on the diagonal, store = the new diagonal (the
+  ! preconditioner is 1/diagonal, p122.f90:112-118).
   ALLOCATE(nodef(fixed_freedoms), sense(fixed_freedoms), valf(fixed_freedoms), no_f(fixed_freedoms), &
            store(fixed_freedoms))
   IF (fixed_freedoms > 0) THEN
@@ -161,22 +161,15 @@ PROGRAM pf_p122
       no_f(i) = nf(sense(i), nodef(i))
     END DO
     CLOSE(10)
-    CALL pf_fixed(no_f, penalty, store)       ! the rows pf_fixed_rows acts on
-    DO i = 1, fixed_freedoms
-      diag(no_f(i)) = diag(no_f(i)) + penalty
-      store(i) = diag(no_f(i))
-    END DO
+    CALL pf_fixed(no_f, penalty, store)
   END IF
-  diag = 1.0_iwp / diag
 
   !------------------------------------------------------ device vectors
   CALL pf_alloc(loads_pp, neq); CALL pf_alloc(bdylds_pp, neq); CALL pf_alloc(oldis_pp, neq)
-  CALL pf_alloc(x_pp, neq); CALL pf_alloc(xnew_pp, neq); CALL pf_alloc(r_pp, neq); CALL pf_alloc(d_pp, neq)
-  CALL pf_alloc(p_pp, neq); CALL pf_alloc(u_pp, neq); CALL pf_alloc(diag_precon_pp, neq)
+  CALL pf_alloc(x_pp, neq); CALL pf_alloc(xnew_pp, neq); CALL pf_alloc(r_pp, neq)
   CALL pf_alloc(ld0_pp, neq); CALL pf_alloc(totd_pp, neq)
   CALL pf_alloc(pmul_pp, ntot * nels); CALL pf_alloc(utemp_pp, ntot * nels)
   CALL pf_alloc(evpt_pp, nst * nip * nels); CALL pf_alloc(tensor_pp, nst * nip * nels)
-  CALL pf_put(diag_precon_pp, diag)
   CALL pf_put(ld0_pp, ld0)
   CALL vec_fill(oldis_pp, 0.0_iwp); CALL vec_fill(x_pp, 0.0_iwp); CALL vec_fill(totd_pp, 0.0_iwp)
   CALL vec_fill(tensor_pp, 0.0_iwp)
@@ -211,33 +204,17 @@ PROGRAM pf_p122
         END IF
         CALL pf_fixed_rows(loads_pp, loads_pp, .FALSE.)
       END IF
-      ! r = loads - K x (x warm from the previous solve), d = dinv r, p = d
+      ! r = loads - K x (x warm from the previous solve)
       CALL vec_fill(r_pp, 0.0_iwp)
       CALL gather(x_pp, pmul_pp)
       CALL elem_matvec(pmul_pp, utemp_pp)
       CALL scatter(r_pp, utemp_pp)
       CALL vec_waxpy(r_pp, loads_pp, r_pp, -1.0_iwp)
-      CALL vec_mul(d_pp, diag_precon_pp, r_pp)
-      CALL vec_waxpy(p_pp, d_pp, d_pp, 0.0_iwp)
-      cjiters = 0
-      DO
-        cjiters = cjiters + 1
-        CALL vec_fill(u_pp, 0.0_iwp)
-        CALL gather(p_pp, pmul_pp)
-        CALL elem_matvec(pmul_pp, utemp_pp)
-        CALL scatter(u_pp, utemp_pp)
-        IF (fixed_freedoms > 0) CALL pf_fixed_rows(u_pp, p_pp, plasiters == 1)
-        up = dot_product_p(r_pp, d_pp)
-        alpha = up / dot_product_p(p_pp, u_pp)
-        CALL vec_waxpy(xnew_pp, x_pp, p_pp, alpha)
-        CALL vec_waxpy(r_pp, r_pp, u_pp, -alpha)
-        CALL vec_mul(d_pp, diag_precon_pp, r_pp)
-        beta = dot_product_p(r_pp, d_pp) / up
-        CALL vec_waxpy(p_pp, d_pp, p_pp, beta)
-        cj_converged = .TRUE.
-        CALL checon_par(xnew_pp, cjtol, cj_converged, x_pp)
-        IF (cj_converged .OR. cjiters == cjits) EXIT
-      END DO
+      ! d = diag_precon r, p = d and the conjugate_gradients loop
+      ! (p122.f90:147-162: u = K p with the fixed rows, alpha, xnew, r, d,
+      ! beta, p, checon_par) on the device as one call, statement for
+      ! statement the same arithmetic
+      CALL pf_cg_warm(r_pp, x_pp, xnew_pp, plasiters == 1, cjits, cjtol, cjiters)
       cjtot = cjtot + cjiters
       CALL vec_waxpy(loads_pp, xnew_pp, xnew_pp, 0.0_iwp)
       CALL checon_par(loads_pp, plastol, plastic_converged, oldis_pp)

@@ -737,6 +737,83 @@ def test_transient_matrices_and_products_bitexact(ctx):
     assert np.array_equal(ctx.apply_raw(1, x), O.apply(g_g, KB, x))
 
 
+def _cg_statements(ctx, loads, x, dinv, n, nels, ntot, fixed_mode, cjtol, cjits):
+    """p122's CG (p122.f90:141-162) statement for statement on the drop-in
+    primitives, device vectors: returns (xnew, cjiters)."""
+    V = lambda: DeviceArray(ctx, n)
+    r, d, p, u, xnew = V(), V(), V(), V(), V()
+    pmul, ut = DeviceArray(ctx, nels * ntot), DeviceArray(ctx, nels * ntot)
+    ctx.vec_fill(r, 0.0)
+    ctx.gather(x, pmul)
+    ctx.elem_matvec(pmul, ut)
+    ctx.scatter(r, ut)
+    ctx.vec_waxpy(r, loads, r, -1.0)
+    ctx.vec_mul(d, dinv, r)
+    ctx.vec_waxpy(p, d, d, 0.0)
+    cj = 0
+    while True:
+        cj += 1
+        ctx.vec_fill(u, 0.0)
+        ctx.gather(p, pmul)
+        ctx.elem_matvec(pmul, ut)
+        ctx.scatter(u, ut)
+        ctx.fixed_rows(u, p, fixed_mode)
+        up = ctx.dot_product_p(r, d)
+        alpha = up / ctx.dot_product_p(p, u)
+        ctx.vec_waxpy(xnew, x, p, alpha)
+        ctx.vec_waxpy(r, r, u, -alpha)
+        ctx.vec_mul(d, dinv, r)
+        beta = ctx.dot_product_p(r, d) / up
+        ctx.vec_waxpy(p, d, p, beta)
+        if ctx.checon_par(xnew, cjtol, x) or cj == cjits:
+            return xnew.download(), cj
+
+
+@pytest.mark.parametrize("fixed_mode", [1, 0])
+def test_warm_started_cg_equals_statements(ctx, fixed_mode):
+    """pf_pcg_solve_warm (x = x0, caller's r0 = loads - K x0, fixed rows
+    u = p*store or 0) against the same CG written statement for statement
+    on the primitives: the same iteration count and the same bits, both when
+    checon stops it and at the iteration limit."""
+    m = M.box_hex8(4, 3, 3, 0.5, 0.5, 0.5)
+    rng = np.random.default_rng(23)
+    m.coords = m.coords + rng.uniform(-0.05, 0.05, m.coords.shape)
+    nf, neq = M.steering(m.nn, 3, M.base_fixed_rest(m))
+    g_g = M.g_g_from(nf, m.g_num)
+    say(f"warm-started cg mode={fixed_mode} neq={neq}")
+    ctx.mesh_set(m.g_coord())
+    ctx.steer_set(g_g, neq)
+    ctx.assemble_elastic(1e4, 0.3)
+    no_f = np.arange(1, neq + 1, 37, dtype=np.int32)
+    store = ctx.fixed_set(no_f, 1.0e20)
+    diag = ctx.diag_get()
+    assert np.array_equal(diag[no_f - 1], store)
+    # as in p122: prescribed increments store*value on the fixed rows in
+    # the first plastic iteration (mode 1), 0 there in the later ones
+    loads_h = rng.normal(size=neq)
+    loads_h[no_f - 1] = store * 1e-3 * rng.normal(size=no_f.size) if fixed_mode else 0.0
+    x0_h = 1e-3 * rng.normal(size=neq)
+    loads, dinv = DeviceArray.from_host(ctx, loads_h), DeviceArray.from_host(ctx, diag)
+    ctx.vec_recip(dinv, dinv)
+    for cjtol, cjits in ((1e-6, 2000), (1e-12, 9)):
+        x = DeviceArray.from_host(ctx, x0_h)
+        xs, its = _cg_statements(ctx, loads, x, dinv, neq, m.nels, 24, fixed_mode, cjtol, cjits)
+        # the fused call: r0 formed the same way, then one call
+        x0 = DeviceArray.from_host(ctx, x0_h)
+        r0, pmul, ut = DeviceArray(ctx, neq), DeviceArray(ctx, m.nels * 24), DeviceArray(ctx, m.nels * 24)
+        ctx.vec_fill(r0, 0.0)
+        ctx.gather(x0, pmul)
+        ctx.elem_matvec(pmul, ut)
+        ctx.scatter(r0, ut)
+        ctx.vec_waxpy(r0, loads, r0, -1.0)
+        xnew = DeviceArray(ctx, neq)
+        res = ctx.pcg_solve_warm(r0, x0, xnew, fixed_mode, cjits, cjtol, stop="checon")
+        assert res.iters == its and res.converged == (its < cjits), (res, its)
+        assert np.array_equal(xnew.download(), xs)
+    with pytest.raises(PFError):
+        ctx.pcg_solve_warm(r0, x0, xnew, fixed_mode, 10, 1e-6, stop="relres", precision="mixed")
+
+
 def test_plastic_update_matches_oracle(ctx):
     """p122's Gauss-point stress update on the device against the oracle
     on a distorted brick mesh loaded past yield (prestressed Gauss points,
