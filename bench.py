@@ -43,12 +43,14 @@ def log(msg):
 
 def ebe_bytes(ntot, store, precision="mixed"):
     """Algorithmic HBM bytes of one element product: the element matrix once
-    (full FP64, or the FP32 hi part packed symmetric: ntot/2 + 1 entries per
-    row, hi and lo for the split precision) and the int32 steering once."""
+    (full FP64, or the FP32 hi part packed symmetric: the ntot(ntot+1)/2
+    entries of one triangle padded to a multiple of 4, hi and lo for the
+    split precision) and the int32 steering once."""
     if store == "fp64":
         return ntot * ntot * 8 + ntot * 4
     parts = 2 if precision == "split" else 1
-    return parts * (ntot // 2 + 1) * ntot * 4 + ntot * 4
+    packed = (ntot * (ntot + 1) // 2 + 3) // 4 * 4
+    return parts * packed * 4 + ntot * 4
 
 
 def iteration_bytes(nels, ntot, neq, store, precision="mixed"):

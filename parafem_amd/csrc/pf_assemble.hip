@@ -253,7 +253,7 @@ __global__ void k_split_fp32(const double *__restrict__ K, float *__restrict__ h
                              int nt, long n)
 {
     const long per = pf_sym_per(nt);
-    const int D = pf_sym_diags(nt), Q4 = D & ~3, R = D - Q4;
+    const int Q4 = 4 * pf_sym_quads(nt), RF = pf_sym_rem(nt), H = nt / 2;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
         const long e = q / per;
         const int w = (int)(q - e * per);
@@ -262,10 +262,17 @@ __global__ void k_split_fp32(const double *__restrict__ K, float *__restrict__ h
             const int dq = w / (4 * nt), rem = w - dq * 4 * nt;
             i = rem >> 2;
             d = 4 * dq + (rem & 3);
-        } else {
+        } else if (w < H * nt) {
             const int w2 = w - Q4 * nt;
-            i = w2 / R;
-            d = Q4 + (w2 - i * R);
+            i = w2 / RF;
+            d = Q4 + (w2 - i * RF);
+        } else if (w < H * nt + H) {
+            i = w - H * nt;
+            d = H;
+        } else {                        // padding
+            hi[q] = 0.0f;
+            lo[q] = 0.0f;
+            continue;
         }
         int j = i + d;
         if (j >= nt) j -= nt;

@@ -172,7 +172,8 @@ __device__ __forceinline__ double bpermute64(int addr, double v)
 // Packed symmetric rows: lane i holds K(i, i+d mod NT) for d = 0..NT/2 and
 // forms sum_d K(i,i+d) p(i+d) from p staged twice in LDS, then receives
 // K(i-d,i) p(i) from lane i-d (mod NT) for d = 1..NT/2-1 by ds_bpermute,
-// where lane j sends K(j,j+d) p(j): every entry is read from HBM once.
+// where lane j sends K(j,j+d) p(j): every entry is read from HBM once (the
+// d = NT/2 entries, shared by rows i and i + NT/2, are stored once).
 template <int NT, int KIND, bool GATHER, typename OUT, bool LOOP = false, int EPLT = Lanes<NT, KIND>::EPL>
 __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
                                                  const float *__restrict__ Klo, const int *__restrict__ gg,
@@ -183,7 +184,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
     if (skip(st, gate)) return;
     using L = Lanes<NT, KIND>;
     constexpr int EPL = EPLT;
-    constexpr int D = pf_sym_diags(NT), Q = D / 4, R = D - 4 * Q;
+    // packed layout: Q quads and R more full diagonals, then the half one (d = H)
+    constexpr int D = pf_sym_diags(NT), Q = pf_sym_quads(NT), R = pf_sym_rem(NT), H = NT / 2;
+    constexpr long PER = pf_sym_per(NT);
+    static_assert(D == 4 * Q + R + 1, "packed layout");
     // Per wave and element slot: p_e (FP64), or p_e twice (packed kinds).
     __shared__ __attribute__((aligned(16))) double ps[PF_BLOCK / 64][EPL][KIND == KIND_F32 ? 64 : 128];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -231,7 +235,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
         float kd[EPL][D];
 #pragma unroll
         for (int t = 0; t < EPL; ++t) {
-            const float *Ke = Khi + ec[t] * (long)(D * NT);
+            const float *Ke = Khi + ec[t] * PER;
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const f4_t v = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Ke) + q * NT + i);
@@ -239,6 +243,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) kd[t][4 * Q + r] = __builtin_nontemporal_load(Ke + 4 * Q * NT + R * i + r);
+            kd[t][H] = __builtin_nontemporal_load(Ke + H * NT + i % H);
         }
         float pv[EPL];
 #pragma unroll
@@ -274,8 +279,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
         double kd[EPL][D];
 #pragma unroll
         for (int t = 0; t < EPL; ++t) {
-            const float *Kh = Khi + ec[t] * (long)(D * NT);
-            const float *Kl = Klo + ec[t] * (long)(D * NT);
+            const float *Kh = Khi + ec[t] * PER;
+            const float *Kl = Klo + ec[t] * PER;
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const f4_t h = __builtin_nontemporal_load(reinterpret_cast<const f4_t *>(Kh) + q * NT + i);
@@ -297,6 +302,9 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
                 kd[t][4 * Q + r] = LO ? (double)__builtin_nontemporal_load(Kh + 4 * Q * NT + R * i + r) +
                                             (double)__builtin_nontemporal_load(Kl + 4 * Q * NT + R * i + r)
                                       : (double)__builtin_nontemporal_load(Kh + 4 * Q * NT + R * i + r);
+            kd[t][H] = LO ? (double)__builtin_nontemporal_load(Kh + H * NT + i % H) +
+                                (double)__builtin_nontemporal_load(Kl + H * NT + i % H)
+                          : (double)__builtin_nontemporal_load(Kh + H * NT + i % H);
         }
         double pv[EPL];
 #pragma unroll

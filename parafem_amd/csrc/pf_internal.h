@@ -34,18 +34,26 @@ __host__ __device__ inline long pf_idx_pair(long e, int nt, int i, int j)
     return e * (long)nt * nt + (long)(j >> 1) * 2 * nt + 2 * i + (j & 1);
 }
 
-// Packed symmetric storage of the FP32 hi and lo parts.  Row i keeps the
+// Packed symmetric storage of the FP32 hi and lo parts.  Row i uses the
 // nt/2 + 1 entries K(i, (i+d) mod nt), d = 0..nt/2, taken from the upper
 // triangle K(min, max); the entries of row i left of the diagonal band are
 // the band entries of other rows (K(i, i-d) = K(i-d, (i-d)+d)).  Per element:
-// d in quads, lane-major (16-byte loads), then the D mod 4 remainder.
+// the nt/2 full diagonals d < nt/2 in quads, lane-major (16-byte loads),
+// then their remainder (nt/2 mod 4, lane-major), then the last diagonal
+// d = nt/2 once: its nt/2 distinct entries, row i reading entry i mod nt/2
+// (K(i, i+nt/2) = K(i+nt/2, i)).  nt(nt+1)/2 values, padded to a multiple of
+// 4 so that every element starts 16-byte aligned.
 __host__ __device__ constexpr int pf_sym_diags(int nt) { return nt / 2 + 1; }
-__host__ __device__ constexpr long pf_sym_per(int nt) { return (long)pf_sym_diags(nt) * nt; }
+__host__ __device__ constexpr int pf_sym_quads(int nt) { return (nt / 2) / 4; }
+__host__ __device__ constexpr int pf_sym_rem(int nt) { return nt / 2 - 4 * pf_sym_quads(nt); }
+__host__ __device__ constexpr long pf_sym_per(int nt) { return ((long)nt * (nt + 1) / 2 + 3) & ~3L; }
 __host__ __device__ inline long pf_idx_sym(long e, int nt, int i, int d)
 {
-    const int D = pf_sym_diags(nt), Q4 = D & ~3, R = D - Q4;
-    const long base = e * (long)D * nt;
-    return d < Q4 ? base + (long)(d >> 2) * 4 * nt + 4 * i + (d & 3) : base + (long)Q4 * nt + (long)R * i + (d - Q4);
+    const int Q4 = 4 * pf_sym_quads(nt), RF = pf_sym_rem(nt), H = nt / 2;
+    const long base = e * pf_sym_per(nt);
+    if (d < Q4) return base + (long)(d >> 2) * 4 * nt + 4 * i + (d & 3);
+    if (d < H) return base + (long)Q4 * nt + (long)RF * i + (d - Q4);
+    return base + (long)H * nt + i % H;
 }
 // Packed slot of K(i, j).
 __host__ __device__ inline long pf_idx_sym_ij(long e, int nt, int i, int j)

@@ -104,7 +104,7 @@ def device_bytes(n: int, precision: str) -> int:
     neq = 3 * ((2 * n + 1) * (n + 1) * (n + 1) + (n + 1) * (n + 1) * n)   # upper bound
     per_el = 28800 + 480 + 240 + 240 + 480
     if precision != "fp64":
-        per_el += 2 * 7440 + 240
+        per_el += 2 * 7328 + 240         # packed symmetric hi and lo, FP32 products
     return nels * per_el + neq * 8 * 12 + (16 << 20)
 
 
