@@ -365,9 +365,34 @@ __device__ __forceinline__ double slot_sum(const int *__restrict__ slot, const U
     return s;
 }
 
+// slot_sum with the slot indices of [a, b) read from LDS, where lds[k]
+// holds slot[a0 + k].
+template <typename UT>
+__device__ __forceinline__ double slot_sum_lds(const int *lds, int a0, const UT *__restrict__ utemp, int a, int b)
+{
+    int sl[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sl[j] = a + j < b ? lds[a - a0 + j] : -1;
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = sl[j] >= 0 ? (double)utemp[sl[j]] : 0.0;
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (sl[j] >= 0) s = __dadd_rn(s, v[j]);
+    for (int k = a + 8; k < b; ++k) s = __dadd_rn(s, (double)utemp[lds[k - a0]]);
+    return s;
+}
+
 // SCATTER fused with p.u.  u and p have the leading 0 slot.  A fixed
 // freedom's u is p*store (p123; p122's first plastic iteration) or, with
-// fix_zero, 0 (p122's later plastic iterations).
+// fix_zero, 0 (p122's later plastic iterations).  Thread t of the grid takes
+// equations t, t + stride, ... and adds their p.u terms in that order; each
+// wavefront first copies the slot indices of its 64 equations (one
+// contiguous range of the slot list) into LDS with whole-line loads, then
+// every lane reads its own slots from there (more than SC_CAP of them: read
+// from memory as they are).
+constexpr int SC_CAP = 1024;
 template <typename UT>
 __global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__ ptr, const int *__restrict__ slot,
                                                         const UT *__restrict__ utemp,
@@ -376,17 +401,34 @@ __global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__
                                                         int neq, int fix_zero, double *partials, const pf_state *st)
 {
     if (skip(st, RUN_LIVE)) return;
+    __shared__ int sls[PF_BLOCK / 64][SC_CAP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double acc[1] = {0.0};
     const int stride = gridDim.x * PF_BLOCK;
-    for (int q = blockIdx.x * PF_BLOCK + threadIdx.x; q < neq; q += stride) {
-        double s = slot_sum(slot, utemp, ptr[q], ptr[q + 1]);
-        const double pq = p[q + 1];
-        if (fixstore) {
-            const double fs = fixstore[q + 1];
-            if (fs != 0.0) s = fix_zero ? 0.0 : __dmul_rn(pq, fs);
+    for (int q0 = blockIdx.x * PF_BLOCK + wave * 64; q0 < neq; q0 += stride) {
+        const int q = q0 + lane;
+        const bool live = q < neq;
+        const int a = live ? ptr[q] : 0, b = live ? ptr[q + 1] : 0;
+        const int a0 = ptr[q0], a1 = ptr[q0 + 64 < neq ? q0 + 64 : neq];
+        double s = 0.0;
+        if (a1 - a0 <= SC_CAP) {
+            for (int k = lane; k < a1 - a0; k += 64) sls[wave][k] = slot[a0 + k];
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (live) s = slot_sum_lds(&sls[wave][0], a0, utemp, a, b);
+            __builtin_amdgcn_wave_barrier();   // these reads finish before the next range is written
+        } else if (live) {
+            s = slot_sum(slot, utemp, a, b);
         }
-        u[q + 1] = s;
-        acc[0] = __dadd_rn(acc[0], __dmul_rn(pq, s));
+        if (live) {
+            const double pq = p[q + 1];
+            if (fixstore) {
+                const double fs = fixstore[q + 1];
+                if (fs != 0.0) s = fix_zero ? 0.0 : __dmul_rn(pq, fs);
+            }
+            u[q + 1] = s;
+            acc[0] = __dadd_rn(acc[0], __dmul_rn(pq, s));
+        }
     }
     constexpr int qid[1] = {PF_Q_PU};
     constexpr bool mx[1] = {false};
