@@ -566,6 +566,58 @@ def write_p121_deck(basename: str, mesh: Mesh, rest, load_node, load_val,
         fh.write(f"{e:12.4E}{v:12.4E}{tol:12.4E}{limit:8d}\n")
 
 
+def box_bc8_rest(nxe: int, nye: int, nze: int) -> np.ndarray:
+    """BOX_BC8 (``src/modules/shared/geometry.f90:655-696``) for the
+    GEOMETRY_8BXZ numbering: in each of the nye layers of (nxe+1)(nze+1)
+    nodes, the first row (z = 0) and the last node of every row (x = max);
+    then the whole last layer (y = max).  Rows (node, 0), ascending node."""
+    face = (nxe + 1) * (nze + 1)
+    nodes = []
+    j = np.arange(1, face + 1)
+    for i in range(nye):
+        jj = i * face + j
+        on = (jj % (nxe + 1) == 0) | (jj < i * face + nxe + 1)
+        nodes.append(jj[on])
+    nodes.append(nye * face + j)
+    nodes = np.concatenate(nodes)
+    return np.stack([nodes, np.zeros_like(nodes)], axis=1).astype(np.int32)
+
+
+def write_p123_deck(basename: str, mesh: Mesh, rest, kx: float, ky: float, kz: float, tol: float,
+                    limit: int, nres: int, loads=None, fix=None, nip: int = 8, abaqus: bool = True) -> None:
+    """A p123 deck in the formats READ_P123, READ_G_NUM_PP, READ_G_COORD_PP,
+    READ_REST, READ_LOADS and READ_FIXED read (input.f90:3620-3806, 935-1080,
+    288-445, 2570-2632, 2350-2411, 2483-2564), laid out as p12meshgen's p123
+    case writes them (p12meshgen.f90:706-800).  loads: (equation, value)
+    rows; fix: (node, sense, value) rows."""
+    os.makedirs(os.path.dirname(os.path.abspath(basename)), exist_ok=True)
+    g = sg_to_abaqus(mesh.g_num) if abaqus else mesh.g_num
+    with open(basename + ".d", "w") as fh:
+        fh.write("*THREE_DIMENSIONAL\n*NODES\n")
+        for i, (x, y, z) in enumerate(mesh.coords, 1):
+            fh.write(f"{i:12d}{x:14.6E}{y:14.6E}{z:14.6E}\n")
+        fh.write("*ELEMENTS\n")
+        for i, row in enumerate(g, 1):
+            fh.write(f"{i:12d} 3 8 1 " + "".join(f"{n:10d}" for n in row) + " 1\n")
+    with open(basename + ".bnd", "w") as fh:
+        for r in rest:
+            fh.write(f"{r[0]:8d}" + "".join(f"{x:6d}" for x in r[1:]) + "\n")
+    nl = 0 if loads is None else len(loads)
+    nfix = 0 if fix is None else len(fix)
+    if nl:
+        with open(basename + ".lds", "w") as fh:
+            for eq, val in loads:
+                fh.write(f"{int(eq):10d}{val:16.8E}\n")
+    if nfix:
+        with open(basename + ".fix", "w") as fh:
+            for node, sense, val in fix:
+                fh.write(f"{int(node):10d}{int(sense):6d}{val:16.8E}\n")
+    with open(basename + ".dat", "w") as fh:
+        fh.write("'hexahedron'\n" + ("2" if abaqus else "1") + "\n1\n")
+        fh.write(f"{mesh.nels:12d}{mesh.nn:12d}{len(rest):12d}{nip:6d}{8:6d}{nl:6d}{nfix:6d}\n")
+        fh.write(f"{kx:12.4E}{ky:12.4E}{kz:12.4E}{tol:12.4E}{limit:8d}{nres:8d}\n")
+
+
 def p124_box_rest(mesh: Mesh) -> np.ndarray:
     """The restraint of the p124 demo cube: temperature held (freedom 0) on
     the faces x = max, y = max and z = 0."""

@@ -119,6 +119,26 @@ def laplace_cube(n: int, lognormal_k: bool = True, seed: int = 2, flux: float = 
                    fix_val=np.zeros(fix_eq.size))
 
 
+def p123_book(nxe: int = 200, nye: int = 200, nze: int = 200, h: float = 0.005, k: float = 2.0,
+              load: float = 10.0) -> Problem:
+    """The p123 deck of the reference's book example
+    (``examples/5th_ed/p123/book/p123.mg``: 200^3 bricks of edge 0.005,
+    kx = ky = kz = 2) as p12meshgen's p123 case builds it
+    (``src/tools/preprocessing/p12meshgen/p12meshgen.f90:658-760``):
+    GEOMETRY_8BXZ mesh, BOX_BC8 restraints, one loaded freedom, equation
+    nres = nxe (nze - 1) + 1, with value 10 (used as an equation number,
+    p123.f90:108-118).  info["nres"] is that equation."""
+    m = M.box_hex8(nxe, nye, nze, h, h, h)
+    rest = M.box_bc8_rest(nxe, nye, nze)
+    nf, neq = M.steering(m.nn, 1, rest)
+    g_g = M.g_g_from(nf, m.g_num)
+    nres = nxe * (nze - 1) + 1
+    r0 = np.zeros(neq)
+    r0[nres - 1] = load
+    return Problem("laplace", m, g_g, neq, r0, nf, kx=k, ky=k, kz=k,
+                   info={"nres": nres, "rest": rest})
+
+
 def block_cube(n: int, k: int = 64, seed: int = 3, e: float = 100.0, v: float = 0.3):
     """Multi-RHS workload: hex20 cube (unit edges, E, nu uniform, base
     fixed) with k load cases; case c is a top-face traction whose three

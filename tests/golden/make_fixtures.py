@@ -23,6 +23,10 @@ Writes (data only, no reference source):
                            the per-increment results its .res prints;
   p122_demo_mesh.npz    -- the demo's nodal coordinates and element nodes
                            (.d; a graded mesh no generator here reproduces).
+  p123_book.json        -- the generator input of examples/5th_ed/p123/book
+                           (p123.mg) and what its p123.res prints (nodes,
+                           restrained, equations, iterations, total load,
+                           the four potentials).
 The xx3_tiny/ directory is a verbatim copy of examples/dev/xx3/demo/xx3-tiny.*
 (the deck and its .res/.dis results).
 """
@@ -134,6 +138,33 @@ def p122_main():
     print(json.dumps({k: v for k, v in out.items() if k not in ("rest",)}, indent=1)[:2000])
 
 
+REF123 = "/root/reference/parafem/examples/5th_ed/p123/book/p123"
+
+
+def p123_main():
+    mg = [line.split() for line in open(REF123 + ".mg").read().splitlines() if line.strip()]
+    res = open(REF123 + ".res").read().splitlines()
+    k = next(i for i, line in enumerate(res) if line.strip().startswith("Freedom"))
+    out = {
+        "source": "examples/5th_ed/p123/book/p123.{mg,res}",
+        "mg": {"nels": int(mg[1][1]), "nxe": int(mg[1][2]), "nze": int(mg[1][3]), "nip": int(mg[1][4]),
+               "aa": float(mg[2][0]), "bb": float(mg[2][1]), "cc": float(mg[2][2]),
+               "kx": float(mg[2][3]), "ky": float(mg[2][4]), "kz": float(mg[2][5]),
+               "tol": float(mg[3][0]), "limit": int(mg[3][1]),
+               "loaded_freedoms": int(mg[4][0]), "fixed_freedoms": int(mg[4][1])},
+        "res_processes": int(res[0].split()[4]),
+        "res_nn": int(res[1].split()[2]),
+        "res_nr": int(res[1].split()[4]),
+        "res_neq": int(res[1].split()[7]),
+        "res_iters": int(res[3].split()[-1]),
+        "res_total_load": res[4].split()[-1],
+        "res_potentials": [[int(line.split()[0]), line.split()[1]] for line in res[k + 1:k + 5]],
+    }
+    with open(os.path.join(HERE, "p123_book.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     d = M.read_deck(REF)
     res = open(REF + ".res").read().splitlines()
@@ -167,3 +198,4 @@ if __name__ == "__main__":
     main()
     p124_main()
     p122_main()
+    p123_main()

@@ -884,3 +884,21 @@ def test_context_reuse_and_trim():
     finally:
         c2.close()
     assert np.array_equal(xa1, xa2) and np.array_equal(xb1, xb2)
+
+
+def test_p123_book_example_full_size(ctx, golden):
+    """examples/5th_ed/p123/book/p123.res on the device: the book's 200^3
+    conduction cube (8,000,000 bricks and equations) rebuilt from its
+    p123.mg, element matrices assembled on the device, FP64 EBE-PCG with
+    checon at 1e-5: 180 iterations and the four potentials exactly as the
+    reference printed them (E12.4)."""
+    with open(os.path.join(golden, "p123_book.json")) as fh:
+        meta = json.load(fh)
+    mg = meta["mg"]
+    pr = P.p123_book(mg["nxe"], mg["nels"] // mg["nxe"] // mg["nze"], mg["nze"], mg["aa"], mg["kx"])
+    say(f"p123 book nels={pr.nels} neq={pr.neq}")
+    assert pr.neq == meta["res_neq"]
+    x, res = P.solve(ctx, pr, mg["tol"], mg["limit"], stop="checon")
+    assert res.converged and res.iters == meta["res_iters"]
+    for eq, printed in meta["res_potentials"]:
+        assert M.fortran_e(x[eq - 1], 12, 4).strip() == printed, (eq, x[eq - 1], printed)

@@ -442,3 +442,24 @@ def test_demo_fixture(golden):
     pub = np.array(meta["res_sigma_elem1"])
     punit = 10.0 ** (np.floor(np.log10(np.abs(pub))) - 3)
     assert np.all(np.abs(sig - pub) <= 0.5 * punit * 1.001 + 1e-5 * np.abs(pub).max()), (sig, pub)
+
+
+def test_p123_book_example_full_size(golden):
+    """examples/5th_ed/p123/book/p123.res: the book's 200^3 conduction cube
+    (8,000,000 bricks, 8,120,601 nodes, 120,601 restrained, 8,000,000
+    equations), rebuilt from its p123.mg with this repository's generator,
+    solved by the oracle's FP64 EBE-PCG with checon at 1e-5: the same 180
+    iterations, total load and the four potentials as printed (E12.4).  The
+    published run used 16 processes; at this tolerance the partial-sum
+    order does not move the stopping iteration."""
+    with open(os.path.join(golden, "p123_book.json")) as fh:
+        meta = json.load(fh)
+    mg = meta["mg"]
+    pr = P.p123_book(mg["nxe"], mg["nels"] // mg["nxe"] // mg["nze"], mg["nze"], mg["aa"], mg["kx"])
+    assert (pr.mesh.nn, len(pr.info["rest"]), pr.neq) == (meta["res_nn"], meta["res_nr"], meta["res_neq"])
+    assert M.fortran_e(O.sum_p(pr.r0), 12, 4).strip() == meta["res_total_load"]
+    K = O.assemble_laplace(pr.g_coord(), mg["kx"], mg["ky"], mg["kz"], None, nthreads=8)
+    x, it, conv = O.pcg(pr.g_g, K, pr.r0, O.diag(pr.g_g, K, pr.neq), mg["tol"], mg["limit"], nthreads=8)
+    assert conv and it == meta["res_iters"]
+    for eq, printed in meta["res_potentials"]:
+        assert M.fortran_e(x[eq - 1], 12, 4).strip() == printed, (eq, x[eq - 1], printed)
