@@ -2,8 +2,8 @@
 
 ``python -m parafem_amd.build`` compiles every ``csrc/*.hip`` with hipcc for
 gfx950 and links them into ``parafem_amd/libparafem_amd.so``, then builds the
-Fortran binding (``fortran/pf_gpu.f90``) and the p121 driver
-(``bin/pf_p121``) with amdflang.  It always recompiles (the caller decides
+Fortran binding (``fortran/pf_gpu.f90``) and the p121, p122, p123 and p124
+drivers (``bin/pf_p12*``) with amdflang.  It always recompiles (the caller decides
 when to build).
 """
 from __future__ import annotations
@@ -22,6 +22,7 @@ OBJDIR = os.path.join(HERE, "_build")
 FSRC = os.path.join(HERE, "fortran")
 P121 = os.path.join(HERE, "bin", "pf_p121")
 P122 = os.path.join(HERE, "bin", "pf_p122")
+P123 = os.path.join(HERE, "bin", "pf_p123")
 P124 = os.path.join(HERE, "bin", "pf_p124")
 FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 
@@ -56,7 +57,8 @@ def build_fortran(verbose: bool = False) -> str | None:
     moddir = os.path.join(OBJDIR, "fmod")
     os.makedirs(moddir, exist_ok=True)
     os.makedirs(os.path.dirname(P121), exist_ok=True)
-    for exe, src in ((P121, "pf_p121.f90"), (P122, "pf_p122.f90"), (P124, "pf_p124.f90")):
+    for exe, src in ((P121, "pf_p121.f90"), (P122, "pf_p122.f90"), (P123, "pf_p123.f90"),
+                     (P124, "pf_p124.f90")):
         tmp = exe + ".tmp"
         cmd = [FLANG, "-O2", "-J", moddir, os.path.join(FSRC, "pf_gpu.f90"),
                os.path.join(FSRC, src), "-L", HERE, "-lparafem_amd",

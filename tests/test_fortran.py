@@ -209,3 +209,69 @@ def test_fortran_p122_demo(tmp_path, golden):
     assert np.abs(x - totd).max() <= 1e-9 * np.abs(totd).max()
     lines = open(base + ".pf.ensi.DISPL-000010").read().split("\n")
     assert lines[:4] == ["Alya Ensight Gold --- Vector per-node variable file", "part", "    1", "coordinates"]
+
+
+EXE123 = os.path.join(ROOT, "parafem_amd", "bin", "pf_p123")
+
+
+def _p123_deck(tmp_path):
+    """A p123 deck as p12meshgen's p123 case lays it out (GEOMETRY_8BXZ
+    mesh, BOX_BC8 restraints, the load 10 on equation nres) on a 24 x 16 x
+    20 box, plus one fixed freedom to exercise the penalty rows."""
+    pr = P.p123_book(24, 16, 20, 0.005, 2.0)
+    nres = pr.info["nres"]
+    node = int(np.nonzero(pr.nf[:, 0] == pr.neq // 3)[0][0]) + 1
+    base = str(tmp_path / "p123_box")
+    M.write_p123_deck(base, pr.mesh, pr.info["rest"], 2.0, 2.0, 2.0, 1e-5, 2000, nres,
+                      loads=[(nres, 10.0)], fix=[(node, 1, 5.0)])
+    return pr, base, node
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_fortran_p123_deck(tmp_path):
+    """pf_p123 on a p123 deck: job data, iteration count, total load and the
+    four potentials print as p123 prints them, xnew equals the oracle's in
+    the device's dot order bit for bit, the NDPTL file holds the oracle's
+    nodal averages; the mixed mode reaches relres 1e-10 within 1e-8 of the
+    FP64 solution."""
+    sys.path.insert(0, os.path.dirname(__file__))
+    import oracle as O
+    pr, base, node = _p123_deck(tmp_path)
+    say(f"pf_p123 deck neq={pr.neq}")
+    r = subprocess.run([EXE123, os.path.basename(base)], cwd=os.path.dirname(base), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, f"pf_p123 failed ({r.returncode}):\n{r.stdout}\n{r.stderr}"
+    res = open(base + ".pf.res").read().splitlines()
+    d = M.read_deck(base, "p123")
+    nf, neq = M.steering(d["nn"], 1, d["rest"])
+    g_g = M.g_g_from(nf, d["g_num"])
+    assert neq == pr.neq and np.array_equal(g_g, pr.g_g)
+    assert res[1] == f"There are {d['nn']:12d} nodes{d['nr']:12d} restrained and   {neq:12d} equations"
+    K = O.assemble_laplace(M.element_coords(d["g_num"], d["coords"]), d["kx"], d["ky"], d["kz"])
+    dg = O.diag(g_g, K, neq)
+    no_f = nf[d["fix_node"] - 1, 0]
+    store = dg[no_f - 1] + 1.0e20
+    dg[no_f - 1] = store
+    rhs = np.zeros(neq)
+    rhs[d["load_node"] - 1] = d["load_val"][:, 0]
+    q = rhs.sum()
+    rhs[no_f - 1] = store * d["fix_val"]
+    xo, it, conv = O.pcg(g_g, K, rhs, dg, d["tol"], d["limit"], no_f=no_f, store=store,
+                         dot_order=O.DOT_DEVICE)
+    x = np.fromfile(base + ".pf.x", dtype=np.float64)
+    assert conv and np.array_equal(x, xo)
+    assert res[3] == f"The number of iterations to convergence was {it:5d}"
+    assert res[4] == "The total load is " + M.fortran_e(q, 12, 4)
+    nres = d["nres"]
+    assert res[7:11] == [f"{nres + i:8d}     " + M.fortran_e(xo[nres + i - 1], 12, 4) for i in range(4)]
+    lines = open(base + ".pf.ensi.NDPTL-000001").read().split("\n")
+    assert lines[:4] == ["Alya Ensight Gold --- Scalar per-node variable file", "part", "    1", "coordinates"]
+    nodal = O.scatter_nodes(d["g_num"], O.gather(g_g, xo).reshape(d["nels"], 8, 1), d["nn"], 1)[:, 0]
+    assert lines[4:-1] == [M.fortran_e(v, 12, 5) for v in nodal]
+    r = subprocess.run([EXE123, os.path.basename(base), "mixed", "1e-10"], cwd=os.path.dirname(base),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    xm = np.fromfile(base + ".pf.x", dtype=np.float64)
+    x64, _, conv64 = O.pcg(g_g, K, rhs, dg, 1e-12, 20000, stop=O.STOP_RELRES, no_f=no_f, store=store)
+    assert conv64 and np.abs(xm - x64).max() <= 1e-8 * np.abs(x64).max()

@@ -463,3 +463,25 @@ def test_p123_book_example_full_size(golden):
     assert conv and it == meta["res_iters"]
     for eq, printed in meta["res_potentials"]:
         assert M.fortran_e(x[eq - 1], 12, 4).strip() == printed, (eq, x[eq - 1], printed)
+
+
+def test_p123_deck_written_and_read_back(tmp_path):
+    """write_p123_deck lays a p123 deck out as p12meshgen's p123 case does;
+    read_deck (READ_P123 and friends) recovers the mesh in S&G order, the
+    BOX_BC8 restraints, the loaded equation and the fixed freedom."""
+    pr = P.p123_book(6, 4, 5, 0.005, 2.0)
+    nres = pr.info["nres"]
+    base = str(tmp_path / "deck")
+    M.write_p123_deck(base, pr.mesh, pr.info["rest"], 2.0, 1.5, 1.0, 1e-5, 500, nres,
+                      loads=[(nres, 10.0)], fix=[(7, 1, 2.5)])
+    d = M.read_deck(base, "p123")
+    assert (d["nels"], d["nn"], d["nr"], d["nod"], d["meshgen"]) == (120, 210, 90, 8, 2)
+    assert (d["kx"], d["ky"], d["kz"], d["tol"], d["limit"], d["nres"]) == (2.0, 1.5, 1.0, 1e-5, 500, nres)
+    assert np.array_equal(d["g_num"], pr.mesh.g_num) and np.array_equal(d["coords"], pr.mesh.coords)
+    assert np.array_equal(d["rest"], pr.info["rest"])
+    assert d["load_node"].tolist() == [nres] and d["load_val"].tolist() == [[10.0]]
+    assert (d["fix_node"].tolist(), d["fix_sense"].tolist(), d["fix_val"].tolist()) == ([7], [1], [2.5])
+    # BOX_BC8 on 6 x 4 x 5: each of the 4 layers restrains its z = 0 row
+    # and its x = max column, then the whole last layer -- p12meshgen's
+    # nr = (nxe+1)(nye+1) + (nxe+1) nze + nye nze
+    assert len(pr.info["rest"]) == 4 * (6 + 5 + 1) + 7 * 6 == 7 * 5 + 7 * 5 + 4 * 5
