@@ -119,6 +119,28 @@ def laplace_cube(n: int, lognormal_k: bool = True, seed: int = 2, flux: float = 
                    fix_val=np.zeros(fix_eq.size))
 
 
+def p121_mg(nels: int, nxe: int, nze: int, aa: float, bb: float, cc: float, e: float = 100.0,
+            v: float = 0.3) -> Problem:
+    """A p121 model as p12meshgen's p121 case builds it for 20-node bricks
+    (``src/tools/preprocessing/p12meshgen/p12meshgen.f90:118-174``):
+    nye = nels / nxe / nze, GEOMETRY_20BXZ mesh, CUBE_BC20 restraints and
+    LOAD_P121's pressure patch (nle = nxe / 5) scaled by
+    -aa bb 25/12.  The reference's demo (``examples/5th_ed/p121/demo``) is
+    p121_mg(8000, 20, 20, .5, .5, .5) and its book run
+    (``examples/5th_ed/p121/book``) p121_mg(64000, 40, 40, .25, .25, .25)."""
+    nye = nels // nxe // nze
+    m = M.box_hex20(nxe, nye, nze, aa, bb, cc)
+    rest = M.cube_bc20(nxe, nye, nze)
+    node, val = M.load_p121(nxe // 5, 20, nxe, nze)
+    val = -np.asarray(val) * aa * bb * (25.0 / 12.0)
+    nf, neq = M.steering(m.nn, 3, rest)
+    g_g = M.g_g_from(nf, m.g_num)
+    vals = np.zeros((len(node), 3))
+    vals[:, 2] = val
+    r0 = M.nodal_loads_to_rhs(nf, np.asarray(node), vals, neq)
+    return Problem("elastic", m, g_g, neq, r0, nf, e=e, v=v, info={"rest": rest, "load_node": node})
+
+
 def p123_book(nxe: int = 200, nye: int = 200, nze: int = 200, h: float = 0.005, k: float = 2.0,
               load: float = 10.0) -> Problem:
     """The p123 deck of the reference's book example

@@ -23,6 +23,10 @@ Writes (data only, no reference source):
                            the per-increment results its .res prints;
   p122_demo_mesh.npz    -- the demo's nodal coordinates and element nodes
                            (.d; a graded mesh no generator here reproduces).
+  p121_book.json        -- the generator input of examples/5th_ed/p121/book
+                           (p121.mg) and what its p121.res prints (nodes,
+                           restrained, equations, iterations, the central
+                           displacement, element 1's centroid stresses);
   p123_book.json        -- the generator input of examples/5th_ed/p123/book
                            (p123.mg) and what its p123.res prints (nodes,
                            restrained, equations, iterations, total load,
@@ -165,6 +169,31 @@ def p123_main():
     print(json.dumps(out, indent=1))
 
 
+REF121B = "/root/reference/parafem/examples/5th_ed/p121/book/p121"
+
+
+def p121_book_main():
+    mg = " ".join(line for line in open(REF121B + ".mg").read().splitlines()[1:]).split()
+    res = open(REF121B + ".res").read().splitlines()
+    k = next(i for i, line in enumerate(res) if line.startswith("The Centroid point stresses"))
+    out = {
+        "source": "examples/5th_ed/p121/book/p121.{mg,res}",
+        "mg": {"nels": int(mg[1]), "nxe": int(mg[2]), "nze": int(mg[3]), "nod": int(mg[4]), "nip": int(mg[5]),
+               "aa": float(mg[6]), "bb": float(mg[7]), "cc": float(mg[8]), "e": float(mg[9]), "v": float(mg[10]),
+               "tol": float(mg[11]), "limit": int(mg[12])},
+        "res_processes": int(res[0].split()[4]),
+        "res_nn": int(res[1].split()[2]),
+        "res_nr": int(res[1].split()[4]),
+        "res_neq": int(res[1].split()[7]),
+        "res_iters": int(next(line for line in res if "iterations to convergence" in line).split()[-1]),
+        "res_x1": next(line for line in res if "central nodal displacement" in line).split()[-1],
+        "res_sigma_elem1": res[k + 2].split(),
+    }
+    with open(os.path.join(HERE, "p121_book.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     d = M.read_deck(REF)
     res = open(REF + ".res").read().splitlines()
@@ -199,3 +228,4 @@ if __name__ == "__main__":
     p124_main()
     p122_main()
     p123_main()
+    p121_book_main()

@@ -485,3 +485,24 @@ def test_p123_deck_written_and_read_back(tmp_path):
     # and its x = max column, then the whole last layer -- p12meshgen's
     # nr = (nxe+1)(nye+1) + (nxe+1) nze + nye nze
     assert len(pr.info["rest"]) == 4 * (6 + 5 + 1) + 7 * 6 == 7 * 5 + 7 * 5 + 4 * 5
+
+
+def test_p121_book_example(golden):
+    """examples/5th_ed/p121/book/p121.res: the book's 40^3 cube of 20-node
+    bricks (64,000 elements, 270,641 nodes, 24,161 restrained, 777,520
+    equations) rebuilt from its p121.mg with this repository's generator;
+    the oracle's FP64 EBE-PCG (checon 1e-5) stops after the published 569
+    iterations with the published central displacement.  (The element-1
+    stresses of that 2013 run match no element of this mesh -- sigma_zz
+    agrees, sigma_xx = sigma_yy is -17.58 here against -16.57 -- while the
+    current reference code's p121 demo stresses are reproduced by
+    test_demo_fixture; they are not compared.)"""
+    with open(os.path.join(golden, "p121_book.json")) as fh:
+        meta = json.load(fh)
+    mg = meta["mg"]
+    pr = P.p121_mg(mg["nels"], mg["nxe"], mg["nze"], mg["aa"], mg["bb"], mg["cc"], mg["e"], mg["v"])
+    assert (pr.mesh.nn, len(pr.info["rest"]), pr.neq) == (meta["res_nn"], meta["res_nr"], meta["res_neq"])
+    K = O.assemble_elastic(pr.g_coord(), mg["e"], mg["v"], None, nthreads=8)
+    x, it, conv = O.pcg(pr.g_g, K, pr.r0, O.diag(pr.g_g, K, pr.neq), mg["tol"], mg["limit"], nthreads=8)
+    assert conv and it == meta["res_iters"]
+    assert M.fortran_e(x[0], 12, 4).strip() == meta["res_x1"]
