@@ -37,7 +37,8 @@ Pinning status (how each function is checked in ``tests/test_oracle.py``):
       conserved, and the p121 demo's printed displacement field;
   pcg -- the reference's committed fixtures (xx3-tiny: 1640 equations, 79
       iterations, displacements; p121 demo: 98360 equations, total load
-      -100, 295 iterations, displacement field; p123 book: 8,000,000
+      -100, 295 iterations, displacement field; p121 book: 777,520
+      equations, 569 iterations, x(1); p123 book: 8,000,000
       equations, 180 iterations, the four printed potentials, with
       laplace_kc on the full 200^3 model) and scipy direct solves.
 """
