@@ -168,11 +168,21 @@ __global__ __launch_bounds__(256) void k_blk_scatter(const int *__restrict__ ptr
     const int cl = threadIdx.x & 15, rl = threadIdx.x >> 4;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int q = blockIdx.x * 16 + rl; q < neq; q += gridDim.x * 16) {
+        // the slot indices of up to 8 slots, then their rows, are loaded
+        // before the sums (in slot order); longer lists continue in a loop
+        const int a = ptr[q], b = ptr[q + 1];
+        int sl[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sl[j] = a + j < b ? slot[a + j] : -1;
+        T4 t[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            t[j] = sl[j] >= 0 ? *reinterpret_cast<const T4 *>(Ut + (long)sl[j] * PF_BLK + 4 * cl) : T4{0, 0, 0, 0};
         T4 s = T4{0, 0, 0, 0};
-        for (int k = ptr[q]; k < ptr[q + 1]; ++k) {
-            const T4 t = *reinterpret_cast<const T4 *>(Ut + (long)slot[k] * PF_BLK + 4 * cl);
-            s = s + t;
-        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (sl[j] >= 0) s = s + t[j];
+        for (int k = a + 8; k < b; ++k) s = s + *reinterpret_cast<const T4 *>(Ut + (long)slot[k] * PF_BLK + 4 * cl);
         *reinterpret_cast<T4 *>(U + (long)(q + 1) * PF_BLK + 4 * cl) = s;
         if (DOT) {
             const T4 p = *reinterpret_cast<const T4 *>(P + (long)(q + 1) * PF_BLK + 4 * cl);
@@ -237,7 +247,9 @@ __global__ __launch_bounds__(256) void k_blk_update(float *__restrict__ X, float
     float a[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) a[k] = (float)st->alpha[4 * cl + k];
-    for (int q = blockIdx.x * 16 + rl; q < neq; q += gridDim.x * 16) {
+    const int qc = (neq + (int)gridDim.x - 1) / (int)gridDim.x;   // rows per block, one contiguous range
+    const int qe = ((int)blockIdx.x + 1) * qc < neq ? ((int)blockIdx.x + 1) * qc : neq;
+    for (int q = (int)blockIdx.x * qc + rl; q < qe; q += 16) {
         const long o = (long)(q + 1) * PF_BLK + 4 * cl;
         f4_t x = *reinterpret_cast<const f4_t *>(X + o), r = *reinterpret_cast<const f4_t *>(R + o);
         const f4_t p = *reinterpret_cast<const f4_t *>(P + o), u = *reinterpret_cast<const f4_t *>(U + o);
@@ -298,7 +310,9 @@ __global__ __launch_bounds__(256) void k_blk_pupdate(float *__restrict__ P, cons
     float b[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) b[k] = (float)st->beta[4 * cl + k];
-    for (int q = blockIdx.x * 16 + rl; q < neq; q += gridDim.x * 16) {
+    const int qc = (neq + (int)gridDim.x - 1) / (int)gridDim.x;   // rows per block, one contiguous range
+    const int qe = ((int)blockIdx.x + 1) * qc < neq ? ((int)blockIdx.x + 1) * qc : neq;
+    for (int q = (int)blockIdx.x * qc + rl; q < qe; q += 16) {
         const long o = (long)(q + 1) * PF_BLK + 4 * cl;
         f4_t p = *reinterpret_cast<const f4_t *>(P + o);
         const f4_t r = *reinterpret_cast<const f4_t *>(R + o);
@@ -319,7 +333,9 @@ __global__ __launch_bounds__(256) void k_blk_inner_init(float *__restrict__ X, f
 {
     const int cl = threadIdx.x & 15, rl = threadIdx.x >> 4;
     double rd[4] = {0, 0, 0, 0}, rr[4] = {0, 0, 0, 0};
-    for (int q = blockIdx.x * 16 + rl; q < neq; q += gridDim.x * 16) {
+    const int qc = (neq + (int)gridDim.x - 1) / (int)gridDim.x;   // rows per block, one contiguous range
+    const int qe = ((int)blockIdx.x + 1) * qc < neq ? ((int)blockIdx.x + 1) * qc : neq;
+    for (int q = (int)blockIdx.x * qc + rl; q < qe; q += 16) {
         const long o = (long)(q + 1) * PF_BLK + 4 * cl;
         const float di = dinv[q + 1];
         f4_t r, d;
@@ -372,7 +388,9 @@ __global__ __launch_bounds__(256) void k_blk_residual(double *__restrict__ R64, 
 {
     const int cl = threadIdx.x & 15, rl = threadIdx.x >> 4;
     double rr[4] = {0, 0, 0, 0};
-    for (int q = blockIdx.x * 16 + rl; q < neq; q += gridDim.x * 16) {
+    const int qc = (neq + (int)gridDim.x - 1) / (int)gridDim.x;   // rows per block, one contiguous range
+    const int qe = ((int)blockIdx.x + 1) * qc < neq ? ((int)blockIdx.x + 1) * qc : neq;
+    for (int q = (int)blockIdx.x * qc + rl; q < qe; q += 16) {
         const long o = (long)(q + 1) * PF_BLK + 4 * cl;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
