@@ -46,3 +46,20 @@ def test_no_cpu_fallback_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(api, "_lib", None)
     with pytest.raises(api.LibraryNotBuilt):
         api.Context(0)
+
+
+def test_bench_byte_counts_match_the_storage_layouts():
+    """bench.py's algorithmic bytes per element: FP64 K_e in full, the packed
+    symmetric FP32 layout's ntot(ntot+1)/2 entries padded to a multiple of 4
+    (1832 for hex20, 300 for hex8 elastic, 36 for hex8 Laplace), twice for
+    the split precision, plus the int32 steering."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    assert bench.ebe_bytes(60, "fp64") == 60 * 60 * 8 + 240
+    assert bench.ebe_bytes(60, "fp32", "mixed") == 1832 * 4 + 240
+    assert bench.ebe_bytes(60, "fp32", "split") == 2 * 1832 * 4 + 240
+    assert bench.ebe_bytes(24, "fp32", "mixed") == 300 * 4 + 96
+    assert bench.ebe_bytes(8, "fp32", "mixed") == 36 * 4 + 32
+    assert bench.iteration_bytes(10, 8, 7, "fp32") == 10 * (36 * 4 + 32) + 15 * 7 * 8
