@@ -173,7 +173,9 @@ __device__ __forceinline__ double bpermute64(int addr, double v)
 // forms sum_d K(i,i+d) p(i+d) from p staged twice in LDS, then receives
 // K(i-d,i) p(i) from lane i-d (mod NT) for d = 1..NT/2-1 by ds_bpermute,
 // where lane j sends K(j,j+d) p(j): every entry is read from HBM once (the
-// d = NT/2 entries, shared by rows i and i + NT/2, are stored once).
+// d = NT/2 entries, shared by rows i and i + NT/2, are stored once).  The
+// products go out with non-temporal stores (they are read back by the
+// scatter, not by this kernel): 2% faster FP64 products, 0.5-1% FP32.
 template <int NT, int KIND, bool GATHER, typename OUT, bool LOOP = false, int EPLT = Lanes<NT, KIND>::EPL>
 __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
                                                  const float *__restrict__ Klo, const int *__restrict__ gg,
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
                 acc = __fma_rn(kc[t][j].x, pw[2 * j], acc);
                 acc = __fma_rn(kc[t][j].y, pw[2 * j + 1], acc);
             }
-            if (ev[t] < nels && li < NT) out[ev[t] * NT + li] = (OUT)acc;
+            if (ev[t] < nels && li < NT) __builtin_nontemporal_store((OUT)acc, &out[ev[t] * NT + li]);
         }
     } else if constexpr (KIND == KIND_F32) {
         float kd[EPL][D];
@@ -272,7 +274,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
                 const float tt = __fmul_rn(kd[t][d], pv[t]);
                 acc = __fadd_rn(acc, __int_as_float(__builtin_amdgcn_ds_bpermute(from << 2, __float_as_int(tt))));
             }
-            if (ev[t] < nels && li < NT) out[ev[t] * NT + li] = (OUT)acc;
+            if (ev[t] < nels && li < NT) __builtin_nontemporal_store((OUT)acc, &out[ev[t] * NT + li]);
         }
     } else {
         constexpr bool LO = KIND == KIND_ACC;
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
                 const int from = gb + (i >= d ? i - d : i - d + NT);
                 acc = __dadd_rn(acc, bpermute64(from << 2, __dmul_rn(kd[t][d], pv[t])));
             }
-            if (ev[t] < nels && li < NT) out[ev[t] * NT + li] = (OUT)acc;
+            if (ev[t] < nels && li < NT) __builtin_nontemporal_store((OUT)acc, &out[ev[t] * NT + li]);
         }
     }
     };
