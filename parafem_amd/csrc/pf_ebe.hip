@@ -607,7 +607,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_pupdate(double *__restrict__ p, fl
     const int stride = gridDim.x * PF_BLOCK;
     for (int q = blockIdx.x * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
         const double pq = __dadd_rn(d[q], __dmul_rn(p[q], beta));
-        p[q] = pq;
+        __builtin_nontemporal_store(pq, &p[q]);   // read next by the gathers, not from this L2 line
         if (p32) p32[q] = (float)pq;
     }
 }
