@@ -124,9 +124,9 @@ __global__ __launch_bounds__(256) void k_blk_ebe(const T *__restrict__ Kt, long 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = 16 * (m0 + m) + (sizeof(T) == 4 ? 4 * rq + v : 4 * v + rq);
-                if (m0 + m < TL::MT && row < NT)
-                    *reinterpret_cast<T4 *>(Ut + (e * NT + row) * PF_BLK + 4 * cl) =
-                        T4{acc[m][0][v], acc[m][1][v], acc[m][2][v], acc[m][3][v]};
+                if (m0 + m < TL::MT && row < NT)   // read back by the scatter only: non-temporal
+                    __builtin_nontemporal_store(T4{acc[m][0][v], acc[m][1][v], acc[m][2][v], acc[m][3][v]},
+                                                reinterpret_cast<T4 *>(Ut + (e * NT + row) * PF_BLK + 4 * cl));
             }
     }
 }
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(256) void k_blk_pupdate(float *__restrict__ P, cons
         const float di = dinv[q + 1];
 #pragma unroll
         for (int k = 0; k < 4; ++k) p[k] = __fmaf_rn(b[k], p[k], di * r[k]);
-        *reinterpret_cast<f4_t *>(P + o) = p;
+        __builtin_nontemporal_store(p, reinterpret_cast<f4_t *>(P + o));
     }
 }
 
