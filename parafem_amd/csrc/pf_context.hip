@@ -170,6 +170,7 @@ static void reset_problem(pf_ctx *c)
     free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
     free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
     free_arr(c, c->csr_slot, (size_t)c->nnz);
+    free_arr(c, c->csr_slot_t, (size_t)c->nnz);
     free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
     free_arr(c, c->utemp32, (size_t)c->ntot * c->nels);
     c->nod = c->nels = c->ntot = c->neq = 0;
@@ -193,6 +194,7 @@ int pf_steer_install(pf_ctx *c, int ntot, int nels, int neq)
     free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
     free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
     free_arr(c, c->csr_slot, (size_t)c->nnz);
+    free_arr(c, c->csr_slot_t, (size_t)c->nnz);
     free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
     free_arr(c, c->utemp32, (size_t)c->ntot * c->nels);
     c->ntot = ntot;
@@ -223,6 +225,7 @@ void pf_steer_drop(pf_ctx *c)
     free_arr(c, c->g_g, (size_t)c->ntot * c->nels);
     free_arr(c, c->csr_ptr, (size_t)c->neq + 1);
     free_arr(c, c->csr_slot, (size_t)c->nnz);
+    free_arr(c, c->csr_slot_t, (size_t)c->nnz);
     free_arr(c, c->utemp, (size_t)c->ntot * c->nels);
     free_arr(c, c->utemp32, (size_t)c->ntot * c->nels);
     c->ntot = c->neq = 0;

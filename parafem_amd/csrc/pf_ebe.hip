@@ -181,7 +181,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
                                                  const float *__restrict__ Klo, const int *__restrict__ gg,
                                                  const double *__restrict__ src, const float *__restrict__ src32,
                                                  OUT *__restrict__ out, int nels, long kstride,
-                                                 const pf_state *__restrict__ st, int gate, long nlb)
+                                                 const pf_state *__restrict__ st, int gate, long nlb, int rowmajor)
 {
     if (skip(st, gate)) return;
     using L = Lanes<NT, KIND>;
@@ -231,7 +231,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
                 acc = __fma_rn(kc[t][j].x, pw[2 * j], acc);
                 acc = __fma_rn(kc[t][j].y, pw[2 * j + 1], acc);
             }
-            if (ev[t] < nels && li < NT) __builtin_nontemporal_store((OUT)acc, &out[ev[t] * NT + li]);
+            if (ev[t] < nels && li < NT) {
+                if (NT == 8 && rowmajor)
+                    out[(long)li * nels + ev[t]] = (OUT)acc;
+                else
+                    __builtin_nontemporal_store((OUT)acc, &out[ev[t] * NT + li]);
+            }
         }
     } else if constexpr (KIND == KIND_F32) {
         float kd[EPL][D];
@@ -274,7 +279,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
                 const float tt = __fmul_rn(kd[t][d], pv[t]);
                 acc = __fadd_rn(acc, __int_as_float(__builtin_amdgcn_ds_bpermute(from << 2, __float_as_int(tt))));
             }
-            if (ev[t] < nels && li < NT) __builtin_nontemporal_store((OUT)acc, &out[ev[t] * NT + li]);
+            if (ev[t] < nels && li < NT) {
+                if (NT == 8 && rowmajor)
+                    out[(long)li * nels + ev[t]] = (OUT)acc;
+                else
+                    __builtin_nontemporal_store((OUT)acc, &out[ev[t] * NT + li]);
+            }
         }
     } else {
         constexpr bool LO = KIND == KIND_ACC;
@@ -331,7 +341,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
                 const int from = gb + (i >= d ? i - d : i - d + NT);
                 acc = __dadd_rn(acc, bpermute64(from << 2, __dmul_rn(kd[t][d], pv[t])));
             }
-            if (ev[t] < nels && li < NT) __builtin_nontemporal_store((OUT)acc, &out[ev[t] * NT + li]);
+            if (ev[t] < nels && li < NT) {
+                if (NT == 8 && rowmajor)
+                    out[(long)li * nels + ev[t]] = (OUT)acc;
+                else
+                    __builtin_nontemporal_store((OUT)acc, &out[ev[t] * NT + li]);
+            }
         }
     }
     };
@@ -771,18 +786,18 @@ __global__ void k_scatter_fix(const int *__restrict__ ptr, const int *__restrict
 
 template <int NT, int KIND, bool GATHER, typename OUT, int EPL>
 void launch_ebe_epl(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *K64,
-                    const float *src32)
+                    const float *src32, int rowmajor)
 {
     constexpr int EPB = Lanes<NT, KIND>::EPW * EPL * (PF_BLOCK / 64);
     const long nlb = (c->nels + EPB - 1) / EPB;
     hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, false, EPL>), dim3((int)nlb), dim3(PF_BLOCK), 0, c->stream,
                        K64, c->K32hi, c->K32lo, c->g_g, src, src32, out, c->nels, pf_k64_estride(c), st, gate,
-                       nlb);
+                       nlb, rowmajor);
 }
 
 template <int NT, int KIND, bool GATHER, typename OUT>
 int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *Kalt,
-                  const float *src32)
+                  const float *src32, int rowmajor)
 {
     const double *K64 = Kalt ? Kalt : c->K64;
     if (gate == RUN_REFINE) {
@@ -791,20 +806,20 @@ int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, in
         const int grid = (int)std::min<long>(nlb, 2048);
         hipLaunchKernelGGL((k_ebe<NT, KIND, GATHER, OUT, true>), dim3(grid), dim3(PF_BLOCK), 0, c->stream, K64,
                            c->K32hi, c->K32lo, c->g_g, src, src32, out, c->nels, pf_k64_estride(c), st, gate,
-                           nlb);
+                           nlb, rowmajor);
     } else {
         // the tuned shape of the iteration's product (an out-of-range value,
         // e.g. from before the mesh changed, keeps the built-in one)
         constexpr int E0 = Lanes<NT, KIND>::EPL;
         const int e = GATHER && c->tune_epl <= pf_epl_max(NT) ? c->tune_epl : 0;
         if constexpr (GATHER) {
-            if (e == 1) launch_ebe_epl<NT, KIND, GATHER, OUT, 1>(c, src, out, st, gate, K64, src32);
-            else if (e == 2) launch_ebe_epl<NT, KIND, GATHER, OUT, 2>(c, src, out, st, gate, K64, src32);
-            else if (e == 4) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 24 ? 4 : E0)>(c, src, out, st, gate, K64, src32);
-            else if (e == 8) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 8 ? 8 : E0)>(c, src, out, st, gate, K64, src32);
-            else launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate, K64, src32);
+            if (e == 1) launch_ebe_epl<NT, KIND, GATHER, OUT, 1>(c, src, out, st, gate, K64, src32, rowmajor);
+            else if (e == 2) launch_ebe_epl<NT, KIND, GATHER, OUT, 2>(c, src, out, st, gate, K64, src32, rowmajor);
+            else if (e == 4) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 24 ? 4 : E0)>(c, src, out, st, gate, K64, src32, rowmajor);
+            else if (e == 8) launch_ebe_epl<NT, KIND, GATHER, OUT, (NT <= 8 ? 8 : E0)>(c, src, out, st, gate, K64, src32, rowmajor);
+            else launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate, K64, src32, rowmajor);
         } else {
-            launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate, K64, src32);
+            launch_ebe_epl<NT, KIND, GATHER, OUT, E0>(c, src, out, st, gate, K64, src32, rowmajor);
         }
     }
     PF_HIP(hipGetLastError());
@@ -813,12 +828,12 @@ int launch_ebe_nt(pf_ctx *c, const double *src, OUT *out, const pf_state *st, in
 
 template <int KIND, bool GATHER, typename OUT>
 int launch_ebe(pf_ctx *c, const double *src, OUT *out, const pf_state *st, int gate, const double *Kalt = nullptr,
-               const float *src32 = nullptr)
+               const float *src32 = nullptr, int rowmajor = 0)
 {
     switch (c->ntot) {
-    case 60: return launch_ebe_nt<60, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt, src32);
-    case 24: return launch_ebe_nt<24, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt, src32);
-    case 8: return launch_ebe_nt<8, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt, src32);
+    case 60: return launch_ebe_nt<60, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt, src32, rowmajor);
+    case 24: return launch_ebe_nt<24, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt, src32, rowmajor);
+    case 8: return launch_ebe_nt<8, KIND, GATHER, OUT>(c, src, out, st, gate, Kalt, src32, rowmajor);
     default: return PF_ERR_ARG;
     }
 }
@@ -1014,30 +1029,37 @@ int pf_pcg_run(pf_ctx *c, int n)
     const bool wide = c->opts.precision == PF_PREC_MIXED64;
     const bool split = c->opts.precision == PF_PREC_SPLIT;
     float *p32 = mixed && c->ntot <= 24 ? c->p32 : nullptr;
+    // hex8 with FP64 products: row-major (row i of every element side by
+    // side, read back through the matching slot list; the scatter's reads of
+    // consecutive equations then fall on consecutive words).  The FP32
+    // products of the mixed path stay in element order: their 4-byte
+    // row-major writes cost the product more than the scatter saves.
+    const int rm = c->csr_slot_t && !mixed ? 1 : 0;
+    const int *slot = rm ? c->csr_slot_t : c->csr_slot;
     hipEvent_t t0, t1;
     for (int it = 0; it < n; ++it) {
         int rc;
         bool tm = pf_timing_begin(c, PF_K_EBE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
         if (mixed)
-            rc = launch_ebe<KIND_F32, true, float>(c, c->p, c->utemp32, c->state, RUN_LIVE, nullptr, p32);
+            rc = launch_ebe<KIND_F32, true, float>(c, c->p, c->utemp32, c->state, RUN_LIVE, nullptr, p32, rm);
         else if (wide)
-            rc = launch_ebe<KIND_WIDE, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE);
+            rc = launch_ebe<KIND_WIDE, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE, nullptr, nullptr, rm);
         else if (split)
-            rc = launch_ebe<KIND_ACC, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE);
+            rc = launch_ebe<KIND_ACC, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE, nullptr, nullptr, rm);
         else
-            rc = launch_ebe<KIND_F64, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE);
+            rc = launch_ebe<KIND_F64, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE, nullptr, nullptr, rm);
         if (rc) return rc;
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         tm = pf_timing_begin(c, PF_K_SCATTER, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
         if (mixed)
-            hipLaunchKernelGGL(k_scatter_pu<float>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, c->csr_slot,
+            hipLaunchKernelGGL(k_scatter_pu<float>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, slot,
                                c->utemp32, c->fixstore, c->p, c->u, c->neq, c->fix_zero ? 1 : 0, c->partials,
                                c->state);
         else
             hipLaunchKernelGGL(k_scatter_pu<double>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr,
-                               c->csr_slot, c->utemp, c->fixstore, c->p, c->u, c->neq, c->fix_zero ? 1 : 0,
+                               slot, c->utemp, c->fixstore, c->p, c->u, c->neq, c->fix_zero ? 1 : 0,
                                c->partials, c->state);
         hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
@@ -1060,9 +1082,9 @@ int pf_pcg_run(pf_ctx *c, int n)
             // fold above flagged this iteration.
             tm = pf_timing_begin(c, PF_K_REFINE, &t0, &t1);
             if (tm) PF_HIP(hipEventRecord(t0, c->stream));
-            rc = launch_ebe<KIND_ACC, true, double>(c, c->x, c->utemp, c->state, RUN_REFINE);
+            rc = launch_ebe<KIND_ACC, true, double>(c, c->x, c->utemp, c->state, RUN_REFINE, nullptr, nullptr, rm);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_scatter_res, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, c->csr_slot,
+            hipLaunchKernelGGL(k_scatter_res, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, slot,
                                c->utemp, c->fixstore, c->x, c->f, c->r, c->d, c->dinv, c->neq, c->partials,
                                c->state);
             hipLaunchKernelGGL(k_fold_refine, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);

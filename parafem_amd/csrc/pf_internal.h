@@ -138,6 +138,7 @@ struct pf_ctx {
     int *g_g = nullptr;                // (ntot,nels), 1-based, 0 = restrained
     int *csr_ptr = nullptr;            // (neq+1) offsets into csr_slot
     int *csr_slot = nullptr;           // element slots e*ntot+i, ascending
+    int *csr_slot_t = nullptr;         // hex8: the same list as i*nels+e (row-major iteration products)
     int64_t nnz = 0;
     // element matrices (internal layouts above)
     int store = PF_STORE_FP64;

@@ -135,6 +135,19 @@ __global__ __launch_bounds__(256) void k_csr_sort(const int *__restrict__ ptr, i
     }
 }
 
+// slot_t[s] = i*nels + e for slot[s] = e*ntot + i: where the iteration's
+// products of a hex8 mesh keep row i of element e (all elements' row i side
+// by side, so the scatter's reads of one slot across consecutive equations
+// fall on consecutive words).  The list order is unchanged.
+__global__ __launch_bounds__(256) void k_csr_transpose(const int *__restrict__ slot, long nnz, int ntot, int nels,
+                                                       int *__restrict__ slot_t)
+{
+    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < nnz; s += (long)gridDim.x * blockDim.x) {
+        const int v = slot[s], e = v / ntot, i = v - e * ntot;
+        slot_t[s] = i * nels + e;
+    }
+}
+
 // g_g(i, iel) = nf(d, g_num(n, iel)) for element node n, freedom d, i = n*nodof + d.
 __global__ __launch_bounds__(256) void k_steer_gather(const int *__restrict__ g_num, const int *__restrict__ nf,
                                                      int *__restrict__ gg, long nslot, int nod, int nodof)
@@ -197,6 +210,8 @@ int pf_csr_build(pf_ctx *c)
     }
     c->nnz = host[1];
     rc = pf_alloc(c, (void **)&c->csr_slot, (size_t)std::max<int64_t>(c->nnz, 1) * sizeof(int));
+    if (!rc && c->ntot == 8)
+        rc = pf_alloc(c, (void **)&c->csr_slot_t, (size_t)std::max<int64_t>(c->nnz, 1) * sizeof(int));
     if (rc) {
         release();
         return rc;
@@ -208,6 +223,9 @@ int pf_csr_build(pf_ctx *c)
                            c->csr_slot);
         hipLaunchKernelGGL(k_csr_sort, dim3((neq + 255) / 256), dim3(256), 0, c->stream, c->csr_ptr, c->csr_slot,
                            neq);
+        if (c->csr_slot_t)
+            hipLaunchKernelGGL(k_csr_transpose, dim3(grid_for_stride(c->nnz)), dim3(256), 0, c->stream, c->csr_slot,
+                               (long)c->nnz, c->ntot, c->nels, c->csr_slot_t);
         he = hipGetLastError();
     }
     if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
