@@ -158,7 +158,9 @@ __device__ __forceinline__ double bpermute64(int addr, double v)
 
 // Element products.  GATHER: src is a neq+1 vector with src[0] = 0 and the
 // element vector is src[g_g]; otherwise src is pmul (ntot,nels).  Lane i of
-// an element's lane group forms row i of u_e = K_e p_e.
+// an element's lane group forms row i of u_e = K_e p_e and writes it to
+// out[e*NT + i], or, with rowmajor on an 8-node mesh (NT = 8), to
+// out[i*nels + e].
 //   KIND_F64: full K_e in FP64 column pairs, FP64 fused-multiply-add chain in
 //             ascending column order -> out (double);
 //   KIND_F32: packed symmetric FP32 hi, FP32 arithmetic on fl32(p) -> out
