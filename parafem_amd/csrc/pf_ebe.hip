@@ -209,7 +209,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
     for (int t = 0; t < EPL; ++t) {
         ev[t] = ebase + t * L::EPW + sub;
         ec[t] = ev[t] < nels ? ev[t] : (long)nels - 1;
-        g[t] = GATHER ? gg[ec[t] * NT + i] : 0;
+        g[t] = GATHER ? __builtin_nontemporal_load(&gg[ec[t] * NT + i]) : 0;   // read once per product
     }
     if constexpr (KIND == KIND_F64) {
         // Every column pair of K_e is in flight before the dependent gather.
@@ -625,7 +625,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_pupdate(double *__restrict__ p, fl
     for (int q = blockIdx.x * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
         const double pq = __dadd_rn(d[q], __dmul_rn(p[q], beta));
         __builtin_nontemporal_store(pq, &p[q]);   // read next by the gathers, not from this L2 line
-        if (p32) p32[q] = (float)pq;
+        if (p32) __builtin_nontemporal_store((float)pq, &p32[q]);
     }
 }
 
