@@ -275,3 +275,43 @@ def test_fortran_p123_deck(tmp_path):
     xm = np.fromfile(base + ".pf.x", dtype=np.float64)
     x64, _, conv64 = O.pcg(g_g, K, rhs, dg, 1e-12, 20000, stop=O.STOP_RELRES, no_f=no_f, store=store)
     assert conv64 and np.abs(xm - x64).max() <= 1e-8 * np.abs(x64).max()
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_fortran_p123_deck_without_restraints(tmp_path):
+    """pf_p123 with nr = 0 (g_g_pp = g_num_pp, p123.f90:53-55) and the
+    temperature held by penalty fixed freedoms on x = 0, unit flux on x = max
+    (the conduction cube of the benchmark, uniform k): xnew equals the
+    oracle's in the device's dot order bit for bit."""
+    sys.path.insert(0, os.path.dirname(__file__))
+    import oracle as O
+    pr = P.laplace_cube(6, lognormal_k=False, nels_xyz=(6, 5, 4))
+    m = pr.mesh
+    fix_nodes = np.nonzero(np.abs(m.coords[:, 0]) < 1e-12)[0] + 1
+    base = str(tmp_path / "p123_free")
+    loads = [(int(q) + 1, float(v)) for q, v in enumerate(pr.r0) if v != 0.0]
+    M.write_p123_deck(base, m, np.zeros((0, 2), dtype=np.int32), 1.0, 1.0, 1.0, 1e-6, 2000, 1,
+                      loads=loads, fix=[(int(n), 1, 0.0) for n in fix_nodes])
+    say(f"pf_p123 nr=0 neq={pr.neq}")
+    r = subprocess.run([EXE123, os.path.basename(base)], cwd=os.path.dirname(base), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, f"pf_p123 failed ({r.returncode}):\n{r.stdout}\n{r.stderr}"
+    d = M.read_deck(base, "p123")
+    nf, neq = M.steering(d["nn"], 1, None)
+    g_g = M.g_g_from(nf, d["g_num"])
+    assert d["nr"] == 0 and neq == d["nn"] and np.array_equal(g_g, d["g_num"])
+    K = O.assemble_laplace(M.element_coords(d["g_num"], d["coords"]), 1.0, 1.0, 1.0)
+    dg = O.diag(g_g, K, neq)
+    no_f = d["fix_node"]
+    store = dg[no_f - 1] + 1.0e20
+    dg[no_f - 1] = store
+    rhs = np.zeros(neq)
+    rhs[d["load_node"] - 1] = d["load_val"][:, 0]
+    rhs[no_f - 1] = store * d["fix_val"]
+    xo, it, conv = O.pcg(g_g, K, rhs, dg, d["tol"], d["limit"], no_f=no_f, store=store,
+                         dot_order=O.DOT_DEVICE)
+    x = np.fromfile(base + ".pf.x", dtype=np.float64)
+    assert conv and np.array_equal(x, xo)
+    res = open(base + ".pf.res").read().splitlines()
+    assert res[3] == f"The number of iterations to convergence was {it:5d}"
