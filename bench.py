@@ -352,7 +352,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU element "
-                          "sharding is a design only (see DESIGN.md); use --gpus 1"}))
+                          "the sharded path is a design only; use --gpus 1"}))
         sys.exit(2)
 
     from parafem_amd import Context, DeviceArray, problems as P, tune as T
