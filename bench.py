@@ -351,8 +351,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU: the multi-GPU element "
-                          "the sharded path is a design only; use --gpus 1"}))
+        print(json.dumps({"error": "parafem_amd runs on one GPU: element sharding across GPUs "
+                          "is a design only; use --gpus 1"}))
         sys.exit(2)
 
     from parafem_amd import Context, DeviceArray, problems as P, tune as T
