@@ -693,11 +693,18 @@ def test_steering_built_on_device(ctx):
 
 def test_steering_rejects_bad_maps(ctx):
     """An equation number outside 0..neq, an equation no element touches,
-    or a node number outside 1..nn is refused; the context then takes a
-    valid steering again."""
+    a node number outside 1..nn, an empty mesh, a steering with every
+    freedom restrained (neq = 0) or an element size other than 60, 24 or 8
+    freedoms is refused; the context then takes a valid steering again."""
     pr = P.elastic_cube(2)
     say("bad steering maps")
+    with pytest.raises(PFError):
+        ctx.mesh_set(np.zeros((0, 3, 20)))
     ctx.mesh_set(pr.g_coord())
+    with pytest.raises(PFError):
+        ctx.steer_set(np.zeros_like(pr.g_g), 0)
+    with pytest.raises(PFError):
+        ctx.steer_set(pr.g_g[:, :59], pr.neq)
     bad = pr.g_g.copy()
     bad[0, 0] = pr.neq + 1
     with pytest.raises(PFError):
