@@ -364,7 +364,15 @@ def main():
         for _ in range(5):
             ctx.dev_copy(b, a)
         ctx.sync()
-        print(json.dumps({"copy_bytes": n * 8, "launches": 5}), flush=True)
+        reps = 50
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.dev_copy(b, a)
+        ctx.sync()
+        t = (time.perf_counter() - t0) / reps
+        # read n*8 bytes + write n*8 bytes per launch
+        print(json.dumps({"copy_bytes": n * 8, "launches": 5 + reps, "ms_per_copy": round(t * 1e3, 4),
+                          "copy_gbs": round(2 * n * 8 / t / 1e9, 1)}), flush=True)
         return
 
     if args.workload == "serve":
