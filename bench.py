@@ -370,9 +370,19 @@ def main():
             ctx.dev_copy(b, a)
         ctx.sync()
         t = (time.perf_counter() - t0) / reps
-        # read n*8 bytes + write n*8 bytes per launch
+        # A read-only stream: DOT_PRODUCT_P of the two arrays (each call
+        # returns its sum to the host, so the wall clock includes that wait).
+        ctx.vec_fill(a, 1.0)
+        ctx.vec_fill(b, 0.5)
+        ctx.dot_product_p(a, b)
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            ctx.dot_product_p(a, b)
+        td = (time.perf_counter() - t1) / reps
+        # copy: read n*8 bytes + write n*8 bytes per launch; dot: read 2*n*8
         print(json.dumps({"copy_bytes": n * 8, "launches": 5 + reps, "ms_per_copy": round(t * 1e3, 4),
-                          "copy_gbs": round(2 * n * 8 / t / 1e9, 1)}), flush=True)
+                          "copy_gbs": round(2 * n * 8 / t / 1e9, 1), "ms_per_dot": round(td * 1e3, 4),
+                          "dot_read_gbs": round(2 * n * 8 / td / 1e9, 1)}), flush=True)
         return
 
     if args.workload == "serve":
