@@ -734,7 +734,22 @@ __global__ __launch_bounds__(PF_BLOCK) void k_reduce_prim(const double *__restri
 {
     double v[2] = {0.0, 0.0};
     const long stride = (long)gridDim.x * PF_BLOCK;
-    for (long i = (long)blockIdx.x * PF_BLOCK + threadIdx.x; i < n; i += stride) {
+    long i = (long)blockIdx.x * PF_BLOCK + threadIdx.x;
+    if (mode < 2) {
+        // Four strided entries in flight per lane, summed in the same order
+        // as one at a time.
+        for (; i + 3 * stride < n; i += 4 * stride) {
+            double x[4], y[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                x[k] = a[i + k * stride];
+                y[k] = mode == 0 ? b[i + k * stride] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[0] = __dadd_rn(v[0], mode == 0 ? __dmul_rn(x[k], y[k]) : x[k]);
+        }
+    }
+    for (; i < n; i += stride) {
         const double ai = a[i];
         if (mode == 0) {
             v[0] = __dadd_rn(v[0], __dmul_rn(ai, b[i]));
