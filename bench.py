@@ -125,12 +125,26 @@ def build_problem(workload, n):
     return P.laplace_cube(n, lognormal_k=True)
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(workload: str, n_sample: int, nels_full: int, budget_s: float = 12.0):
-    """The CPU oracle's FP64 EBE-PCG on an n^3 sample of the same workload,
-    all host threads (capped at 16), rate scaled by elements to the full
-    problem."""
+    """The CPU oracle's FP64 EBE-PCG on an n^3 sample of the same workload
+    with OMP_NUM_THREADS threads (the host cores granted to this GPU's
+    process: 16 per GPU on the MI355X hosts here, whatever nproc shows), rate
+    scaled by elements to the full problem.  nproc and the CPU model are
+    reported beside it."""
     import oracle as O
-    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+    nproc = os.cpu_count() or 1
+    threads = min(nproc, int(os.environ.get("OMP_NUM_THREADS", str(nproc))))
     pr = build_problem(workload, n_sample)
     if pr.kind == "elastic":
         K = O.assemble_elastic(pr.g_coord(), pr.e, pr.v, pr.e_elem, nthreads=threads)
@@ -159,6 +173,8 @@ def cpu_baseline(workload: str, n_sample: int, nels_full: int, budget_s: float =
         "value": rate * pr.nels / nels_full,
         "unit": "iter/s",
         "cores": threads,
+        "nproc": nproc,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": f"{n_sample}^3 {workload} cube ({pr.nels} elements, {pr.neq} equations), {n} FP64 "
                   f"PCG iterations in {t:.1f} s = {rate:.2f} iter/s, scaled by elements to "

@@ -408,6 +408,22 @@ int pf_pcg_timing_get(pf_ctx *ctx, double ms[PF_TIMING_SLOTS], int counts[PF_TIM
  * wavefronts, then the 4 wavefront sums left to right). */
 int pf_reduce_grid(int64_t n);
 
+/* Summation order of the FP64 dot products and sums (DOT_PRODUCT_P, SUM_P
+ * and the dots of the PCG loop) on a context:
+ *   PF_DOT_TREE (default): the fixed two-level tree above;
+ *   PF_DOT_SEQ: the reference's order at one rank, s = s + a(i)*b(i) for
+ *     i = 1..n ascending (DOT_PRODUCT inside DOT_PRODUCT_P,
+ *     src/modules/mpi/maths.f90:168-216; SUM inside SUM_P, :271-315),
+ *     every product and sum rounded as written.  One wavefront walks the
+ *     chain, so a dot costs about 12 clocks per entry: meant for parity runs
+ *     against the reference's FP64 trajectory, not for throughput.
+ * Maxima (checon_par) are exact in any order.  The order applies to the
+ * calls made after the change; a captured iteration chunk is re-captured. */
+#define PF_DOT_TREE        0
+#define PF_DOT_SEQ         1
+int pf_dot_order_set(pf_ctx *ctx, int order);
+int pf_dot_order_get(pf_ctx *ctx, int *order);
+
 #ifdef __cplusplus
 }
 #endif

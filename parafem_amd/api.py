@@ -39,6 +39,7 @@ PF_ERR_NODEVICE = 7
 STORE = {"fp64": 0, "fp32": 1}
 STOP = {"checon": 0, "relres": 1}
 PREC = {"fp64": 0, "mixed": 1, "mixed64": 2, "split": 3}
+DOT_ORDER = {"tree": 0, "seq": 1}
 
 
 class LibraryNotBuilt(RuntimeError):
@@ -154,6 +155,8 @@ def lib():
         "pf_pcg_timing_enable": (i32, [vp, i32]),
         "pf_pcg_timing_get": (i32, [vp, pd, pi]),
         "pf_dev_copy": (i32, [vp, vp, vp, sz]),
+        "pf_dot_order_set": (i32, [vp, i32]),
+        "pf_dot_order_get": (i32, [vp, pi]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -292,6 +295,19 @@ class Context:
         if dst.n != src.n:
             raise ValueError("size mismatch")
         _check(lib().pf_dev_copy(self.h, dst.ptr, src.ptr, src.n * 8), "pf_dev_copy")
+
+    @property
+    def dot_order(self) -> str:
+        """Summation order of dots and sums: "tree" (the device's fixed
+        two-level order, default) or "seq" (the reference's sequential
+        DOT_PRODUCT order at one rank, for parity runs)."""
+        o = C.c_int()
+        _check(lib().pf_dot_order_get(self.h, C.byref(o)), "pf_dot_order_get")
+        return {v: k for k, v in DOT_ORDER.items()}[o.value]
+
+    @dot_order.setter
+    def dot_order(self, order: str):
+        _check(lib().pf_dot_order_set(self.h, DOT_ORDER[order]), "pf_dot_order_set")
 
     def mem_used(self) -> int:
         n = C.c_size_t()

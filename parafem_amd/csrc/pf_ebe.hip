@@ -682,6 +682,85 @@ __global__ __launch_bounds__(PF_BLOCK) void k_fold_init(const double *partials, 
     }
 }
 
+// ------------------------------------------- reference-order reductions
+// DOT_PRODUCT at one rank (maths.f90:168-216): s = 0; s = s + a(i)*b(i) for
+// i = 1..n in ascending order, each product and sum rounded as written (b
+// null: SUM, s = s + a(i), maths.f90:271-315).  The chain is sequential by
+// definition, so one wavefront walks it: the lanes load and multiply 64
+// consecutive terms, a uniform register takes them in lane order through
+// v_readlane, and the next 256 terms are loaded while the current ones are
+// added.
+__device__ __forceinline__ double readlane64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ double seq_sum(const double *__restrict__ a, const double *__restrict__ b, long n)
+{
+    constexpr int U = 4;
+    const int lane = threadIdx.x & 63;
+    double t[U], nx[U];
+    auto load = [&](long i0, double (&v)[U]) {
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const long i = i0 + 64 * k + lane;
+            v[k] = i < n ? (b ? __dmul_rn(a[i], b[i]) : a[i]) : 0.0;
+        }
+    };
+    double s = 0.0;
+    load(0, t);
+    for (long i0 = 0; i0 < n; i0 += 64 * U) {
+        load(i0 + 64 * U, nx);
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const long m = n - (i0 + 64 * k);
+            if (m >= 64) {
+#pragma unroll
+                for (int l = 0; l < 64; ++l) s = __dadd_rn(s, readlane64(t[k], l));
+            } else {
+                for (int l = 0; l < (int)m; ++l) s = __dadd_rn(s, readlane64(t[k], l));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) t[k] = nx[k];
+    }
+    return s;
+}
+
+// Up to two reference-order sums (a[k], b[k] -> quantity qs[k]) and up to
+// two maxima (the G block maxima of quantity qm[k], exact in any order),
+// written to slot 0 of their quantities: the fold kernels then run with one
+// block partial.  One wavefront.
+struct pf_seq_args {
+    const double *a[2];
+    const double *b[2];
+    int qs[2];
+    int ns;
+    int qm[2];
+    int nm;
+};
+
+__global__ __launch_bounds__(64) void k_seq_reduce(pf_seq_args A, long n, int G, double *partials,
+                                                  const pf_state *st, int gate)
+{
+    if (skip(st, gate)) return;
+    for (int k = 0; k < A.nm; ++k) {
+        const double *P = partials + (long)A.qm[k] * MAXG;
+        double m = 0.0;
+        for (int j = threadIdx.x; j < G; j += 64) m = fmax(m, P[j]);
+        m = wave_max(m);
+        __builtin_amdgcn_wave_barrier();   // every lane has read P[0] before lane 0 writes it
+        if (threadIdx.x == 0) partials[(long)A.qm[k] * MAXG] = m;
+    }
+    for (int k = 0; k < A.ns; ++k) {
+        const double s = seq_sum(A.a[k], A.b[k], n);
+        if (threadIdx.x == 0) partials[(long)A.qs[k] * MAXG] = s;
+    }
+}
+
 // ---------------------------------------------------------- primitives
 typedef unsigned u4_t __attribute__((ext_vector_type(4)));
 
@@ -864,15 +943,37 @@ int launch_ebe_exact(pf_ctx *c, const double *src, double *out)
     return launch_ebe<KIND_ACC, GATHER, double>(c, src, out, nullptr, RUN_ALWAYS);
 }
 
+// With the PF_DOT_SEQ order: the reference-order sums (and the folded
+// maxima) of A replace the block partials' slot 0; returns the block count
+// the following fold kernel reads (1).  With the tree order: nothing, G.
+int seq_reduce(pf_ctx *c, int G, const double *a0, const double *b0, int q0, const double *a1,
+               const double *b1, int q1, bool maxima, long n, const pf_state *st, int gate)
+{
+    if (c->dot_order != PF_DOT_SEQ) return G;
+    pf_seq_args A{};
+    A.a[0] = a0; A.b[0] = b0; A.qs[0] = q0;
+    A.a[1] = a1; A.b[1] = b1; A.qs[1] = q1;
+    A.ns = a1 ? 2 : 1;
+    A.qm[0] = PF_Q_MAXX; A.qm[1] = PF_Q_MAXDX;
+    A.nm = maxima ? 2 : 0;
+    hipLaunchKernelGGL(k_seq_reduce, dim3(1), dim3(64), 0, c->stream, A, n, G, c->partials, st, gate);
+    return 1;
+}
+
 int reduce_prim(pf_ctx *c, const double *a, double *b, int64_t n, int mode, double *out2)
 {
     if (n <= 0) {
         out2[0] = out2[1] = 0.0;
         return PF_OK;
     }
-    const int G = pf_grid_for(n);
+    int G = pf_grid_for(n);
     double *dout = c->partials + (size_t)PF_NQ * MAXG;   // spare slots past the PCG partials
-    hipLaunchKernelGGL(k_reduce_prim, dim3(G), dim3(PF_BLOCK), 0, c->stream, a, b, (long)n, mode, c->partials);
+    if (mode < 2 && c->dot_order == PF_DOT_SEQ)
+        G = seq_reduce(c, G, a, mode == 0 ? b : nullptr, 0, nullptr, nullptr, 0, false, (long)n, nullptr,
+                       RUN_ALWAYS);
+    else
+        hipLaunchKernelGGL(k_reduce_prim, dim3(G), dim3(PF_BLOCK), 0, c->stream, a, b, (long)n, mode,
+                           c->partials);
     hipLaunchKernelGGL(k_fold_prim, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, mode, dout);
     PF_HIP(hipGetLastError());
     PF_HIP(hipMemcpyAsync(c->scalar_host, dout, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -971,6 +1072,20 @@ int pf_sum_p(pf_ctx *c, const double *a, int64_t n, double *out)
     return rc;
 }
 
+int pf_dot_order_set(pf_ctx *c, int order)
+{
+    if (!c || (order != PF_DOT_TREE && order != PF_DOT_SEQ)) return PF_ERR_ARG;
+    c->dot_order = order;
+    return PF_OK;
+}
+
+int pf_dot_order_get(pf_ctx *c, int *order)
+{
+    if (!c || !order) return PF_ERR_ARG;
+    *order = c->dot_order;
+    return PF_OK;
+}
+
 int pf_checon_par(pf_ctx *c, const double *xnew, double tol, int *converged, double *x, int64_t n)
 {
     if (!c || !converged || n < 0 || (n && (!xnew || !x))) return PF_ERR_ARG;
@@ -1022,7 +1137,9 @@ static int pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const double
     float *p32 = opts->precision == PF_PREC_MIXED && c->ntot <= 24 ? c->p32 : nullptr;
     hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, p32, c->dinv, c->diag,
                        c->f, x0, c->neq, c->precon_given ? 1 : 0, c->partials);
-    hipLaunchKernelGGL(k_fold_init, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
+    const int Gi = seq_reduce(c, G, c->r + 1, c->d + 1, PF_Q_RD, c->r + 1, c->r + 1, PF_Q_RR, false, c->neq,
+                              nullptr, RUN_ALWAYS);
+    hipLaunchKernelGGL(k_fold_init, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, Gi, c->state);
     PF_HIP(hipGetLastError());
     // The host mirror is read back by pf_pcg_poll; wait so it is not reused
     // while the copy above is still in flight.
@@ -1078,19 +1195,25 @@ int pf_pcg_run(pf_ctx *c, int n)
             hipLaunchKernelGGL(k_scatter_pu<double>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr,
                                slot, c->utemp, c->fixstore, c->p, c->u, c->neq, c->fix_zero ? 1 : 0,
                                c->partials, c->state);
-        hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
+        const int Ga = seq_reduce(c, G, c->p + 1, c->u + 1, PF_Q_PU, nullptr, nullptr, 0, false, c->neq, c->state,
+                                  RUN_LIVE);
+        hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, Ga, c->state);
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         tm = pf_timing_begin(c, PF_K_UPDATE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
         if (c->opts.stop_mode == PF_STOP_CHECON) {
             hipLaunchKernelGGL(k_update<true>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, c->u,
                                c->dinv, c->neq, c->partials, c->state);
-            hipLaunchKernelGGL(k_fold_update<true>, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G,
+            const int Gu = seq_reduce(c, G, c->r + 1, c->d + 1, PF_Q_RD, nullptr, nullptr, 0, true, c->neq,
+                                      c->state, RUN_LIVE);
+            hipLaunchKernelGGL(k_fold_update<true>, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, Gu,
                                c->state);
         } else {
             hipLaunchKernelGGL(k_update<false>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->x, c->r, c->d, c->p, c->u,
                                c->dinv, c->neq, c->partials, c->state);
-            hipLaunchKernelGGL(k_fold_update<false>, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G,
+            const int Gu = seq_reduce(c, G, c->r + 1, c->d + 1, PF_Q_RD, c->r + 1, c->r + 1, PF_Q_RR, false,
+                                      c->neq, c->state, RUN_LIVE);
+            hipLaunchKernelGGL(k_fold_update<false>, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, Gu,
                                c->state);
         }
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
@@ -1104,7 +1227,9 @@ int pf_pcg_run(pf_ctx *c, int n)
             hipLaunchKernelGGL(k_scatter_res, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, slot,
                                c->utemp, c->fixstore, c->x, c->f, c->r, c->d, c->dinv, c->neq, c->partials,
                                c->state);
-            hipLaunchKernelGGL(k_fold_refine, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, G, c->state);
+            const int Gr = seq_reduce(c, G, c->r + 1, c->d + 1, PF_Q_RD, c->r + 1, c->r + 1, PF_Q_RR, false,
+                                      c->neq, c->state, RUN_REFINE);
+            hipLaunchKernelGGL(k_fold_refine, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, Gr, c->state);
             if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         }
         tm = pf_timing_begin(c, PF_K_PUPDATE, &t0, &t1);
@@ -1122,7 +1247,7 @@ int pf_pcg_run_graph(pf_ctx *c, int n)
     if (!c->pcg_ready) return PF_ERR_STATE;
     if (c->timing) return pf_pcg_run(c, n);   // timing records an event per launch
     if (!c->graph_exec || c->graph_iters != n || c->graph_precision != c->opts.precision ||
-        c->graph_stop != c->opts.stop_mode) {
+        c->graph_stop != c->opts.stop_mode || c->graph_dot != c->dot_order) {
         pf_graph_drop(c);
         PF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         const int rc = pf_pcg_run(c, n);
@@ -1142,6 +1267,7 @@ int pf_pcg_run_graph(pf_ctx *c, int n)
         c->graph_iters = n;
         c->graph_precision = c->opts.precision;
         c->graph_stop = c->opts.stop_mode;
+        c->graph_dot = c->dot_order;
     }
     PF_HIP(hipGraphLaunch(c->graph_exec, c->stream));
     return PF_OK;

@@ -176,7 +176,9 @@ struct pf_ctx {
     // Captured iteration chunk (pf_pcg_run_graph) and what it was captured for.
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_iters = 0, graph_precision = -1, graph_stop = -1;
+    int graph_iters = 0, graph_precision = -1, graph_stop = -1, graph_dot = -1;
+    // Summation order of dots and sums (PF_DOT_TREE or PF_DOT_SEQ).
+    int dot_order = PF_DOT_TREE;
     // Launch shapes (pf_tune_set); 0 = the built-in choice.
     int tune_epl = 0;                  // single-RHS products: elements per lane group
     int tune_blk_waves = 0;            // block products: elements per workgroup

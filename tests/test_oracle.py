@@ -303,6 +303,28 @@ def test_device_order_dot_is_a_sum():
         assert abs(O.dot_device(a) - math.fsum(a)) <= 1e-13 * np.abs(a).sum()
 
 
+def test_sequential_dot_and_sum_are_the_reference_order():
+    """a20/a15: DOT_PRODUCT_P and SUM_P at one rank add a(i)*b(i) (or a(i))
+    to s = 0 one at a time in ascending i, every product and sum rounded
+    (maths.f90:168-216,271-315).  Python floats are IEEE doubles rounded the
+    same way, so a plain loop is an independent statement of that order; the
+    data are chosen so that any other order gives different bits."""
+    rng = np.random.default_rng(10)
+    for n in (1, 2, 64, 257, 3001):
+        a = rng.normal(size=n) * np.exp(rng.uniform(-20, 20, size=n))
+        b = rng.normal(size=n)
+        s = 0.0
+        for ai, bi in zip(a.tolist(), b.tolist()):
+            s = s + ai * bi
+        t = 0.0
+        for ai in a.tolist():
+            t = t + ai
+        assert O.dot(a, b) == s
+        assert O.sum_p(a) == t
+        if n > 64:
+            assert O.dot(a[::-1], b[::-1]) != s   # the order is visible in the bits
+
+
 # ------------------------------------------------------ PCG, end to end
 def test_pcg_matches_scipy_direct_solve():
     pr = P.elastic_cube(3, lognormal_e=True)
