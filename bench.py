@@ -269,6 +269,42 @@ def block_bench(args):
     ctx.close()
 
 
+def serve_sample(requests: int, nmin: int, nmax: int, precision: str, slots: int = 32):
+    """A reduced serving stream for the default line: closed-loop capacity of
+    the batched server, then the same requests at Poisson arrivals of 80%
+    of it (p50/p99 latency), and the FP64 one-by-one capacity on them."""
+    from parafem_amd import serve as S
+    cache = S.CubeCache(keep=64)
+    closed = S.make_stream(requests, 0.0, seed=4, nmin=nmin, nmax=nmax)
+    for r in closed:
+        cache.get(r.n)
+    srv = S.Server(slots=slots, cache=cache, precision=precision)
+    try:
+        srv.serve(S.make_stream(4, 0.0, seed=5, nmin=nmin, nmax=nmin + 2))    # warm-up
+        t0 = time.perf_counter()
+        cap = S.summarize(srv.serve(closed), time.perf_counter() - t0)
+        rate = 0.8 * cap["solves_per_s"]
+        stream = S.make_stream(requests, rate, seed=4, nmin=nmin, nmax=nmax)
+        t0 = time.perf_counter()
+        mix = S.summarize(srv.serve(stream), time.perf_counter() - t0)
+    finally:
+        srv.close()
+    t0 = time.perf_counter()
+    seq = S.summarize(S.serve_sequential(closed, precision="fp64", cache=cache), time.perf_counter() - t0)
+    return {
+        "stream": f"{requests} hex20 cube requests, n log-uniform in [{nmin},{nmax}], E U[1,200], nu U[0.2,0.4], "
+                  f"pressure U[0.1,10] (seed 4), relres 1e-10, {precision} server with {slots} slots",
+        "closed_loop_solves_per_s": round(cap["solves_per_s"], 3),
+        "rate_per_s": round(rate, 3),
+        "solves_per_s": round(mix["solves_per_s"], 3),
+        "p50_latency_s": round(mix["p50_latency_s"], 4),
+        "p99_latency_s": round(mix["p99_latency_s"], 4),
+        "fp64_one_by_one_solves_per_s": round(seq["solves_per_s"], 3),
+        "capacity_vs_fp64_one_by_one": round(cap["solves_per_s"] / seq["solves_per_s"], 3),
+        "all_converged": cap["all_converged"] and mix["all_converged"] and seq["all_converged"],
+    }
+
+
 def serve_bench(args):
     """The serving loop on a request stream: closed-loop capacity, then the
     stream at a fixed Poisson rate through the mixed-precision server, then
@@ -350,11 +386,14 @@ def main():
                     help="block: single-RHS FP64 solves timed for the sequential baseline")
     ap.add_argument("--requests", type=int, default=200, help="serve: stream length")
     ap.add_argument("--nmin", type=int, default=10, help="serve: smallest cube edge")
-    ap.add_argument("--nmax", type=int, default=40, help="serve: largest cube edge")
+    ap.add_argument("--nmax", type=int, default=80, help="serve: largest cube edge")
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serve: Poisson arrival rate (0 = 80%% of the measured closed-loop capacity)")
-    ap.add_argument("--slots", type=int, default=8, help="serve: concurrent solves")
-    ap.add_argument("--serve-precision", default="split", choices=["mixed", "mixed64", "split"],
+    ap.add_argument("--slots", type=int, default=32, help="serve: concurrent solves (one batch)")
+    ap.add_argument("--serve-sample", type=int, default=48,
+                    help="hex20: requests of the reduced serving stream reported beside the iteration rate "
+                         "(0 = skip)")
+    ap.add_argument("--serve-precision", default="mixed64", choices=["fp64", "mixed", "mixed64", "split"],
                     help="serve: arithmetic of the element products in the server")
     ap.add_argument("--n", type=int, default=0, help="elements per cube edge (0 = the workload's size)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
@@ -459,6 +498,14 @@ def main():
             solves[f"speedup_{prec}_vs_fp64"] = round(solves["fp64"]["time_s"] / solves[prec]["time_s"], 3)
             log(f"{prec} solve: {solves[prec]}")
 
+    serving = None
+    if args.serve_sample > 0 and args.workload == "hex20":
+        ctx.reset()
+        ctx.trim()
+        log("serving sample ...")
+        serving = serve_sample(args.serve_sample, 10, 40, args.serve_precision)
+        log(f"serving sample: {serving}")
+
     cpu = None
     if args.cpu_sample > 0:
         log("cpu baseline ...")
@@ -496,6 +543,7 @@ def main():
             "paths": {k: {kk: vv for kk, vv in v.items() if kk != "elapsed_s"} for k, v in paths.items()},
             "solve_relres": args.solve_tol,
             "solves": solves,
+            "serving": serving,
         },
         "roofline": {
             "bound": "hbm",

@@ -254,6 +254,35 @@ int pf_pcg_r(pf_ctx *ctx, double *r, int r_on_device);
  * res is filled) once the snapshot has arrived. */
 int pf_pcg_run_graph(pf_ctx *ctx, int n_iters);
 int pf_pcg_query(pf_ctx *ctx, pf_pcg_result *res, int *ready);
+/* No counterpart in the reference (serving many solves at once).  A batch
+ * advances up to PF_BATCH_MAX solves -- contexts each holding a begun PCG
+ * (pf_pcg_begin) with the same ntot, precision and stopping rule and the
+ * tree dot order -- with one launch per kernel of the iteration for all of
+ * them: each solve's blocks run its own launch's work (same local blocks,
+ * same reduction grid), so every solve's iterates are bit-identical to
+ * pf_pcg_run on its context alone, and a solve that has stopped does
+ * nothing.  The batch has its own stream; pf_batch_run orders it after
+ * every context's stream and every context's stream after the run, so the
+ * context calls that follow (pf_pcg_x, pf_pcg_query, pf_reset) see its
+ * results.  The set of contexts may change from call to call (serving:
+ * admit and retire solves between runs).
+ *   pf_batch_run       : n_iters iterations as direct launches;
+ *   pf_batch_run_graph : the same, captured once into a HIP graph for the
+ *                        current set and replayed;
+ *   pf_batch_query     : non-blocking poll.  The first call after a
+ *                        snapshot was delivered takes a new one of the
+ *                        solves of the last run (in the order given) at
+ *                        that point of the batch stream; runs enqueued
+ *                        after it do not change it.  *ready = 1 and res[k]
+ *                        filled, one per solve of that run, once it has
+ *                        arrived. */
+#define PF_BATCH_MAX 64
+typedef struct pf_batch pf_batch;
+int pf_batch_create(int device, pf_batch **batch);
+int pf_batch_destroy(pf_batch *batch);
+int pf_batch_run(pf_batch *batch, int nctx, pf_ctx *const *ctxs, int n_iters);
+int pf_batch_run_graph(pf_batch *batch, int nctx, pf_ctx *const *ctxs, int n_iters);
+int pf_batch_query(pf_batch *batch, pf_pcg_result *res, int *ready);
 /* y = K x over the assembled operator in FP64 (host vectors, neq), with the
  * p123 fixed-freedom rows applied.  Used to report true residuals. */
 int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);

@@ -7,7 +7,7 @@ Modules:
   build     -- compiles csrc/*.hip for gfx950 into libparafem_amd.so
 """
 from . import mesh  # noqa: F401
-from .api import (Context, DeviceArray, LibraryNotBuilt, PFError, lib,  # noqa: F401
+from .api import (Batch, Context, DeviceArray, LibraryNotBuilt, PFError, lib,  # noqa: F401
                   reduce_grid)
 
 __version__ = "0.1.0"

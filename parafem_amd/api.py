@@ -156,6 +156,11 @@ def lib():
         "pf_pcg_timing_get": (i32, [vp, pd, pi]),
         "pf_dev_copy": (i32, [vp, vp, vp, sz]),
         "pf_dot_order_set": (i32, [vp, i32]),
+        "pf_batch_create": (i32, [i32, C.POINTER(vp)]),
+        "pf_batch_destroy": (i32, [vp]),
+        "pf_batch_run": (i32, [vp, i32, C.POINTER(vp), i32]),
+        "pf_batch_run_graph": (i32, [vp, i32, C.POINTER(vp), i32]),
+        "pf_batch_query": (i32, [vp, vp, pi]),
         "pf_dot_order_get": (i32, [vp, pi]),
     }
     for name, (res, args) in sig.items():
@@ -662,6 +667,69 @@ class Context:
         y = np.empty(self.neq)
         _check(lib().pf_apply(self.h, _pd(xx), _pd(y)), "pf_apply")
         return y
+
+
+BATCH_MAX = 64
+
+
+class Batch:
+    """Many begun solves advanced by one launch per kernel (pf_batch_*):
+    each solve's iterates equal what pf_pcg_run on its context alone gives,
+    bit for bit."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().pf_batch_create(int(device), C.byref(h)), "pf_batch_create")
+        self.h = h.value
+        self._last = []
+        self._snap = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pf_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _handles(ctxs):
+        arr = (C.c_void_p * len(ctxs))(*[c.h for c in ctxs])
+        return arr
+
+    def run(self, ctxs, n: int, graph: bool = True):
+        """Enqueue n iterations of every context in ctxs (no host sync)."""
+        if not 1 <= len(ctxs) <= BATCH_MAX:
+            raise ValueError(f"1..{BATCH_MAX} contexts per batch")
+        arr = self._handles(ctxs)
+        f = lib().pf_batch_run_graph if graph else lib().pf_batch_run
+        _check(f(self.h, len(ctxs), arr, int(n)), "pf_batch_run")
+        self._last = list(ctxs)
+
+    def query(self):
+        """Non-blocking poll: (contexts, [PcgResult]) of the snapshot taken
+        at the first call after the previous one was delivered -- the
+        contexts of the last run at that point, in its order -- once it has
+        arrived, else None.  Runs enqueued after the snapshot do not change
+        it."""
+        if self._snap is None:
+            self._snap = self._last
+        res = (PcgResult * len(self._snap))()
+        ready = C.c_int(0)
+        _check(lib().pf_batch_query(self.h, C.cast(res, C.c_void_p), C.byref(ready)), "pf_batch_query")
+        if not ready.value:
+            return None
+        snap, self._snap = self._snap, None
+        return snap, [res[k] for k in range(len(snap))]
 
 
 def reduce_grid(n: int) -> int:

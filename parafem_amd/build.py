@@ -60,7 +60,9 @@ def build_fortran(verbose: bool = False) -> str | None:
     for exe, src in ((P121, "pf_p121.f90"), (P122, "pf_p122.f90"), (P123, "pf_p123.f90"),
                      (P124, "pf_p124.f90")):
         tmp = exe + ".tmp"
-        cmd = [FLANG, "-O2", "-J", moddir, os.path.join(FSRC, "pf_gpu.f90"),
+        # no contraction: the host-array loop rounds every product and sum
+        # as written, as the device does (fused multiply-adds are explicit)
+        cmd = [FLANG, "-O2", "-ffp-contract=off", "-J", moddir, os.path.join(FSRC, "pf_gpu.f90"),
                os.path.join(FSRC, src), "-L", HERE, "-lparafem_amd",
                "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=moddir)

@@ -2,9 +2,13 @@
 //
 // One 64-lane wavefront builds one element: SAMPLE's 2x2x2 Gauss points,
 // SHAPE_DER at each point, the jacobian, its determinant and inverse, global
-// derivatives, then the element matrix, written once to HBM in the
-// library's FP64 column-pair layout (pf_internal.h).  Nothing but the element
-// coordinates is read and nothing but the matrix is written.
+// derivatives, then the element matrix, written once to HBM: in the
+// library's FP64 column-pair layout (pf_internal.h), or -- for FP32 storage
+// -- straight into the packed symmetric hi/lo layout (hi = fl32(K), lo =
+// fl32(K - hi) of the upper triangle) together with the element's FP64
+// diagonal, so an FP32-stored problem never holds the FP64 matrices.
+// Nothing but the element coordinates is read and nothing but the matrix is
+// written.
 //
 // Reference: src/programs/5th_ed/p121/p121.f90:54-64 (elastic),
 // src/programs/5th_ed/p123/p123.f90:70-84 (Laplace); element routines in
@@ -109,10 +113,37 @@ __device__ __forceinline__ void element_geometry(ElemLds<NOD> &L, const double *
 //   x dof: B(1,i)=dx B(4,i)=dy B(6,i)=dz   y dof: B(2,i)=dy B(4,i)=dx B(5,i)=dz
 //   z dof: B(3,i)=dz B(5,i)=dy B(6,i)=dx   (BEEMAT ih=6)
 // and km(i,j) += (s*det)*w with SAMPLE's w = 1.
-template <int NOD>
+// FP32 storage: entry (i, j), j >= i, of element e (the upper triangle the
+// packed layout keeps) as hi and lo; the diagonal also in FP64 into dg(i,e)
+// for the assembled Jacobi diagonal.
+template <int NT>
+__device__ __forceinline__ void store_packed(long e, int i, int j, double k, float *__restrict__ hi,
+                                             float *__restrict__ lo, double *__restrict__ dg)
+{
+    if (j < i) return;
+    const long t = pf_idx_sym_ij(e, NT, i, j);
+    const float h = (float)k;
+    hi[t] = h;
+    lo[t] = (float)(k - (double)h);
+    if (j == i) dg[e * NT + i] = k;
+}
+
+// Zero padding at the end of element e's packed block.
+template <int NT>
+__device__ __forceinline__ void zero_padding(long e, int lane, float *__restrict__ hi, float *__restrict__ lo)
+{
+    constexpr int used = NT * (NT + 1) / 2, pad = (int)pf_sym_per(NT) - used;
+    if (lane < pad) {
+        hi[e * pf_sym_per(NT) + used + lane] = 0.0f;
+        lo[e * pf_sym_per(NT) + used + lane] = 0.0f;
+    }
+}
+
+template <int NOD, bool S32>
 __global__ __launch_bounds__(64 * WAVES) void k_assemble_elastic(
     const double *__restrict__ g_coord, const double *__restrict__ e_elem, double v2, double vv,
-    double den, double e0, double *__restrict__ K, int nels)
+    double den, double e0, double *__restrict__ K, float *__restrict__ hi, float *__restrict__ lo,
+    double *__restrict__ dg, int nels)
 {
     constexpr int NTOT = 3 * NOD;
     __shared__ ElemLds<NOD> lds[WAVES];
@@ -121,7 +152,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_assemble_elastic(
     const bool valid = e < nels;
     ElemLds<NOD> &L = lds[wave];
     element_geometry<NOD>(L, g_coord + (valid ? e : 0) * (NOD * 3), lane, valid);
-    if (!valid || lane >= NTOT) return;
+    if (!valid) return;
+    if constexpr (S32) zero_padding<NTOT>(e, lane, hi, lo);
+    if (lane >= NTOT) return;
     // DEEMAT: dee = dee0 * (e / (2(1+v)vv)) with dee0 = 1 on the normal
     // diagonal, v2 off it, vv on the shear diagonal.
     const double f = (e_elem ? e_elem[e] : e0) / den;
@@ -167,15 +200,22 @@ __global__ __launch_bounds__(64 * WAVES) void k_assemble_elastic(
             o[3 * h + 1] = o1;
             o[3 * h + 2] = o2;
         }
+        if constexpr (S32) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) Kp[(3 * bb + q) * NTOT] = make_double2(o[2 * q], o[2 * q + 1]);
+            for (int q = 0; q < 6; ++q) store_packed<NTOT>(e, lane, 6 * bb + q, o[q], hi, lo, dg);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Kp[(3 * bb + q) * NTOT] = make_double2(o[2 * q], o[2 * q + 1]);
+        }
     }
 }
 
 // p123: kc{x,y,z} = sum_g d{x,y,z}_a d{x,y,z}_b det w; storkc = kx*kcx + ky*kcy + kz*kcz.
+template <bool S32>
 __global__ __launch_bounds__(64 * WAVES) void k_assemble_laplace8(
     const double *__restrict__ g_coord, const double *__restrict__ k_elem, double kx, double ky,
-    double kz, double *__restrict__ K, int nels)
+    double kz, double *__restrict__ K, float *__restrict__ hi, float *__restrict__ lo, double *__restrict__ dg,
+    int nels)
 {
     __shared__ ElemLds<8> lds[WAVES];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -194,7 +234,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_assemble_laplace8(
         cz += L.deriv[g][2][a] * L.deriv[g][2][b] * w;
     }
     const double s = k_elem ? k_elem[e] : 1.0;
-    K[pf_idx_pair(e, 8, a, b)] = cx * (kx * s) + cy * (ky * s) + cz * (kz * s);
+    const double k = cx * (kx * s) + cy * (ky * s) + cz * (kz * s);
+    if constexpr (S32) store_packed<8>(e, a, b, k, hi, lo, dg);
+    else K[pf_idx_pair(e, 8, a, b)] = k;
 }
 
 // p124's element loop (src/programs/5th_ed/p124/p124.f90:73-86) on an
@@ -330,29 +372,70 @@ __global__ void k_diag(const int *__restrict__ ptr, const int *__restrict__ slot
     if (blockIdx.x == 0 && threadIdx.x == 0) diag[0] = 0.0;
 }
 
+// The same diagonal from the FP64 element diagonals dg(ntot,nels) that the
+// FP32-storage assembly writes.
+__global__ void k_diag_elem(const int *__restrict__ ptr, const int *__restrict__ slot,
+                            const double *__restrict__ dg, double *__restrict__ diag, int neq)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < neq; q += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = ptr[q]; k < ptr[q + 1]; ++k) s = __dadd_rn(s, dg[slot[k]]);
+        diag[q + 1] = s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) diag[0] = 0.0;
+}
+
 }  // namespace
 
-int pf_launch_assemble_elastic(pf_ctx *c, double v2, double vv, double den, double e0, const double *e_elem)
+// FP64 storage writes K64; FP32 storage writes K32hi/K32lo and the element
+// diagonals into dg (ntot*nels doubles).
+int pf_launch_assemble_elastic(pf_ctx *c, double v2, double vv, double den, double e0, const double *e_elem,
+                               double *dg)
 {
     const int grid = (c->nels + WAVES - 1) / WAVES;
-    if (c->nod == 20)
-        hipLaunchKernelGGL(k_assemble_elastic<20>, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
-                           e_elem, v2, vv, den, e0, c->K64, c->nels);
+    const bool s32 = c->store == PF_STORE_FP32;
+    if (s32 && (!c->K32hi || !c->K32lo || !dg)) return PF_ERR_STATE;
+    if (c->nod == 20 && s32)
+        hipLaunchKernelGGL((k_assemble_elastic<20, true>), dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
+                           e_elem, v2, vv, den, e0, (double *)nullptr, c->K32hi, c->K32lo, dg, c->nels);
+    else if (c->nod == 20)
+        hipLaunchKernelGGL((k_assemble_elastic<20, false>), dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
+                           e_elem, v2, vv, den, e0, c->K64, (float *)nullptr, (float *)nullptr, (double *)nullptr,
+                           c->nels);
+    else if (c->nod == 8 && s32)
+        hipLaunchKernelGGL((k_assemble_elastic<8, true>), dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
+                           e_elem, v2, vv, den, e0, (double *)nullptr, c->K32hi, c->K32lo, dg, c->nels);
     else if (c->nod == 8)
-        hipLaunchKernelGGL(k_assemble_elastic<8>, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
-                           e_elem, v2, vv, den, e0, c->K64, c->nels);
+        hipLaunchKernelGGL((k_assemble_elastic<8, false>), dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
+                           e_elem, v2, vv, den, e0, c->K64, (float *)nullptr, (float *)nullptr, (double *)nullptr,
+                           c->nels);
     else
         return PF_ERR_ARG;
     PF_HIP(hipGetLastError());
     return PF_OK;
 }
 
-int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem)
+int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem, double *dg)
 {
     if (c->nod != 8) return PF_ERR_ARG;
     const int grid = (c->nels + WAVES - 1) / WAVES;
-    hipLaunchKernelGGL(k_assemble_laplace8, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord, k_elem, kx,
-                       ky, kz, c->K64, c->nels);
+    if (c->store == PF_STORE_FP32) {
+        if (!c->K32hi || !c->K32lo || !dg) return PF_ERR_STATE;
+        hipLaunchKernelGGL(k_assemble_laplace8<true>, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
+                           k_elem, kx, ky, kz, (double *)nullptr, c->K32hi, c->K32lo, dg, c->nels);
+    } else {
+        hipLaunchKernelGGL(k_assemble_laplace8<false>, dim3(grid), dim3(64 * WAVES), 0, c->stream, c->g_coord,
+                           k_elem, kx, ky, kz, c->K64, (float *)nullptr, (float *)nullptr, (double *)nullptr,
+                           c->nels);
+    }
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+int pf_launch_diag_elem(pf_ctx *c, const double *dg, double *diag)
+{
+    const int grid = (c->neq + 255) / 256 < 8192 ? (c->neq + 255) / 256 : 8192;
+    hipLaunchKernelGGL(k_diag_elem, dim3(grid), dim3(256), 0, c->stream, c->csr_ptr, c->csr_slot, dg, diag, c->neq);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }

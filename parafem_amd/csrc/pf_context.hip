@@ -467,6 +467,39 @@ static int alloc_k64(pf_ctx *c, int store, bool shared = false)
     return pf_alloc(c, (void **)&c->K64, pf_k64_count(c) * sizeof(double));
 }
 
+// Storage for an assembly on the device: K64 for FP64, the packed hi/lo
+// matrices (and the FP32 products' utemp32) for FP32 -- the FP32 assembly
+// writes them directly, with no FP64 copy.
+static int alloc_assembly(pf_ctx *c, int store)
+{
+    if (store == PF_STORE_FP64) return alloc_k64(c, store);
+    if (store != PF_STORE_FP32) return PF_ERR_ARG;
+    free_matrices(c);
+    c->store = store;
+    c->k_shared = false;
+    const size_t ns = (size_t)pf_sym_per(c->ntot) * c->nels;
+    int rc = pf_alloc(c, (void **)&c->K32hi, ns * sizeof(float));
+    if (!rc) rc = pf_alloc(c, (void **)&c->K32lo, ns * sizeof(float));
+    if (!rc && !c->utemp32) rc = pf_alloc(c, (void **)&c->utemp32, (size_t)c->ntot * c->nels * sizeof(float));
+    return rc;
+}
+
+// After a direct FP32 assembly: the assembled diagonal from the element
+// diagonals dg the assembly wrote (the same sums as from FP64 matrices).
+static int finish_direct32(pf_ctx *c, const double *dg)
+{
+    const size_t nv = (size_t)c->neq + 1;
+    int rc = pf_launch_diag_elem(c, dg, c->diag0);
+    if (rc) return rc;
+    PF_HIP(hipMemcpyAsync(c->diag, c->diag0, nv * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    c->have_K = true;
+    c->pcg_ready = false;
+    free_arr(c, c->fixstore, nv);
+    c->nfix = 0;
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
+}
+
 // After a FP64 assembly or upload into K64: the assembled diagonal
 // (ascending element order, as SCATTER adds it) and, for FP32 storage, the
 // hi/lo split, after which the FP64 copy is released.
@@ -502,7 +535,7 @@ int pf_assemble_elastic(pf_ctx *c, double e, double v, const double *e_elem, int
     if (c->ntot != 3 * c->nod) return PF_ERR_ARG;
     if (!(v > -1.0 && v < 0.5)) return PF_ERR_ARG;
     PF_HIP(hipSetDevice(c->device));
-    int rc = alloc_k64(c, store);
+    int rc = alloc_assembly(c, store);
     if (rc) return rc;
     double *ee = nullptr;
     if (e_elem) {
@@ -515,13 +548,15 @@ int pf_assemble_elastic(pf_ctx *c, double e, double v, const double *e_elem, int
     const double v2 = v / (1.0 - v);
     const double vv = (1.0 - 2.0 * v) / (1.0 - v) * 0.5;
     const double den = 2.0 * (1.0 + v) * vv;
-    rc = pf_launch_assemble_elastic(c, v2, vv, den, e, ee);
+    // FP32 storage: the element diagonals go to utemp (ntot*nels doubles,
+    // free until the PCG runs)
+    rc = pf_launch_assemble_elastic(c, v2, vv, den, e, ee, c->utemp);
     if (ee) {
         (void)hipStreamSynchronize(c->stream);
         pf_free(c, ee, sizeof(double) * c->nels);
     }
     if (rc) return rc;
-    return finish_matrices(c);
+    return store == PF_STORE_FP32 ? finish_direct32(c, c->utemp) : finish_matrices(c);
 }
 
 int pf_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem, int store)
@@ -530,7 +565,7 @@ int pf_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double
     if (!c->g_coord || !c->g_g) return PF_ERR_STATE;
     if (c->ntot != c->nod || c->nod != 8) return PF_ERR_ARG;
     PF_HIP(hipSetDevice(c->device));
-    int rc = alloc_k64(c, store);
+    int rc = alloc_assembly(c, store);
     if (rc) return rc;
     double *kk = nullptr;
     if (k_elem) {
@@ -538,13 +573,13 @@ int pf_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double
         if (rc) return rc;
         PF_HIP(hipMemcpyAsync(kk, k_elem, sizeof(double) * c->nels, hipMemcpyHostToDevice, c->stream));
     }
-    rc = pf_launch_assemble_laplace(c, kx, ky, kz, kk);
+    rc = pf_launch_assemble_laplace(c, kx, ky, kz, kk, c->utemp);
     if (kk) {
         (void)hipStreamSynchronize(c->stream);
         pf_free(c, kk, sizeof(double) * c->nels);
     }
     if (rc) return rc;
-    return finish_matrices(c);
+    return store == PF_STORE_FP32 ? finish_direct32(c, c->utemp) : finish_matrices(c);
 }
 
 int pf_assemble_transient(pf_ctx *c, double theta, double dtim, int np_types, const double *prop,

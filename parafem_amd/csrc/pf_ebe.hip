@@ -33,6 +33,8 @@
 // (pf_reduce_grid in the public header) and no atomics, so results are
 // reproducible bit for bit.
 #include <cmath>
+#include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "pf_internal.h"
@@ -66,9 +68,10 @@ __device__ __forceinline__ double wave_max(double v)
 }
 
 // Folds NQ per-thread values of this block (sum or max per quantity) and
-// stores the block results in partials[qid*MAXG + blockIdx.x].
+// stores the block results in partials[qid*MAXG + bidx].
 template <int NQ>
-__device__ void block_to_partials(double (&v)[NQ], const int (&qid)[NQ], const bool (&ismax)[NQ], double *partials)
+__device__ void block_to_partials(double (&v)[NQ], const int (&qid)[NQ], const bool (&ismax)[NQ], double *partials,
+                                  int bidx)
 {
     __shared__ double ws[NQ][PF_BLOCK / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -83,7 +86,7 @@ __device__ void block_to_partials(double (&v)[NQ], const int (&qid)[NQ], const b
         for (int k = 0; k < NQ; ++k) {
             double s = ws[k][0];
             for (int w = 1; w < PF_BLOCK / 64; ++w) s = ismax[k] ? fmax(s, ws[k][w]) : __dadd_rn(s, ws[k][w]);
-            partials[qid[k] * MAXG + blockIdx.x] = s;
+            partials[qid[k] * MAXG + bidx] = s;
         }
     }
 }
@@ -178,16 +181,15 @@ __device__ __forceinline__ double bpermute64(int addr, double v)
 // d = NT/2 entries, shared by rows i and i + NT/2, are stored once).  The
 // products go out with non-temporal stores (they are read back by the
 // scatter, not by this kernel): 2% faster FP64 products, 0.5-1% FP32.
-template <int NT, int KIND, bool GATHER, typename OUT, bool LOOP = false, int EPLT = Lanes<NT, KIND>::EPL>
-__global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
-                                                 const float *__restrict__ Klo, const int *__restrict__ gg,
-                                                 const double *__restrict__ src, const float *__restrict__ src32,
-                                                 OUT *__restrict__ out, int nels, long kstride,
-                                                 const pf_state *__restrict__ st, int gate, long nlb, int rowmajor)
+// ebe_block forms the products of logical block lb (PF_BLOCK/64 * EPW * EPL
+// consecutive elements); the kernels below choose lb.
+template <int NT, int KIND, bool GATHER, typename OUT, int EPL>
+__device__ __forceinline__ void ebe_block(long lb, const double *__restrict__ K64, const float *__restrict__ Khi,
+                                          const float *__restrict__ Klo, const int *__restrict__ gg,
+                                          const double *__restrict__ src, const float *__restrict__ src32,
+                                          OUT *__restrict__ out, int nels, long kstride, int rowmajor)
 {
-    if (skip(st, gate)) return;
     using L = Lanes<NT, KIND>;
-    constexpr int EPL = EPLT;
     // packed layout: Q quads and R more full diagonals, then the half one (d = H)
     constexpr int D = pf_sym_diags(NT), Q = pf_sym_quads(NT), R = pf_sym_rem(NT), H = NT / 2;
     constexpr long PER = pf_sym_per(NT);
@@ -198,10 +200,6 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
     const int sub = lane / L::W, li = lane % L::W;
     const int i = li < NT ? li : NT - 1;
     const int gb = sub * L::W;   // first lane of the element's group
-    // nlb logical blocks of PF_BLOCK/64 * EPW * EPL elements; with LOOP a
-    // grid smaller than nlb (the gated true-residual products) loops over
-    // them, otherwise every workgroup takes one.
-    auto body = [&](long lb) {
     const long ebase = (lb * (PF_BLOCK / 64) + wave) * L::EPW * EPL;
     long ev[EPL], ec[EPL];
     int g[EPL];
@@ -275,9 +273,11 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
             float acc = 0.0f;
 #pragma unroll
             for (int d = 0; d < D; ++d) acc = __fmaf_rn(kd[t][d], pg[i + d], acc);
+            int j = i;   // (i - d) mod NT: the lane sending K(i-d, i) p(i)
 #pragma unroll
             for (int d = 1; d < D - 1; ++d) {
-                const int from = gb + (i >= d ? i - d : i - d + NT);
+                j = j == 0 ? NT - 1 : j - 1;
+                const int from = gb + j;
                 const float tt = __fmul_rn(kd[t][d], pv[t]);
                 acc = __fadd_rn(acc, __int_as_float(__builtin_amdgcn_ds_bpermute(from << 2, __float_as_int(tt))));
             }
@@ -338,9 +338,11 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
             double acc = 0.0;
 #pragma unroll
             for (int d = 0; d < D; ++d) acc = __fma_rn(kd[t][d], pg[i + d], acc);
+            int j = i;   // (i - d) mod NT
 #pragma unroll
             for (int d = 1; d < D - 1; ++d) {
-                const int from = gb + (i >= d ? i - d : i - d + NT);
+                j = j == 0 ? NT - 1 : j - 1;
+                const int from = gb + j;
                 acc = __dadd_rn(acc, bpermute64(from << 2, __dmul_rn(kd[t][d], pv[t])));
             }
             if (ev[t] < nels && li < NT) {
@@ -351,15 +353,27 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64
             }
         }
     }
-    };
+}
+
+// nlb logical blocks; with LOOP a grid smaller than nlb (the gated
+// true-residual products) loops over them, otherwise every workgroup takes
+// one.
+template <int NT, int KIND, bool GATHER, typename OUT, bool LOOP = false, int EPLT = Lanes<NT, KIND>::EPL>
+__global__ __launch_bounds__(PF_BLOCK) void k_ebe(const double *__restrict__ K64, const float *__restrict__ Khi,
+                                                 const float *__restrict__ Klo, const int *__restrict__ gg,
+                                                 const double *__restrict__ src, const float *__restrict__ src32,
+                                                 OUT *__restrict__ out, int nels, long kstride,
+                                                 const pf_state *__restrict__ st, int gate, long nlb, int rowmajor)
+{
+    if (skip(st, gate)) return;
     const long lb0 = xcd_block(blockIdx.x, gridDim.x);
     if constexpr (LOOP) {
         for (long lb = lb0; lb < nlb; lb += gridDim.x) {
-            body(lb);
+            ebe_block<NT, KIND, GATHER, OUT, EPLT>(lb, K64, Khi, Klo, gg, src, src32, out, nels, kstride, rowmajor);
             __builtin_amdgcn_wave_barrier();   // this wave's LDS reads finish before the next block's writes
         }
     } else {
-        body(lb0);
+        ebe_block<NT, KIND, GATHER, OUT, EPLT>(lb0, K64, Khi, Klo, gg, src, src32, out, nels, kstride, rowmajor);
     }
 }
 
@@ -411,20 +425,20 @@ __device__ __forceinline__ double slot_sum_lds(const int *lds, int a0, const UT 
 // contiguous range of the slot list) into LDS with whole-line loads, then
 // every lane reads its own slots from there (more than SC_CAP of them: read
 // from memory as they are).
+// Block lb of G (the problem's reduction grid) does the work thread
+// lb*PF_BLOCK + t of a G-block grid would do.
 constexpr int SC_CAP = 1024;
 template <typename UT>
-__global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__ ptr, const int *__restrict__ slot,
-                                                        const UT *__restrict__ utemp,
-                                                        const double *__restrict__ fixstore,
-                                                        const double *__restrict__ p, double *__restrict__ u,
-                                                        int neq, int fix_zero, double *partials, const pf_state *st)
+__device__ __forceinline__ void scatter_pu_block(int lb, int G, const int *__restrict__ ptr,
+                                                 const int *__restrict__ slot, const UT *__restrict__ utemp,
+                                                 const double *__restrict__ fixstore, const double *__restrict__ p,
+                                                 double *__restrict__ u, int neq, int fix_zero, double *partials)
 {
-    if (skip(st, RUN_LIVE)) return;
     __shared__ int sls[PF_BLOCK / 64][SC_CAP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double acc[1] = {0.0};
-    const int stride = gridDim.x * PF_BLOCK;
-    for (int q0 = blockIdx.x * PF_BLOCK + wave * 64; q0 < neq; q0 += stride) {
+    const int stride = G * PF_BLOCK;
+    for (int q0 = lb * PF_BLOCK + wave * 64; q0 < neq; q0 += stride) {
         const int q = q0 + lane;
         const bool live = q < neq;
         const int a = live ? ptr[q] : 0, b = live ? ptr[q + 1] : 0;
@@ -451,12 +465,22 @@ __global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__
     }
     constexpr int qid[1] = {PF_Q_PU};
     constexpr bool mx[1] = {false};
-    block_to_partials<1>(acc, qid, mx, partials);
+    block_to_partials<1>(acc, qid, mx, partials, lb);
 }
 
-__global__ __launch_bounds__(PF_BLOCK) void k_fold_alpha(const double *partials, int G, pf_state *st)
+template <typename UT>
+__global__ __launch_bounds__(PF_BLOCK) void k_scatter_pu(const int *__restrict__ ptr, const int *__restrict__ slot,
+                                                        const UT *__restrict__ utemp,
+                                                        const double *__restrict__ fixstore,
+                                                        const double *__restrict__ p, double *__restrict__ u,
+                                                        int neq, int fix_zero, double *partials, const pf_state *st)
 {
     if (skip(st, RUN_LIVE)) return;
+    scatter_pu_block<UT>(blockIdx.x, gridDim.x, ptr, slot, utemp, fixstore, p, u, neq, fix_zero, partials);
+}
+
+__device__ __forceinline__ void fold_alpha_block(const double *partials, int G, pf_state *st)
+{
     constexpr int qid[1] = {PF_Q_PU};
     constexpr bool mx[1] = {false};
     double out[1];
@@ -471,22 +495,27 @@ __global__ __launch_bounds__(PF_BLOCK) void k_fold_alpha(const double *partials,
     }
 }
 
+__global__ __launch_bounds__(PF_BLOCK) void k_fold_alpha(const double *partials, int G, pf_state *st)
+{
+    if (skip(st, RUN_LIVE)) return;
+    fold_alpha_block(partials, G, st);
+}
+
 // x/r/d updates and the partial (r,d) with, for checon_par, max|xnew| and
 // max|xnew-x|, or, for the residual rule, (r,r).
 template <bool CHECON>
-__global__ __launch_bounds__(PF_BLOCK) void k_update(double *__restrict__ x, double *__restrict__ r,
-                                                    double *__restrict__ d, const double *__restrict__ p,
-                                                    const double *__restrict__ u, const double *__restrict__ dinv,
-                                                    int neq, double *partials, const pf_state *st)
+__device__ __forceinline__ void update_block(int lb, int G, double *__restrict__ x, double *__restrict__ r,
+                                             double *__restrict__ d, const double *__restrict__ p,
+                                             const double *__restrict__ u, const double *__restrict__ dinv, int neq,
+                                             double *partials, const pf_state *st)
 {
-    if (skip(st, RUN_LIVE)) return;
     const double alpha = st->alpha;
     constexpr int NQ = CHECON ? 3 : 2;
     double v[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) v[k] = 0.0;
-    const int stride = gridDim.x * PF_BLOCK;
-    for (int q = blockIdx.x * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
+    const int stride = G * PF_BLOCK;
+    for (int q = lb * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
         const double xo = x[q];
         const double xn = __dadd_rn(xo, __dmul_rn(p[q], alpha));
         const double rq = __dsub_rn(r[q], __dmul_rn(u[q], alpha));
@@ -505,12 +534,22 @@ __global__ __launch_bounds__(PF_BLOCK) void k_update(double *__restrict__ x, dou
     if constexpr (CHECON) {
         constexpr int qid[3] = {PF_Q_RD, PF_Q_MAXX, PF_Q_MAXDX};
         constexpr bool mx[3] = {false, true, true};
-        block_to_partials<3>(v, qid, mx, partials);
+        block_to_partials<3>(v, qid, mx, partials, lb);
     } else {
         constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
         constexpr bool mx[2] = {false, false};
-        block_to_partials<2>(v, qid, mx, partials);
+        block_to_partials<2>(v, qid, mx, partials, lb);
     }
+}
+
+template <bool CHECON>
+__global__ __launch_bounds__(PF_BLOCK) void k_update(double *__restrict__ x, double *__restrict__ r,
+                                                    double *__restrict__ d, const double *__restrict__ p,
+                                                    const double *__restrict__ u, const double *__restrict__ dinv,
+                                                    int neq, double *partials, const pf_state *st)
+{
+    if (skip(st, RUN_LIVE)) return;
+    update_block<CHECON>(blockIdx.x, gridDim.x, x, r, d, p, u, dinv, neq, partials, st);
 }
 
 // End of an iteration: beta = rd/up, the stopping rule, the count.
@@ -529,9 +568,8 @@ __device__ void finish_iteration(pf_state *st, double rd, double rr, double maxx
 }
 
 template <bool CHECON>
-__global__ __launch_bounds__(PF_BLOCK) void k_fold_update(const double *partials, int G, pf_state *st)
+__device__ __forceinline__ void fold_update_block(const double *partials, int G, pf_state *st)
 {
-    if (skip(st, RUN_LIVE)) return;
     double out[4] = {0.0, 0.0, 0.0, 0.0};   // rd, rr, max|xnew|, max|xnew-x|
     if constexpr (CHECON) {
         constexpr int qid[3] = {PF_Q_RD, PF_Q_MAXX, PF_Q_MAXDX};
@@ -567,20 +605,25 @@ __global__ __launch_bounds__(PF_BLOCK) void k_fold_update(const double *partials
     }
 }
 
+template <bool CHECON>
+__global__ __launch_bounds__(PF_BLOCK) void k_fold_update(const double *partials, int G, pf_state *st)
+{
+    if (skip(st, RUN_LIVE)) return;
+    fold_update_block<CHECON>(partials, G, st);
+}
+
 // True residual r = f - K x (K from hi + lo, or the fixed-row override
 // x*store), d = dinv*r, partial (r,d), (r,r).
-__global__ __launch_bounds__(PF_BLOCK) void k_scatter_res(const int *__restrict__ ptr, const int *__restrict__ slot,
-                                                         const double *__restrict__ utemp,
-                                                         const double *__restrict__ fixstore,
-                                                         const double *__restrict__ x, const double *__restrict__ f,
-                                                         double *__restrict__ r, double *__restrict__ d,
-                                                         const double *__restrict__ dinv, int neq, double *partials,
-                                                         const pf_state *st)
+__device__ __forceinline__ void scatter_res_block(int lb, int G, const int *__restrict__ ptr,
+                                                  const int *__restrict__ slot, const double *__restrict__ utemp,
+                                                  const double *__restrict__ fixstore, const double *__restrict__ x,
+                                                  const double *__restrict__ f, double *__restrict__ r,
+                                                  double *__restrict__ d, const double *__restrict__ dinv, int neq,
+                                                  double *partials)
 {
-    if (skip(st, RUN_REFINE)) return;
     double v[2] = {0.0, 0.0};
-    const int stride = gridDim.x * PF_BLOCK;
-    for (int q = blockIdx.x * PF_BLOCK + threadIdx.x; q < neq; q += stride) {
+    const int stride = G * PF_BLOCK;
+    for (int q = lb * PF_BLOCK + threadIdx.x; q < neq; q += stride) {
         double s = slot_sum(slot, utemp, ptr[q], ptr[q + 1]);
         if (fixstore) {
             const double fs = fixstore[q + 1];
@@ -595,12 +638,23 @@ __global__ __launch_bounds__(PF_BLOCK) void k_scatter_res(const int *__restrict_
     }
     constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
     constexpr bool mx[2] = {false, false};
-    block_to_partials<2>(v, qid, mx, partials);
+    block_to_partials<2>(v, qid, mx, partials, lb);
 }
 
-__global__ __launch_bounds__(PF_BLOCK) void k_fold_refine(const double *partials, int G, pf_state *st)
+__global__ __launch_bounds__(PF_BLOCK) void k_scatter_res(const int *__restrict__ ptr, const int *__restrict__ slot,
+                                                         const double *__restrict__ utemp,
+                                                         const double *__restrict__ fixstore,
+                                                         const double *__restrict__ x, const double *__restrict__ f,
+                                                         double *__restrict__ r, double *__restrict__ d,
+                                                         const double *__restrict__ dinv, int neq, double *partials,
+                                                         const pf_state *st)
 {
     if (skip(st, RUN_REFINE)) return;
+    scatter_res_block(blockIdx.x, gridDim.x, ptr, slot, utemp, fixstore, x, f, r, d, dinv, neq, partials);
+}
+
+__device__ __forceinline__ void fold_refine_block(const double *partials, int G, pf_state *st)
+{
     constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
     constexpr bool mx[2] = {false, false};
     double out[2];
@@ -614,19 +668,31 @@ __global__ __launch_bounds__(PF_BLOCK) void k_fold_refine(const double *partials
     }
 }
 
+__global__ __launch_bounds__(PF_BLOCK) void k_fold_refine(const double *partials, int G, pf_state *st)
+{
+    if (skip(st, RUN_REFINE)) return;
+    fold_refine_block(partials, G, st);
+}
+
 // p = d + p*beta, and p32 = fl32(p) when the FP32 products gather it.
+__device__ __forceinline__ void pupdate_block(int lb, int G, double *__restrict__ p, float *__restrict__ p32,
+                                              const double *__restrict__ d, int neq, const pf_state *__restrict__ st)
+{
+    const double beta = st->beta;
+    const int stride = G * PF_BLOCK;
+    for (int q = lb * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
+        const double pq = __dadd_rn(d[q], __dmul_rn(p[q], beta));
+        __builtin_nontemporal_store(pq, &p[q]);   // read next by the gathers, not from this L2 line
+        if (p32) __builtin_nontemporal_store((float)pq, &p32[q]);
+    }
+}
+
 __global__ __launch_bounds__(PF_BLOCK) void k_pupdate(double *__restrict__ p, float *__restrict__ p32,
                                                      const double *__restrict__ d, int neq,
                                                      const pf_state *__restrict__ st)
 {
     if (skip(st, RUN_LIVE)) return;
-    const double beta = st->beta;
-    const int stride = gridDim.x * PF_BLOCK;
-    for (int q = blockIdx.x * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
-        const double pq = __dadd_rn(d[q], __dmul_rn(p[q], beta));
-        __builtin_nontemporal_store(pq, &p[q]);   // read next by the gathers, not from this L2 line
-        if (p32) __builtin_nontemporal_store((float)pq, &p32[q]);
-    }
+    pupdate_block(blockIdx.x, gridDim.x, p, p32, d, neq, st);
 }
 
 // PCG start: dinv = 1/diag, d = dinv*r, p = d, x = x0 (or 0), f = r; partial
@@ -659,7 +725,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_pcg_init(double *__restrict__ x, c
     }
     constexpr int qid[2] = {PF_Q_RD, PF_Q_RR};
     constexpr bool mx[2] = {false, false};
-    block_to_partials<2>(v, qid, mx, partials);
+    block_to_partials<2>(v, qid, mx, partials, blockIdx.x);
 }
 
 __global__ __launch_bounds__(PF_BLOCK) void k_fold_init(const double *partials, int G, pf_state *st)
@@ -843,10 +909,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_reduce_prim(const double *__restri
     constexpr int qid[2] = {0, 1};
     if (mode == 2) {
         constexpr bool mx[2] = {true, true};
-        block_to_partials<2>(v, qid, mx, partials);
+        block_to_partials<2>(v, qid, mx, partials, blockIdx.x);
     } else {
         constexpr bool mx[2] = {false, false};
-        block_to_partials<2>(v, qid, mx, partials);
+        block_to_partials<2>(v, qid, mx, partials, blockIdx.x);
     }
 }
 
@@ -981,6 +1047,140 @@ int reduce_prim(pf_ctx *c, const double *a, double *b, int64_t n, int mode, doub
     out2[0] = c->scalar_host[0];
     out2[1] = c->scalar_host[1];
     return PF_OK;
+}
+
+// ------------------------------------------------- batched iterations
+// Several solves advanced by one launch per kernel (pf_batch_run).  Each
+// solve keeps its own buffers and state; a launch's blocks are dealt to the
+// solves in ranges (est: first logical element block of each solve in the
+// element launches; qst: first block of each solve in the equation
+// launches, G blocks per solve as its own launches use), and every block
+// runs the same body with the same local block index and grid size as the
+// solve's own launch would: the results are those of the solve alone, bit
+// for bit.
+struct pf_bslot {
+    const double *K64;
+    const float *Khi, *Klo;
+    const int *g_g, *csr_ptr, *slot;   // slot: the list matching the products' layout
+    double *x, *r, *p, *d, *u, *dinv, *f, *utemp;
+    float *p32, *utemp32;
+    const double *fixstore;
+    double *partials;
+    pf_state *st;
+    long kstride;
+    int nels, neq, G, fix_zero, rowmajor;
+};
+
+// The solve owning block b: the last solve whose first block is <= b (at
+// most 64 solves, one lane each).
+template <typename I>
+__device__ __forceinline__ int bslot_find(const I *__restrict__ start, int ns, long b)
+{
+    const int lane = threadIdx.x & 63;
+    const bool le = lane < ns && (long)start[lane] <= b;
+    return __popcll(__ballot(le)) - 1;
+}
+
+// Element products of every solve: the iteration's product of p (gate
+// RUN_LIVE) or, with LOOP, the true-residual product of x (gate RUN_REFINE,
+// a grid smaller than the blocks, looping).
+template <int NT, int KIND, typename OUT, bool LOOP>
+__global__ __launch_bounds__(PF_BLOCK) void kb_ebe(const pf_bslot *__restrict__ T, const long *__restrict__ est,
+                                                  int ns, long total, int gate)
+{
+    constexpr int EPL = Lanes<NT, KIND>::EPL;
+    auto one = [&](long gb) {
+        const int s = bslot_find(est, ns, gb);
+        const pf_bslot &S = T[s];
+        if (skip(S.st, gate)) return;
+        const long lb = gb - est[s];
+        if constexpr (LOOP)
+            ebe_block<NT, KIND, true, OUT, EPL>(lb, S.K64, S.Khi, S.Klo, S.g_g, S.x, nullptr, (OUT *)S.utemp, S.nels,
+                                                S.kstride, S.rowmajor);
+        else if constexpr (KIND == KIND_F32)
+            ebe_block<NT, KIND, true, OUT, EPL>(lb, S.K64, S.Khi, S.Klo, S.g_g, S.p, S.p32, (OUT *)S.utemp32,
+                                                S.nels, S.kstride, S.rowmajor);
+        else
+            ebe_block<NT, KIND, true, OUT, EPL>(lb, S.K64, S.Khi, S.Klo, S.g_g, S.p, nullptr, (OUT *)S.utemp, S.nels,
+                                                S.kstride, S.rowmajor);
+    };
+    if constexpr (LOOP) {
+        // solve by solve, so that a solve not due for a replacement costs one
+        // state read per workgroup (the usual case: every solve skips)
+        for (int s = 0; s < ns; ++s) {
+            if (skip(T[s].st, gate)) continue;
+            const long end = s + 1 < ns ? est[s + 1] : total;
+            for (long gb = est[s] + blockIdx.x; gb < end; gb += gridDim.x) {
+                one(gb);
+                __builtin_amdgcn_wave_barrier();   // this wave's LDS reads finish before the next block's writes
+            }
+        }
+    } else {
+        one(xcd_block(blockIdx.x, gridDim.x));
+    }
+}
+
+template <typename UT>
+__global__ __launch_bounds__(PF_BLOCK) void kb_scatter_pu(const pf_bslot *__restrict__ T, const int *__restrict__ qst,
+                                                         int ns)
+{
+    const int s = bslot_find(qst, ns, blockIdx.x);
+    const pf_bslot &S = T[s];
+    if (skip(S.st, RUN_LIVE)) return;
+    const UT *ut;
+    if constexpr (sizeof(UT) == 4) ut = (const UT *)S.utemp32;
+    else ut = (const UT *)S.utemp;
+    scatter_pu_block<UT>(blockIdx.x - qst[s], S.G, S.csr_ptr, S.slot, ut, S.fixstore, S.p, S.u, S.neq, S.fix_zero,
+                         S.partials);
+}
+
+template <bool CHECON>
+__global__ __launch_bounds__(PF_BLOCK) void kb_update(const pf_bslot *__restrict__ T, const int *__restrict__ qst,
+                                                     int ns)
+{
+    const int s = bslot_find(qst, ns, blockIdx.x);
+    const pf_bslot &S = T[s];
+    if (skip(S.st, RUN_LIVE)) return;
+    update_block<CHECON>(blockIdx.x - qst[s], S.G, S.x, S.r, S.d, S.p, S.u, S.dinv, S.neq, S.partials, S.st);
+}
+
+__global__ __launch_bounds__(PF_BLOCK) void kb_scatter_res(const pf_bslot *__restrict__ T, const int *__restrict__ qst,
+                                                          int ns)
+{
+    const int s = bslot_find(qst, ns, blockIdx.x);
+    const pf_bslot &S = T[s];
+    if (skip(S.st, RUN_REFINE)) return;
+    scatter_res_block(blockIdx.x - qst[s], S.G, S.csr_ptr, S.slot, S.utemp, S.fixstore, S.x, S.f, S.r, S.d, S.dinv,
+                      S.neq, S.partials);
+}
+
+__global__ __launch_bounds__(PF_BLOCK) void kb_pupdate(const pf_bslot *__restrict__ T, const int *__restrict__ qst,
+                                                      int ns)
+{
+    const int s = bslot_find(qst, ns, blockIdx.x);
+    const pf_bslot &S = T[s];
+    if (skip(S.st, RUN_LIVE)) return;
+    pupdate_block(blockIdx.x - qst[s], S.G, S.p, S.p32, S.d, S.neq, S.st);
+}
+
+// One block per solve: 0 alpha, 1 update (checon), 2 update (residual),
+// 3 true-residual replacement.
+template <int WHICH>
+__global__ __launch_bounds__(PF_BLOCK) void kb_fold(const pf_bslot *__restrict__ T)
+{
+    const pf_bslot &S = T[blockIdx.x];
+    if (skip(S.st, WHICH == 3 ? RUN_REFINE : RUN_LIVE)) return;
+    if constexpr (WHICH == 0) fold_alpha_block(S.partials, S.G, S.st);
+    else if constexpr (WHICH == 1) fold_update_block<true>(S.partials, S.G, S.st);
+    else if constexpr (WHICH == 2) fold_update_block<false>(S.partials, S.G, S.st);
+    else fold_refine_block(S.partials, S.G, S.st);
+}
+
+// Copies of every solve's state (pf_batch_query's snapshot).
+__global__ void kb_snapshot(const pf_bslot *__restrict__ T, int ns, pf_state *__restrict__ out)
+{
+    const int s = blockIdx.x;
+    if (s < ns && threadIdx.x == 0) out[s] = *T[s].st;
 }
 
 }  // namespace
@@ -1457,6 +1657,352 @@ int pf_apply_raw(pf_ctx *c, int which, const double *x_host, double *y_host)
     if (!c || !x_host || !y_host || (which != 0 && which != 1)) return PF_ERR_ARG;
     if (!c->have_K || (which == 1 && !c->K64b)) return PF_ERR_STATE;
     return apply_host(c, x_host, y_host, which == 1 ? c->K64b : nullptr, false);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- batched iterations
+// pf_batch: one stream, a device table of up to PF_BATCH_MAX solves, and
+// the captured iteration chunk for the current table.
+struct pf_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_done = nullptr;                  // end of the last enqueued run
+    hipEvent_t ev_table = nullptr;                 // the last table upload has left the host buffer
+    hipEvent_t ev_snap = nullptr;
+    std::vector<hipEvent_t> ev_ctx;                // one per solve: its context's stream reached the run
+    pf_bslot *T = nullptr;                         // device table
+    long *est = nullptr;                           // device: first element block per solve (iteration
+                                                   // kind), then the same for the FP64 hi+lo products
+    int *qst = nullptr;                            // device: first equation block per solve
+    pf_bslot *T_host = nullptr;                    // pinned staging of the three (one allocation)
+    long *est_host = nullptr;
+    int *qst_host = nullptr;
+    std::vector<pf_bslot> cur;                     // the table on the device
+    std::vector<long> cur_est;                     // iteration-kind starts, then ACC-kind starts
+    std::vector<int> cur_qst;
+    int ns = 0, ntot = 0, precision = -1, stop = -1;
+    long total_eb = 0, total_ebr = 0;              // element blocks: iteration kind, true-residual kind
+    int total_qb = 0;
+    pf_state *snap = nullptr, *snap_host = nullptr;
+    int snap_n = 0;
+    bool snap_pending = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_iters = 0;
+};
+
+static const size_t kBatchTable = sizeof(pf_bslot) * PF_BATCH_MAX + 2 * sizeof(long) * PF_BATCH_MAX +
+                                  sizeof(int) * PF_BATCH_MAX;
+
+static void batch_graph_drop(pf_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->graph_iters = 0;
+}
+
+// Builds the table of the given contexts, uploads it when it changed and
+// orders the batch stream after each context's stream.
+static int batch_prepare(pf_batch *b, int nctx, pf_ctx *const *ctxs)
+{
+    if (!b || nctx < 1 || nctx > PF_BATCH_MAX || !ctxs) return PF_ERR_ARG;
+    const pf_ctx *c0 = ctxs[0];
+    if (!c0) return PF_ERR_ARG;
+    std::vector<pf_bslot> T(nctx);
+    std::vector<long> est(2 * (size_t)nctx);
+    std::vector<int> qst(nctx);
+    long eb = 0;
+    int qb = 0;
+    for (int k = 0; k < nctx; ++k) {
+        pf_ctx *c = ctxs[k];
+        if (!c || c->device != b->device) return PF_ERR_ARG;
+        for (int j = 0; j < k; ++j)
+            if (ctxs[j] == c) return PF_ERR_ARG;   // a solve appears once
+        if (!c->pcg_ready || c->timing) return PF_ERR_STATE;
+        if (c->dot_order != PF_DOT_TREE || c->ntot != c0->ntot || c->opts.precision != c0->opts.precision ||
+            c->opts.stop_mode != c0->opts.stop_mode)
+            return PF_ERR_ARG;
+        const bool mixed = c->opts.precision == PF_PREC_MIXED;
+        const int rm = c->csr_slot_t && !mixed ? 1 : 0;
+        pf_bslot &S = T[k];
+        S.K64 = c->K64;
+        S.Khi = c->K32hi;
+        S.Klo = c->K32lo;
+        S.g_g = c->g_g;
+        S.csr_ptr = c->csr_ptr;
+        S.slot = rm ? c->csr_slot_t : c->csr_slot;
+        S.x = c->x; S.r = c->r; S.p = c->p; S.d = c->d; S.u = c->u; S.dinv = c->dinv; S.f = c->f;
+        S.utemp = c->utemp;
+        S.p32 = mixed && c->ntot <= 24 ? c->p32 : nullptr;
+        S.utemp32 = c->utemp32;
+        S.fixstore = c->fixstore;
+        S.partials = c->partials;
+        S.st = c->state;
+        S.kstride = pf_k64_estride(c);
+        S.nels = c->nels;
+        S.neq = c->neq;
+        S.G = pf_grid_for(c->neq);
+        S.fix_zero = c->fix_zero ? 1 : 0;
+        S.rowmajor = rm;
+        qst[k] = qb;
+        qb += S.G;
+        if (mixed && !c->utemp32) return PF_ERR_STATE;
+    }
+    // elements per logical block of the iteration's product kind and of the
+    // FP64 hi+lo products (the true residual)
+    int EPB = 0, EPBR = 0;
+    const int prec = c0->opts.precision;
+    auto epb = [&](auto nt) {
+        constexpr int NT = decltype(nt)::value;
+        EPB = prec == PF_PREC_MIXED ? Lanes<NT, KIND_F32>::EPB
+            : prec == PF_PREC_FP64 ? Lanes<NT, KIND_F64>::EPB : Lanes<NT, KIND_ACC>::EPB;
+        EPBR = Lanes<NT, KIND_ACC>::EPB;
+        static_assert(Lanes<NT, KIND_ACC>::EPB == Lanes<NT, KIND_WIDE>::EPB, "one lane layout");
+    };
+    switch (c0->ntot) {
+    case 60: epb(std::integral_constant<int, 60>()); break;
+    case 24: epb(std::integral_constant<int, 24>()); break;
+    case 8: epb(std::integral_constant<int, 8>()); break;
+    default: return PF_ERR_ARG;
+    }
+    long ebr = 0;
+    for (int k = 0; k < nctx; ++k) {
+        est[k] = eb;
+        est[nctx + k] = ebr;
+        eb += (T[k].nels + EPB - 1) / EPB;
+        ebr += (T[k].nels + EPBR - 1) / EPBR;
+    }
+    PF_HIP(hipSetDevice(b->device));
+    const bool same = nctx == b->ns && std::memcmp(T.data(), b->cur.data(), sizeof(pf_bslot) * nctx) == 0 &&
+                      est == b->cur_est && qst == b->cur_qst;
+    if (!same) {
+        PF_HIP(hipEventSynchronize(b->ev_table));   // the staging buffer is free again
+        std::memcpy(b->T_host, T.data(), sizeof(pf_bslot) * nctx);
+        std::memcpy(b->est_host, est.data(), sizeof(long) * nctx);
+        std::memcpy(b->est_host + PF_BATCH_MAX, est.data() + nctx, sizeof(long) * nctx);
+        std::memcpy(b->qst_host, qst.data(), sizeof(int) * nctx);
+        PF_HIP(hipMemcpyAsync(b->T, b->T_host, kBatchTable, hipMemcpyHostToDevice, b->stream));
+        PF_HIP(hipEventRecord(b->ev_table, b->stream));
+        if (eb != b->total_eb || ebr != b->total_ebr || qb != b->total_qb || nctx != b->ns || c0->ntot != b->ntot ||
+            c0->opts.precision != b->precision || c0->opts.stop_mode != b->stop)
+            batch_graph_drop(b);
+        b->cur = T;
+        b->cur_est = est;
+        b->cur_qst = qst;
+        b->ns = nctx;
+        b->total_eb = eb;
+        b->total_ebr = ebr;
+        b->total_qb = qb;
+        b->ntot = c0->ntot;
+        b->precision = c0->opts.precision;
+        b->stop = c0->opts.stop_mode;
+    }
+    // the solves' own streams first (their set-up, pf_pcg_begin)
+    for (int k = 0; k < nctx; ++k) {
+        PF_HIP(hipEventRecord(b->ev_ctx[k], ctxs[k]->stream));
+        PF_HIP(hipStreamWaitEvent(b->stream, b->ev_ctx[k], 0));
+    }
+    return PF_OK;
+}
+
+// After a run: each context's stream waits for it (pf_pcg_x, pf_reset and
+// the rest of a context's calls see the batch's work).
+static int batch_release(pf_batch *b, int nctx, pf_ctx *const *ctxs)
+{
+    PF_HIP(hipEventRecord(b->ev_done, b->stream));
+    for (int k = 0; k < nctx; ++k) {
+        ctxs[k]->query_pending = false;
+        PF_HIP(hipStreamWaitEvent(ctxs[k]->stream, b->ev_done, 0));
+    }
+    return PF_OK;
+}
+
+template <int NT, int KIND, typename OUT>
+static void batch_launch_ebe(pf_batch *b)
+{
+    hipLaunchKernelGGL((kb_ebe<NT, KIND, OUT, false>), dim3((unsigned)b->total_eb), dim3(PF_BLOCK), 0, b->stream, b->T,
+                       b->est, b->ns, b->total_eb, (int)RUN_LIVE);
+}
+
+template <int NT>
+static void batch_launch_refine(pf_batch *b)
+{
+    // usually does nothing: few workgroups that loop over every solve's blocks
+    const int grid = (int)std::min<long>(b->total_ebr, 2048);
+    hipLaunchKernelGGL((kb_ebe<NT, KIND_ACC, double, true>), dim3(grid), dim3(PF_BLOCK), 0, b->stream, b->T,
+                       b->est + PF_BATCH_MAX, b->ns, b->total_ebr, (int)RUN_REFINE);
+}
+
+template <int NT>
+static void batch_iteration_nt(pf_batch *b)
+{
+    const int prec = b->precision;
+    if (prec == PF_PREC_MIXED) batch_launch_ebe<NT, KIND_F32, float>(b);
+    else if (prec == PF_PREC_MIXED64) batch_launch_ebe<NT, KIND_WIDE, double>(b);
+    else if (prec == PF_PREC_SPLIT) batch_launch_ebe<NT, KIND_ACC, double>(b);
+    else batch_launch_ebe<NT, KIND_F64, double>(b);
+    const dim3 gq((unsigned)b->total_qb), gs((unsigned)b->ns), blk(PF_BLOCK);
+    if (prec == PF_PREC_MIXED)
+        hipLaunchKernelGGL(kb_scatter_pu<float>, gq, blk, 0, b->stream, b->T, b->qst, b->ns);
+    else
+        hipLaunchKernelGGL(kb_scatter_pu<double>, gq, blk, 0, b->stream, b->T, b->qst, b->ns);
+    hipLaunchKernelGGL(kb_fold<0>, gs, blk, 0, b->stream, b->T);
+    if (b->stop == PF_STOP_CHECON) {
+        hipLaunchKernelGGL(kb_update<true>, gq, blk, 0, b->stream, b->T, b->qst, b->ns);
+        hipLaunchKernelGGL(kb_fold<1>, gs, blk, 0, b->stream, b->T);
+    } else {
+        hipLaunchKernelGGL(kb_update<false>, gq, blk, 0, b->stream, b->T, b->qst, b->ns);
+        hipLaunchKernelGGL(kb_fold<2>, gs, blk, 0, b->stream, b->T);
+    }
+    if (prec == PF_PREC_MIXED || prec == PF_PREC_MIXED64) {
+        batch_launch_refine<NT>(b);
+        hipLaunchKernelGGL(kb_scatter_res, gq, blk, 0, b->stream, b->T, b->qst, b->ns);
+        hipLaunchKernelGGL(kb_fold<3>, gs, blk, 0, b->stream, b->T);
+    }
+    hipLaunchKernelGGL(kb_pupdate, gq, blk, 0, b->stream, b->T, b->qst, b->ns);
+}
+
+static int batch_iterations(pf_batch *b, int n)
+{
+    for (int it = 0; it < n; ++it) {
+        switch (b->ntot) {
+        case 60: batch_iteration_nt<60>(b); break;
+        case 24: batch_iteration_nt<24>(b); break;
+        default: batch_iteration_nt<8>(b); break;
+        }
+        PF_HIP(hipGetLastError());
+    }
+    return PF_OK;
+}
+
+extern "C" {
+
+int pf_batch_create(int device, pf_batch **out)
+{
+    if (!out) return PF_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return PF_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) return PF_ERR_NODEVICE;
+    PF_HIP(hipSetDevice(device));
+    pf_batch *b = new pf_batch();
+    b->device = device;
+    b->ev_ctx.assign(PF_BATCH_MAX, nullptr);
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_table, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_snap, hipEventDisableTiming);
+    for (int k = 0; k < PF_BATCH_MAX && e == hipSuccess; ++k)
+        e = hipEventCreateWithFlags(&b->ev_ctx[k], hipEventDisableTiming);
+    void *dt = nullptr, *ht = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&dt, kBatchTable);
+    if (e == hipSuccess) e = hipHostMalloc(&ht, kBatchTable, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->snap, sizeof(pf_state) * PF_BATCH_MAX);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b->snap_host, sizeof(pf_state) * PF_BATCH_MAX, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventRecord(b->ev_table, b->stream);
+    if (dt) {
+        b->T = (pf_bslot *)dt;
+        b->est = (long *)((char *)dt + sizeof(pf_bslot) * PF_BATCH_MAX);
+        b->qst = (int *)((char *)b->est + 2 * sizeof(long) * PF_BATCH_MAX);
+    }
+    if (ht) {
+        b->T_host = (pf_bslot *)ht;
+        b->est_host = (long *)((char *)ht + sizeof(pf_bslot) * PF_BATCH_MAX);
+        b->qst_host = (int *)((char *)b->est_host + 2 * sizeof(long) * PF_BATCH_MAX);
+    }
+    if (e != hipSuccess) {
+        const int rc = pf_set_hip_error(e, "pf_batch_create", __FILE__, __LINE__);
+        pf_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return PF_OK;
+}
+
+int pf_batch_destroy(pf_batch *b)
+{
+    if (!b) return PF_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_graph_drop(b);
+    if (b->T) (void)hipFree(b->T);
+    if (b->T_host) (void)hipHostFree(b->T_host);
+    if (b->snap) (void)hipFree(b->snap);
+    if (b->snap_host) (void)hipHostFree(b->snap_host);
+    for (hipEvent_t e : b->ev_ctx)
+        if (e) (void)hipEventDestroy(e);
+    if (b->ev_done) (void)hipEventDestroy(b->ev_done);
+    if (b->ev_table) (void)hipEventDestroy(b->ev_table);
+    if (b->ev_snap) (void)hipEventDestroy(b->ev_snap);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return PF_OK;
+}
+
+int pf_batch_run(pf_batch *b, int nctx, pf_ctx *const *ctxs, int n_iters)
+{
+    if (!b || n_iters < 0) return PF_ERR_ARG;
+    int rc = batch_prepare(b, nctx, ctxs);
+    if (rc) return rc;
+    rc = batch_iterations(b, n_iters);
+    if (rc) return rc;
+    return batch_release(b, nctx, ctxs);
+}
+
+int pf_batch_run_graph(pf_batch *b, int nctx, pf_ctx *const *ctxs, int n_iters)
+{
+    if (!b || n_iters < 1) return PF_ERR_ARG;
+    int rc = batch_prepare(b, nctx, ctxs);
+    if (rc) return rc;
+    if (!b->gexec || b->graph_iters != n_iters) {
+        batch_graph_drop(b);
+        PF_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        rc = batch_iterations(b, n_iters);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : pf_set_hip_error(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        }
+        const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            b->gexec = nullptr;
+            return pf_set_hip_error(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+        b->graph = g;
+        b->graph_iters = n_iters;
+    }
+    PF_HIP(hipGraphLaunch(b->gexec, b->stream));
+    return batch_release(b, nctx, ctxs);
+}
+
+int pf_batch_query(pf_batch *b, pf_pcg_result *res, int *ready)
+{
+    if (!b || !res || !ready) return PF_ERR_ARG;
+    if (b->ns < 1) return PF_ERR_STATE;
+    *ready = 0;
+    if (!b->snap_pending) {
+        hipLaunchKernelGGL(kb_snapshot, dim3(b->ns), dim3(64), 0, b->stream, b->T, b->ns, b->snap);
+        PF_HIP(hipGetLastError());
+        PF_HIP(hipMemcpyAsync(b->snap_host, b->snap, sizeof(pf_state) * b->ns, hipMemcpyDeviceToHost, b->stream));
+        PF_HIP(hipEventRecord(b->ev_snap, b->stream));
+        b->snap_pending = true;
+        b->snap_n = b->ns;
+    }
+    const hipError_t q = hipEventQuery(b->ev_snap);
+    if (q == hipErrorNotReady) return PF_OK;
+    if (q != hipSuccess) return pf_set_hip_error(q, "hipEventQuery", __FILE__, __LINE__);
+    b->snap_pending = false;
+    for (int k = 0; k < b->snap_n; ++k) fill_result(b->snap_host[k], &res[k]);
+    *ready = 1;
+    return PF_OK;
 }
 
 }  // extern "C"

@@ -232,8 +232,13 @@ void pf_cache_release(pf_ctx *c);
 int pf_grid_for(int64_t n);
 
 // kernels launched across translation units ---------------------------------
-int pf_launch_assemble_elastic(pf_ctx *c, double v2, double vv, double den, double e0, const double *e_elem_dev);
-int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem_dev);
+// Element assembly into K64 (FP64 storage) or, for FP32 storage, straight
+// into K32hi/K32lo with the FP64 element diagonals in dg (ntot*nels).
+int pf_launch_assemble_elastic(pf_ctx *c, double v2, double vv, double den, double e0, const double *e_elem_dev,
+                               double *dg);
+int pf_launch_assemble_laplace(pf_ctx *c, double kx, double ky, double kz, const double *k_elem_dev, double *dg);
+// Assembled diagonal (leading 0 slot) from element diagonals dg(ntot,nels).
+int pf_launch_diag_elem(pf_ctx *c, const double *dg, double *diag);
 int pf_launch_split_fp32(pf_ctx *c);
 int pf_launch_diag(pf_ctx *c, double *diag_dev);
 // storkm_pp (column-major, nels_chunk elements starting at e0) <-> internal

@@ -8,9 +8,14 @@
 !     src/modules/mpi/gather_scatter.f90:547-993
 !   DOT_PRODUCT_P(a,b), SUM_P(a), checon_par(loads,tol,converged,oldlds)
 !     src/modules/mpi/maths.f90:168-216, 271-315, 999-1069
-! Vectors and element arrays live in device memory behind TYPE(pf_dvec)
-! handles instead of host arrays; the one device context is module state, as
-! the reference keeps its halo tables as module state.
+! Each of these is generic over two forms: TYPE(pf_dvec) handles of device
+! vectors (the fast form: the vectors stay in HBM), and REAL(iwp) host arrays
+! with the reference's own shapes -- p_pp(neq_pp), pmul_pp(ntot,nels_pp),
+! u_pp(neq_pp), utemp_pp(ntot,nels_pp) -- so that p121.f90:89-104 compiles
+! unchanged against this module (the host forms stage the arrays through
+! device buffers and return the same bits as the device forms).  The one
+! device context is module state, as the reference keeps its halo tables as
+! module state.
 !------------------------------------------------------------------------------
 MODULE pf_gpu
   USE, INTRINSIC :: ISO_C_BINDING
@@ -22,6 +27,7 @@ MODULE pf_gpu
   INTEGER(c_int), PARAMETER, PUBLIC :: PF_STORE_FP64 = 0, PF_STORE_FP32 = 1
   INTEGER(c_int), PARAMETER, PUBLIC :: PF_STOP_CHECON = 0, PF_STOP_RELRES = 1
   INTEGER(c_int), PARAMETER, PUBLIC :: PF_PREC_FP64 = 0, PF_PREC_MIXED = 1
+  INTEGER(c_int), PARAMETER, PUBLIC :: PF_DOT_TREE = 0, PF_DOT_SEQ = 1
 
   TYPE, BIND(C), PUBLIC :: pf_pcg_opts
     INTEGER(c_int) :: limit
@@ -50,12 +56,36 @@ MODULE pf_gpu
   END TYPE pf_dvec
 
   TYPE(c_ptr), SAVE :: ctx = c_null_ptr
+  ! Sizes of the steering on the device (pf_steer): the host-array forms
+  ! check their arguments against them before any kernel runs.
+  INTEGER(c_int64_t), SAVE :: m_neq = 0, m_nslot = 0
+  ! Device staging buffers of the host-array forms (grown on demand).
+  TYPE(pf_dvec), SAVE :: st_a, st_b
 
   PUBLIC :: pf_init, pf_finalize, pf_alloc, pf_release, pf_put, pf_get
   PUBLIC :: pf_mesh, pf_steer, pf_assemble_elastic, pf_assemble_laplace
   PUBLIC :: pf_storkm, pf_storkm_get, pf_diag, pf_fixed, pf_pcg, pf_check, pcg_km
   PUBLIC :: gather, scatter, scatter_noadd, elem_matvec
-  PUBLIC :: dot_product_p, sum_p, checon_par
+  PUBLIC :: dot_product_p, sum_p, checon_par, pf_dot_order, pf_matvec
+
+  INTERFACE gather
+    MODULE PROCEDURE gather_dev, gather_host
+  END INTERFACE
+  INTERFACE scatter
+    MODULE PROCEDURE scatter_dev, scatter_host
+  END INTERFACE
+  INTERFACE scatter_noadd
+    MODULE PROCEDURE scatter_noadd_dev, scatter_noadd_host
+  END INTERFACE
+  INTERFACE dot_product_p
+    MODULE PROCEDURE dot_product_p_dev, dot_product_p_host
+  END INTERFACE
+  INTERFACE sum_p
+    MODULE PROCEDURE sum_p_dev, sum_p_host
+  END INTERFACE
+  INTERFACE checon_par
+    MODULE PROCEDURE checon_par_dev, checon_par_host
+  END INTERFACE
   PUBLIC :: vec_fill, vec_waxpy, vec_mul, vec_recip
   PUBLIC :: pf_stress, calc_nodes_pp, scatter_nodes, dismsh_ensi_p
   PUBLIC :: pf_assemble_transient, pf_apply_raw, pf_plastic_update, pf_fixed_rows, pf_cg_warm
@@ -196,6 +226,11 @@ MODULE pf_gpu
       INTEGER(c_int) :: conv
       INTEGER(c_int64_t), VALUE :: n
     END FUNCTION
+    INTEGER(c_int) FUNCTION c_dot_order_set(c, order) BIND(C, name="pf_dot_order_set")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: c
+      INTEGER(c_int), VALUE :: order
+    END FUNCTION
     INTEGER(c_int) FUNCTION c_vec_fill(c, x, val, n) BIND(C, name="pf_vec_fill")
       IMPORT :: c_int, c_ptr, c_int64_t, c_double
       TYPE(c_ptr), VALUE :: c, x
@@ -310,8 +345,12 @@ CONTAINS
   !> SHUTDOWN counterpart.
   SUBROUTINE pf_finalize()
     INTEGER(c_int) :: rc
-    rc = c_destroy(ctx)
+    rc = c_destroy(ctx)            ! frees every device buffer, the staging ones too
     ctx = c_null_ptr
+    st_a = pf_dvec()
+    st_b = pf_dvec()
+    m_neq = 0
+    m_nslot = 0
   END SUBROUTINE pf_finalize
 
   SUBROUTINE pf_alloc(v, n)
@@ -357,6 +396,8 @@ CONTAINS
     INTEGER, INTENT(IN) :: g_g_pp(:, :), neq
     CALL pf_check(c_steer_set(ctx, INT(SIZE(g_g_pp, 1), c_int), INT(SIZE(g_g_pp, 2), c_int), &
                               INT(neq, c_int), g_g_pp), 'pf_steer_set')
+    m_neq = INT(neq, c_int64_t)
+    m_nslot = INT(SIZE(g_g_pp), c_int64_t)
   END SUBROUTINE pf_steer
 
   !> p121's element loop (p121.f90:54-64) on the device; e_elem optional.
@@ -438,11 +479,11 @@ CONTAINS
   END SUBROUTINE pf_fixed
 
   !> GATHER(p_pp, pmul_pp): pmul(:,e) = p(g_g(:,e)), 0 for restrained.
-  SUBROUTINE gather(p_pp, pmul_pp)
+  SUBROUTINE gather_dev(p_pp, pmul_pp)
     TYPE(pf_dvec), INTENT(IN) :: p_pp
     TYPE(pf_dvec), INTENT(INOUT) :: pmul_pp
     CALL pf_check(c_gather(ctx, p_pp%ptr, pmul_pp%ptr), 'GATHER')
-  END SUBROUTINE gather
+  END SUBROUTINE gather_dev
 
   !> utemp(:,e) = MATMUL(storkm(:,:,e), pmul(:,e)) for every element.
   SUBROUTINE elem_matvec(pmul_pp, utemp_pp)
@@ -452,37 +493,37 @@ CONTAINS
   END SUBROUTINE elem_matvec
 
   !> SCATTER(u_pp, utemp_pp): u accumulates, ascending element order.
-  SUBROUTINE scatter(u_pp, utemp_pp)
+  SUBROUTINE scatter_dev(u_pp, utemp_pp)
     TYPE(pf_dvec), INTENT(INOUT) :: u_pp
     TYPE(pf_dvec), INTENT(IN) :: utemp_pp
     CALL pf_check(c_scatter(ctx, u_pp%ptr, utemp_pp%ptr), 'SCATTER')
-  END SUBROUTINE scatter
+  END SUBROUTINE scatter_dev
 
   !> SCATTER_NOADD(utemp_pp, u_pp): assignment, the last element wins.
-  SUBROUTINE scatter_noadd(utemp_pp, u_pp)
+  SUBROUTINE scatter_noadd_dev(utemp_pp, u_pp)
     TYPE(pf_dvec), INTENT(IN) :: utemp_pp
     TYPE(pf_dvec), INTENT(INOUT) :: u_pp
     CALL pf_check(c_scatter_noadd(ctx, utemp_pp%ptr, u_pp%ptr), 'SCATTER_NOADD')
-  END SUBROUTINE scatter_noadd
+  END SUBROUTINE scatter_noadd_dev
 
-  FUNCTION dot_product_p(vectora, vectorb) RESULT(s)
+  FUNCTION dot_product_p_dev(vectora, vectorb) RESULT(s)
     TYPE(pf_dvec), INTENT(IN) :: vectora, vectorb
     REAL(iwp) :: s
     REAL(c_double) :: out
     CALL pf_check(c_dot(ctx, vectora%ptr, vectorb%ptr, vectora%n, out), 'DOT_PRODUCT_P')
     s = out
-  END FUNCTION dot_product_p
+  END FUNCTION dot_product_p_dev
 
-  FUNCTION sum_p(vectora) RESULT(s)
+  FUNCTION sum_p_dev(vectora) RESULT(s)
     TYPE(pf_dvec), INTENT(IN) :: vectora
     REAL(iwp) :: s
     REAL(c_double) :: out
     CALL pf_check(c_sum(ctx, vectora%ptr, vectora%n, out), 'SUM_P')
     s = out
-  END FUNCTION sum_p
+  END FUNCTION sum_p_dev
 
   !> checon_par(loads, tol, converged, oldlds): oldlds = loads on return.
-  SUBROUTINE checon_par(loads, tol, converged, oldlds)
+  SUBROUTINE checon_par_dev(loads, tol, converged, oldlds)
     TYPE(pf_dvec), INTENT(IN) :: loads
     REAL(iwp), INTENT(IN) :: tol
     LOGICAL, INTENT(OUT) :: converged
@@ -490,7 +531,156 @@ CONTAINS
     INTEGER(c_int) :: conv
     CALL pf_check(c_checon(ctx, loads%ptr, tol, conv, oldlds%ptr, loads%n), 'checon_par')
     converged = conv /= 0
-  END SUBROUTINE checon_par
+  END SUBROUTINE checon_par_dev
+
+  !------------------------------------------------ host-array forms
+  ! The reference's signatures on REAL(iwp) arrays: GATHER(p_pp,pmul_pp)
+  ! (gather_scatter.f90:580-581), SCATTER(u_pp,utemp_pp) (:726-727),
+  ! SCATTER_NOADD(utemp_pp,u_pp) (:856,913), DOT_PRODUCT_P(vectora,vectorb)
+  ! (maths.f90:206-208), SUM_P(vectora) (:271-315) and
+  ! checon_par(loads,tol,converged,oldlds) (:1047-1050).  Each copies its
+  ! inputs to device staging buffers, runs the device form and copies the
+  ! results back: the same bits as the device form on the same values.
+
+  !> Device staging buffer holding at least n REAL(iwp).
+  SUBROUTINE stage(buf, n)
+    TYPE(pf_dvec), INTENT(INOUT) :: buf
+    INTEGER(c_int64_t), INTENT(IN) :: n
+    IF (buf%n >= n .AND. c_associated(buf%ptr)) RETURN
+    CALL pf_release(buf)
+    buf%n = MAX(n, 1_c_int64_t)
+    CALL pf_check(c_dev_alloc(ctx, INT(buf%n, c_size_t) * 8_c_size_t, buf%ptr), 'pf_dev_alloc (staging)')
+  END SUBROUTINE stage
+
+  SUBROUTINE put_n(buf, a, n)
+    TYPE(pf_dvec), INTENT(INOUT) :: buf
+    REAL(iwp), INTENT(IN), TARGET, CONTIGUOUS :: a(:)
+    INTEGER(c_int64_t), INTENT(IN) :: n
+    CALL stage(buf, n)
+    IF (n > 0) CALL pf_check(c_h2d(ctx, buf%ptr, c_loc(a), INT(n, c_size_t) * 8_c_size_t), 'pf_memcpy_h2d')
+  END SUBROUTINE put_n
+
+  SUBROUTINE get_n(buf, a, n)
+    TYPE(pf_dvec), INTENT(IN) :: buf
+    REAL(iwp), INTENT(INOUT), TARGET, CONTIGUOUS :: a(:)
+    INTEGER(c_int64_t), INTENT(IN) :: n
+    IF (n > 0) CALL pf_check(c_d2h(ctx, c_loc(a), buf%ptr, INT(n, c_size_t) * 8_c_size_t), 'pf_memcpy_d2h')
+  END SUBROUTINE get_n
+
+  !> The sizes the steering fixes: neq_pp for vectors, ntot*nels_pp for
+  !> element arrays (the reference's arrays have these shapes at one rank).
+  SUBROUTINE need_sizes(nvec, nelem, what)
+    INTEGER(c_int64_t), INTENT(IN) :: nvec, nelem
+    CHARACTER(*), INTENT(IN) :: what
+    IF (m_neq == 0 .OR. nvec /= m_neq .OR. (nelem >= 0 .AND. nelem /= m_nslot)) THEN
+      WRITE(*, '(A,A,A,I0,A,I0,A)') 'pf_gpu: ', what, ': arrays must hold neq_pp = ', m_neq, &
+        ' and ntot*nels_pp = ', m_nslot, ' entries (pf_steer sets them)'
+      STOP 1
+    END IF
+  END SUBROUTINE need_sizes
+
+  SUBROUTINE gather_host(p_pp, pmul_pp)
+    REAL(iwp), INTENT(IN), CONTIGUOUS :: p_pp(:)
+    REAL(iwp), INTENT(INOUT), CONTIGUOUS, TARGET :: pmul_pp(:, :)
+    REAL(iwp), POINTER, CONTIGUOUS :: flat(:)
+    CALL need_sizes(INT(SIZE(p_pp), c_int64_t), INT(SIZE(pmul_pp), c_int64_t), 'GATHER')
+    CALL put_n(st_a, p_pp, m_neq)
+    CALL stage(st_b, m_nslot)
+    CALL pf_check(c_gather(ctx, st_a%ptr, st_b%ptr), 'GATHER')
+    flat(1:SIZE(pmul_pp)) => pmul_pp
+    CALL get_n(st_b, flat, m_nslot)
+  END SUBROUTINE gather_host
+
+  SUBROUTINE scatter_host(u_pp, utemp_pp)
+    REAL(iwp), INTENT(INOUT), CONTIGUOUS :: u_pp(:)
+    REAL(iwp), INTENT(IN), CONTIGUOUS, TARGET :: utemp_pp(:, :)
+    REAL(iwp), POINTER, CONTIGUOUS :: flat(:)
+    CALL need_sizes(INT(SIZE(u_pp), c_int64_t), INT(SIZE(utemp_pp), c_int64_t), 'SCATTER')
+    flat(1:SIZE(utemp_pp)) => utemp_pp
+    CALL put_n(st_b, flat, m_nslot)
+    CALL put_n(st_a, u_pp, m_neq)
+    CALL pf_check(c_scatter(ctx, st_a%ptr, st_b%ptr), 'SCATTER')
+    CALL get_n(st_a, u_pp, m_neq)
+  END SUBROUTINE scatter_host
+
+  SUBROUTINE scatter_noadd_host(utemp_pp, u_pp)
+    REAL(iwp), INTENT(IN), CONTIGUOUS, TARGET :: utemp_pp(:, :)
+    REAL(iwp), INTENT(INOUT), CONTIGUOUS :: u_pp(:)
+    REAL(iwp), POINTER, CONTIGUOUS :: flat(:)
+    CALL need_sizes(INT(SIZE(u_pp), c_int64_t), INT(SIZE(utemp_pp), c_int64_t), 'SCATTER_NOADD')
+    flat(1:SIZE(utemp_pp)) => utemp_pp
+    CALL put_n(st_b, flat, m_nslot)
+    CALL put_n(st_a, u_pp, m_neq)      ! equations no element touches keep their value
+    CALL pf_check(c_scatter_noadd(ctx, st_b%ptr, st_a%ptr), 'SCATTER_NOADD')
+    CALL get_n(st_a, u_pp, m_neq)
+  END SUBROUTINE scatter_noadd_host
+
+  FUNCTION dot_product_p_host(vectora, vectorb) RESULT(s)
+    REAL(iwp), INTENT(IN), CONTIGUOUS :: vectora(:), vectorb(:)
+    REAL(iwp) :: s
+    REAL(c_double) :: out
+    INTEGER(c_int64_t) :: n
+    n = INT(SIZE(vectora), c_int64_t)
+    IF (INT(SIZE(vectorb), c_int64_t) /= n) CALL pf_check(1_c_int, 'DOT_PRODUCT_P sizes')
+    CALL put_n(st_a, vectora, n)
+    CALL put_n(st_b, vectorb, n)
+    CALL pf_check(c_dot(ctx, st_a%ptr, st_b%ptr, n, out), 'DOT_PRODUCT_P')
+    s = out
+  END FUNCTION dot_product_p_host
+
+  FUNCTION sum_p_host(vectora) RESULT(s)
+    REAL(iwp), INTENT(IN), CONTIGUOUS :: vectora(:)
+    REAL(iwp) :: s
+    REAL(c_double) :: out
+    INTEGER(c_int64_t) :: n
+    n = INT(SIZE(vectora), c_int64_t)
+    CALL put_n(st_a, vectora, n)
+    CALL pf_check(c_sum(ctx, st_a%ptr, n, out), 'SUM_P')
+    s = out
+  END FUNCTION sum_p_host
+
+  SUBROUTINE checon_par_host(loads, tol, converged, oldlds)
+    REAL(iwp), INTENT(IN), CONTIGUOUS :: loads(:)
+    REAL(iwp), INTENT(IN) :: tol
+    LOGICAL, INTENT(OUT) :: converged
+    REAL(iwp), INTENT(INOUT), CONTIGUOUS :: oldlds(:)
+    INTEGER(c_int) :: conv
+    INTEGER(c_int64_t) :: n
+    n = INT(SIZE(loads), c_int64_t)
+    IF (INT(SIZE(oldlds), c_int64_t) /= n) CALL pf_check(1_c_int, 'checon_par sizes')
+    CALL put_n(st_a, loads, n)
+    CALL put_n(st_b, oldlds, n)
+    CALL pf_check(c_checon(ctx, st_a%ptr, tol, conv, st_b%ptr, n), 'checon_par')
+    converged = conv /= 0
+    oldlds = loads
+  END SUBROUTINE checon_par_host
+
+  !> Summation order of DOT_PRODUCT_P, SUM_P and the PCG's dots:
+  !> PF_DOT_TREE (the device's fixed tree, default) or PF_DOT_SEQ (the
+  !> reference's sequential DOT_PRODUCT order at one rank).
+  SUBROUTINE pf_dot_order(order)
+    INTEGER, INTENT(IN) :: order
+    CALL pf_check(c_dot_order_set(ctx, INT(order, c_int)), 'pf_dot_order_set')
+  END SUBROUTINE pf_dot_order
+
+  !> One element product u = km p as the library forms it: a chain of fused
+  !> multiply-adds in ascending column order, s = fma(km(i,j), p(j), s) from
+  !> s = 0 (MATMUL fixes no order; this is the device kernel's), so that a
+  !> driver's host element loop returns the device's bits.
+  FUNCTION pf_matvec(km, p) RESULT(u)
+    USE, INTRINSIC :: IEEE_ARITHMETIC, ONLY: IEEE_FMA
+    REAL(iwp), INTENT(IN) :: km(:, :), p(:)
+    REAL(iwp) :: u(SIZE(km, 1))
+    INTEGER :: i, j
+    REAL(iwp) :: s
+    DO i = 1, SIZE(km, 1)
+      s = 0.0_iwp
+      DO j = 1, SIZE(km, 2)
+        s = IEEE_FMA(km(i, j), p(j), s)
+      END DO
+      u(i) = s
+    END DO
+  END FUNCTION pf_matvec
 
   SUBROUTINE vec_fill(x, val)
     TYPE(pf_dvec), INTENT(INOUT) :: x

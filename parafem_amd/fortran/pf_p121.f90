@@ -3,7 +3,7 @@
 ! MI355X through the pf_gpu module -- a 3-D elastic solid of 20-node bricks
 ! solved by Jacobi-preconditioned EBE-PCG.
 !
-!   pf_p121 <deck> [dropin|fused|mixed|km] [relres-tol]
+!   pf_p121 <deck> [dropin|host|fused|hostseq|fusedseq|mixed|km] [relres-tol]
 !
 ! Reads <deck>.dat/.d/.bnd/.lds in the formats READ_P121, READ_G_NUM_PP,
 ! READ_G_COORD_PP, READ_REST and READ_LOADS read (src/modules/mpi/
@@ -17,7 +17,15 @@
 !   dropin : p121's own loop, statement for statement (p121.f90:86-104), with
 !            GATHER / element products / SCATTER / DOT_PRODUCT_P / checon_par
 !            and the whole-array statements on device vectors.
+!   host   : p121.f90:89-104 as the reference writes it, on host arrays
+!            p_pp(neq), pmul_pp(ntot,nels), ... : the module's GATHER,
+!            SCATTER, DOT_PRODUCT_P and checon_par generics on REAL(iwp)
+!            arrays, the whole-array statements in Fortran, and the element
+!            loop with pf_matvec (the device's order of the products) for
+!            MATMUL.
 !   fused  : the same iteration as one library call (pf_pcg, FP64, checon).
+!   hostseq, fusedseq : host and fused with DOT_PRODUCT_P and SUM_P in the
+!            reference's sequential order at one rank (PF_DOT_SEQ).
 !   mixed  : FP32-stored element matrices with FP64 vectors and reliable
 !            updates, stopped on the relative residual (default 1e-10).
 !   km     : PCG_KM (maths.f90:1152-1323) with element 1's matrix for all
@@ -40,12 +48,15 @@ PROGRAM pf_p121
   INTEGER(8) :: ta, tst
   INTEGER, ALLOCATABLE :: row(:)
   TYPE(pf_dvec) :: p_pp, r_pp, x_pp, xnew_pp, u_pp, d_pp, diag_precon_pp, pmul_pp, utemp_pp
+  ! host mode: the reference's own arrays (p121.f90:20-25)
+  REAL(iwp), ALLOCATABLE :: ph_pp(:), rh_pp(:), xh_pp(:), xnewh_pp(:), uh_pp(:), dh_pp(:), dph_pp(:)
+  REAL(iwp), ALLOCATABLE :: pmulh_pp(:, :), utemph_pp(:, :), storkm_pp(:, :, :)
   INTEGER(8) :: t0, t1, rate
 
   CALL SYSTEM_CLOCK(tst, rate)
   CALL GET_COMMAND_ARGUMENT(1, argv, nlen)
   IF (nlen == 0) THEN
-    WRITE(*, '(A)') 'usage: pf_p121 <deck> [dropin|fused|mixed|km] [relres-tol]'
+    WRITE(*, '(A)') 'usage: pf_p121 <deck> [dropin|host|fused|hostseq|fusedseq|mixed|km] [relres-tol]'
     STOP 2
   END IF
   mode = 'dropin'
@@ -55,7 +66,8 @@ PROGRAM pf_p121
     CALL GET_COMMAND_ARGUMENT(3, word)
     READ(word, *) rtol
   END IF
-  IF (mode /= 'dropin' .AND. mode /= 'fused' .AND. mode /= 'mixed' .AND. mode /= 'km') THEN
+  IF (mode /= 'dropin' .AND. mode /= 'fused' .AND. mode /= 'mixed' .AND. mode /= 'km' .AND. &
+      mode /= 'host' .AND. mode /= 'hostseq' .AND. mode /= 'fusedseq') THEN
     WRITE(*, '(A,A)') 'pf_p121: unknown mode ', TRIM(mode)
     STOP 2
   END IF
@@ -176,8 +188,32 @@ PROGRAM pf_p121
   q = SUM(r)
   WRITE(11, '(A,E12.4)') "The total load is:", q
 
+  IF (mode == 'hostseq' .OR. mode == 'fusedseq') CALL pf_dot_order(PF_DOT_SEQ)
   CALL SYSTEM_CLOCK(t0, rate)
-  IF (mode == 'dropin') THEN
+  IF (mode == 'host' .OR. mode == 'hostseq') THEN
+    !---------------------------- p121.f90:86-104 on host arrays, as written
+    ALLOCATE(ph_pp(neq), rh_pp(neq), xh_pp(neq), xnewh_pp(neq), uh_pp(neq), dh_pp(neq), dph_pp(neq))
+    ALLOCATE(pmulh_pp(ntot, nels), utemph_pp(ntot, nels), storkm_pp(ntot, ntot, nels))
+    CALL pf_storkm_get(storkm_pp)
+    rh_pp = r; dph_pp = diag; xnewh_pp = 0.0_iwp
+    dph_pp = 1._iwp / dph_pp
+    dh_pp = dph_pp * rh_pp; ph_pp = dh_pp; xh_pp = 0.0_iwp
+    iters = 0
+    iterations: DO
+      iters = iters + 1; uh_pp = 0.0_iwp; pmulh_pp = 0.0_iwp; utemph_pp = 0.0_iwp
+      CALL gather(ph_pp, pmulh_pp)
+      elements_3: DO iel = 1, nels
+        utemph_pp(:, iel) = pf_matvec(storkm_pp(:, :, iel), pmulh_pp(:, iel))
+      END DO elements_3; CALL scatter(uh_pp, utemph_pp)
+      up = DOT_PRODUCT_P(rh_pp, dh_pp); alpha = up / DOT_PRODUCT_P(ph_pp, uh_pp)
+      xnewh_pp = xh_pp + ph_pp * alpha; rh_pp = rh_pp - uh_pp * alpha
+      dh_pp = dph_pp * rh_pp; beta = DOT_PRODUCT_P(rh_pp, dh_pp) / up
+      ph_pp = dh_pp + ph_pp * beta; CALL checon_par(xnewh_pp, tol, converged, xh_pp)
+      IF (converged .OR. iters == limit) EXIT
+    END DO iterations
+    x = xnewh_pp
+    DEALLOCATE(ph_pp, rh_pp, xh_pp, xnewh_pp, uh_pp, dh_pp, dph_pp, pmulh_pp, utemph_pp, storkm_pp)
+  ELSE IF (mode == 'dropin') THEN
     !------------------------------------------- p121.f90:86-104 verbatim flow
     CALL pf_alloc(p_pp, neq); CALL pf_alloc(r_pp, neq); CALL pf_alloc(x_pp, neq)
     CALL pf_alloc(xnew_pp, neq); CALL pf_alloc(u_pp, neq); CALL pf_alloc(d_pp, neq)
@@ -208,7 +244,7 @@ PROGRAM pf_p121
       IF (converged .OR. iters == limit) EXIT
     END DO
     CALL pf_get(xnew_pp, x)
-  ELSE IF (mode == 'fused') THEN
+  ELSE IF (mode == 'fused' .OR. mode == 'fusedseq') THEN
     CALL pf_pcg(r, x, limit, tol, PF_STOP_CHECON, PF_PREC_FP64, iters, converged)
   ELSE IF (mode == 'km') THEN
     ! PCG_KM with element 1's matrix for every element (a uniform mesh and

@@ -6,16 +6,19 @@ pressure.  The loop admits arrived requests in arrival order while a context
 ("slot") and the HBM budget allow (counting what every context holds,
 cached blocks included, and handing cached blocks back when they are in the
 way), builds the problem on that slot (mesh upload, steering and maps built
-on the device, element matrices assembled there), and then drives every
-admitted solve from one host thread: each slot always has one chunk of
-PCG iterations (a replayed HIP graph) queued behind a non-blocking state
-snapshot, so the GPU runs the solves of all slots concurrently on their own
-streams while the host only polls.  A finished solve's displacements are read
-back, its device buffers freed, and the slot takes the next request.
+on the device, element matrices assembled there) and begins its PCG.  All
+admitted solves then run as one batch (pf_batch): every launch of the
+iteration's kernels -- element products, scatter, folds, updates --
+advances every slot, each slot's blocks doing what its own launch would.
+The host drives the batch from one thread in chunks of iterations (a
+replayed HIP graph), keeping one chunk queued behind a non-blocking state
+snapshot; when the snapshot shows a solve finished, its displacements are
+read back, its device buffers freed, the slot takes the next request and
+the next chunk runs the new set.
 
 The arithmetic of a solve does not depend on what else runs: every kernel is
-deterministic and a context's state is its own, so a result is bit-identical
-alone and in the mix.
+deterministic, a solve's blocks do the work of its own launch and its state
+is its own, so a result is bit-identical alone and in the mix.
 
 Stream generator (``make_stream``): numpy ``default_rng(seed)``; arrival gaps
 exponential with the given rate (the first request arrives at 0); n =
@@ -32,7 +35,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import mesh as M
-from .api import Context
+from .api import BATCH_MAX, Batch, Context
 
 
 @dataclass
@@ -98,14 +101,17 @@ class CubeCache:
 
 
 def device_bytes(n: int, precision: str) -> int:
-    """Peak device bytes of one request (assembly writes FP64 matrices before
-    the FP32 split)."""
+    """Peak device bytes of one request: coordinates (480 B per element),
+    steering (240), transposed steering (240 + 4 per equation), FP64
+    element products (480) and the PCG vectors (12 per equation), plus the
+    element matrices as stored -- FP64 (28,800 B per element) or the packed
+    symmetric FP32 hi and lo (2 x 7,328) with FP32 products (240); FP32
+    storage is assembled straight into hi/lo, with no FP64 copy."""
     nels = n ** 3
     neq = 3 * ((2 * n + 1) * (n + 1) * (n + 1) + (n + 1) * (n + 1) * n)   # upper bound
-    per_el = 28800 + 480 + 240 + 240 + 480
-    if precision != "fp64":
-        per_el += 2 * 7328 + 240         # packed symmetric hi and lo, FP32 products
-    return nels * per_el + neq * 8 * 12 + (16 << 20)
+    per_el = 480 + 240 + 240 + 480
+    per_el += 28800 if precision == "fp64" else 2 * 7328 + 240
+    return nels * per_el + neq * 8 * 12 + neq * 4 + (16 << 20)
 
 
 def solve_alone(ctx: Context, cache: CubeCache, req: Request, precision: str, tol: float,
@@ -127,15 +133,17 @@ class _Slot:
     req: Request | None = None
     t_admit: float = 0.0
     bytes: int = 0
-    chunks: int = 0
 
 
 @dataclass
 class Server:
-    slots: int = 8
+    """Continuous batching: every admitted solve is a slot of one pf_batch,
+    so each launch of the iteration's kernels advances all of them; solves
+    join and leave the batch between chunks of iterations."""
+    slots: int = 32
     hbm_budget: float = 240e9
-    chunk: int = 32
-    precision: str = "split"
+    chunk: int = 16
+    precision: str = "mixed64"
     tol: float = 1e-10
     limit: int = 100000
     keep_x: bool = False
@@ -143,9 +151,13 @@ class Server:
     cache: CubeCache = field(default_factory=CubeCache)
 
     def __post_init__(self):
+        if not 1 <= self.slots <= BATCH_MAX:
+            raise ValueError(f"1..{BATCH_MAX} slots")
         self._slots = [_Slot(Context(self.device)) for _ in range(self.slots)]
+        self._batch = Batch(self.device)
 
     def close(self):
+        self._batch.close()
         for s in self._slots:
             s.ctx.close()
 
@@ -176,62 +188,74 @@ class Server:
         ctx.steer_build(c["g_num"], c["rest"], c["nn"])      # steering and maps built on the device
         ctx.assemble_elastic(req.e, req.v, None, store="fp64" if self.precision == "fp64" else "fp32")
         ctx.pcg_begin(c["r_unit"] * req.pressure, self.limit, self.tol, stop="relres", precision=self.precision)
-        s.req, s.t_admit, s.bytes, s.chunks = req, now, device_bytes(req.n, self.precision), 0
-        ctx.pcg_run_graph(self.chunk)
-        ctx.pcg_query()                    # snapshot after the first chunk
-        ctx.pcg_run_graph(self.chunk)      # and one chunk queued behind it
-        s.chunks = 2
+        s.req, s.t_admit, s.bytes = req, now, device_bytes(req.n, self.precision)
 
     def serve(self, requests: list[Request], log=None) -> list[Result]:
-        """Run the stream in real time (arrival times relative to the start)."""
+        """Run the stream in real time (arrival times relative to the start).
+
+        Each cycle enqueues a chunk of iterations for the current slots,
+        takes a state snapshot behind it and enqueues one more chunk, so the
+        GPU keeps working while the host waits for the snapshot; when it
+        arrives, finished solves are read back and retired, waiting requests
+        admitted, and the next cycle runs the new set."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         queue: deque[Request] = deque()
         free = list(self._slots)
         active: list[_Slot] = []
         out: list[Result] = []
+        by_ctx = {s.ctx.h: s for s in self._slots}
+        done: list[tuple] = []           # (slot, result) finished, still to read back
+        in_flight = False
         t0 = time.perf_counter()
-        while pending or queue or active:
+        while pending or queue or active or done:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
-            while queue and free:
-                need = device_bytes(queue[0].n, self.precision)
-                if not self._fits(free[-1], need, bool(active)):
-                    break
-                s = free.pop()
-                req = queue.popleft()
-                self._admit(s, req, time.perf_counter() - t0)
-                active.append(s)
-            progressed = False
-            for s in list(active):
-                res = s.ctx.pcg_query()
-                if res is None:
-                    continue
-                progressed = True
-                # converged, broken down (status 6) or at the limit: the solve
-                # is frozen on the device; anything else is still running
-                if res.converged or res.status == 6 or res.iters >= self.limit:
-                    x = s.ctx.pcg_x()
-                    t = time.perf_counter() - t0
-                    r = s.req
-                    out.append(Result(r.rid, r.n, s.ctx.neq, res.iters, bool(res.converged), res.refinements,
-                                      s.t_admit, t, t - r.arrival, x if self.keep_x else None,
-                                      float(np.abs(x).sum())))
-                    if log:
-                        log(f"request {r.rid} n={r.n} iters={res.iters} latency={t - r.arrival:.3f}s")
-                    s.ctx.reset()
-                    s.req = None
-                    active.remove(s)
-                    free.append(s)
-                else:
-                    s.ctx.pcg_query()                  # snapshot after the queued chunk
-                    s.ctx.pcg_run_graph(self.chunk)    # keep one chunk behind it
-                    s.chunks += 1
-            if not progressed and not (queue and free):
+            if not in_flight:
+                while queue and free:
+                    need = device_bytes(queue[0].n, self.precision)
+                    if not self._fits(free[-1], need, bool(active)):
+                        break
+                    s = free.pop()
+                    self._admit(s, queue.popleft(), time.perf_counter() - t0)
+                    active.append(s)
+                if active:
+                    ctxs = [s.ctx for s in active]
+                    self._batch.run(ctxs, self.chunk)
+                    self._batch.query()              # snapshot behind this chunk
+                    self._batch.run(ctxs, self.chunk)   # and one more chunk queued
+                    in_flight = True
+            # read back the solves that finished in the previous snapshot
+            # while the GPU works on the chunks just queued
+            for s, r in done:
+                x = s.ctx.pcg_x()
+                t = time.perf_counter() - t0
+                q = s.req
+                out.append(Result(q.rid, q.n, s.ctx.neq, r.iters, bool(r.converged), r.refinements,
+                                  s.t_admit, t, t - q.arrival, x if self.keep_x else None,
+                                  float(np.abs(x).sum())))
+                if log:
+                    log(f"request {q.rid} n={q.n} iters={r.iters} latency={t - q.arrival:.3f}s")
+                s.ctx.reset()
+                s.req = None
+                free.append(s)
+            done = []
+            got = self._batch.query() if in_flight else None
+            if got is None:
                 if not active and pending:
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
-                else:
+                elif in_flight:
                     time.sleep(20e-6)
+                continue
+            in_flight = False
+            snap, res = got
+            for ctx, r in zip(snap, res):
+                s = by_ctx[ctx.h]
+                # converged, broken down (status 6) or at the limit: the solve
+                # is frozen on the device; anything else is still running
+                if r.converged or r.status == 6 or r.iters >= self.limit:
+                    active.remove(s)
+                    done.append((s, r))
         return sorted(out, key=lambda r: r.rid)
 
 

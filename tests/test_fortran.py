@@ -97,6 +97,41 @@ def test_fortran_dropin_equals_fused_xx3_tiny(tmp_path, golden):
 
 @needs_exe
 @pytest.mark.gpu
+def test_fortran_host_arrays_equal_fused_xx3_tiny(tmp_path, golden):
+    """p121.f90:89-104 as the reference writes it, on host arrays, against
+    the module's GATHER / SCATTER / DOT_PRODUCT_P / checon_par generics on
+    REAL(iwp) arrays (host mode): the fused device solve's bits and
+    iteration count.  With the reference's sequential dot order (hostseq,
+    fusedseq): bit-identical to each other and to the oracle in that
+    order."""
+    import oracle as O
+    src = os.path.join(golden, "xx3_tiny", "xx3-tiny")
+    for ext in ("dat", "d", "bnd", "lds"):
+        shutil.copy(f"{src}.{ext}", tmp_path)
+    deck = str(tmp_path / "xx3-tiny")
+    say("pf_p121 host xx3-tiny")
+    xh, ih, _ = run(deck, "host")
+    say("pf_p121 fused xx3-tiny")
+    xf, if_, _ = run(deck, "fused")
+    assert ih == if_ == 79
+    assert np.array_equal(xh, xf)
+    say("pf_p121 hostseq / fusedseq xx3-tiny")
+    xhs, ihs, _ = run(deck, "hostseq")
+    xfs, ifs, _ = run(deck, "fusedseq")
+    assert ihs == ifs and np.array_equal(xhs, xfs)
+    d = M.read_deck(src)
+    nf, neq = M.steering(d["nn"], 3, d["rest"])
+    g_g = M.g_g_from(nf, d["g_num"])
+    gc = M.element_coords(d["g_num"], d["coords"])
+    r = M.nodal_loads_to_rhs(nf, d["load_node"], d["load_val"], neq)
+    Ko = O.assemble_elastic(gc, d["e"], d["v"])
+    xo, it, conv = O.pcg(g_g, Ko, r, O.diag(g_g, Ko, neq), d["tol"], d["limit"], dot_order=O.DOT_SEQ)
+    assert conv and ihs == it == 79
+    assert np.array_equal(xhs, xo)
+
+
+@needs_exe
+@pytest.mark.gpu
 def test_fortran_generated_abaqus_deck_and_mixed(tmp_path):
     """A 6^3 hex20 cube deck in Abaqus node order (ABAQUS2SG applied on
     read): dropin and fused bit-identical; the mixed-precision solve within

@@ -66,6 +66,41 @@ def test_assembly_laplace_hex8_matches_oracle(ctx):
     assert np.array_equal(Kg, Ko)
 
 
+@pytest.mark.parametrize("case", ["hex20", "hex8_elastic", "hex8_laplace"])
+def test_fp32_assembly_single_pass(ctx, case):
+    """FP32 storage is assembled straight into the packed hi/lo layout (no
+    FP64 matrices are allocated): the stored matrices equal hi = fl32(K),
+    lo = fl32(K - hi) of the FP64 assembly's upper triangle bit for bit, and
+    the assembled diagonal equals the FP64 one."""
+    if case == "hex20":
+        pr = _distorted(3)
+    elif case == "hex8_elastic":
+        m = M.box_hex8(3, 3, 3, 1.0, 1.0, 1.0)
+        rest = M.base_fixed_rest(m)
+        nf, neq = M.steering(m.nn, 3, rest)
+        pr = P.Problem("elastic", m, M.g_g_from(nf, m.g_num), neq, np.zeros(neq), nf,
+                       e_elem=M.lognormal(1, m.nels, np.log(100.0), 0.5))
+    else:
+        pr = P.laplace_cube(4, lognormal_k=True)
+    say(f"single-pass FP32 assembly {case} nels={pr.nels}")
+    P.setup(ctx, pr, store="fp64")
+    K = ctx.storkm_get()
+    d64 = ctx.diag_get()
+    P.setup(ctx, pr, store="fp32")
+    K32 = ctx.storkm_get()
+    d32 = ctx.diag_get()
+    # expected: the upper triangle's hi + lo, mirrored (storkm_pp blocks are
+    # column-major: K[e][j][i] is entry (i, j))
+    up = np.triu(np.ones((pr.ntot, pr.ntot), dtype=bool))
+    A = np.transpose(K, (0, 2, 1))                 # A[e][i][j] = entry (i, j)
+    U = np.where(up, A, np.transpose(A, (0, 2, 1)))
+    hi = U.astype(np.float32)
+    lo = (U - hi.astype(np.float64)).astype(np.float32)
+    E = hi.astype(np.float64) + lo.astype(np.float64)
+    assert np.array_equal(np.transpose(K32, (0, 2, 1)), E)
+    assert np.array_equal(d32, d64)
+
+
 def test_diag_and_primitives_bitexact(ctx):
     pr = _distorted(3, seed=2)
     say(f"primitives nels={pr.nels} neq={pr.neq}")
@@ -496,9 +531,10 @@ def test_graph_replay_matches_eager(ctx):
 
 
 def test_serving_mix_equals_alone():
-    """Continuous batching: requests solved concurrently on several slots
-    give bit-identical displacements and iteration counts to the same
-    requests solved alone; the FP64 one-by-one path agrees within 1e-8."""
+    """Continuous batching: requests solved as slots of one batch (joining
+    and leaving it as slots free up) give bit-identical displacements and
+    iteration counts to the same requests solved alone; the FP64 one-by-one
+    path agrees within 1e-8."""
     from parafem_amd import serve as S
     reqs = S.make_stream(10, rate=0.0, seed=4, nmin=3, nmax=9)
     say(f"serving mix sizes={[r.n for r in reqs]}")
@@ -513,6 +549,32 @@ def test_serving_mix_equals_alone():
     for a, b, c in zip(mix, seq, f64):
         assert a.iters == b.iters and np.array_equal(a.x, b.x), a.rid
         assert np.abs(a.x - c.x).max() <= 1e-8 * np.abs(c.x).max()
+
+
+def test_serving_fp64_matches_oracle():
+    """The serving loop on the FP64 path: every served request (its own n,
+    E, nu and pressure) equals the oracle's p121 PCG in the device's
+    reduction order bit for bit, with the same iteration count."""
+    from parafem_amd import serve as S
+    reqs = S.make_stream(6, rate=0.0, seed=4, nmin=2, nmax=5)
+    say(f"serving fp64 vs oracle sizes={[r.n for r in reqs]}")
+    cache = S.CubeCache()
+    srv = S.Server(slots=3, chunk=8, keep_x=True, precision="fp64", cache=cache)
+    try:
+        got = srv.serve(reqs)
+    finally:
+        srv.close()
+    for q, g in zip(reqs, got):
+        c = cache.get(q.n)
+        pr = P.elastic_cube(q.n, e=q.e, v=q.v)
+        nf, _ = M.steering(c["nn"], 3, c["rest"])      # the served request's numbering
+        assert np.array_equal(pr.g_g, M.g_g_from(nf, pr.mesh.g_num))
+        Ko = O.assemble_elastic(pr.g_coord(), q.e, q.v)
+        r = c["r_unit"] * q.pressure
+        xo, it, conv = O.pcg(pr.g_g, Ko, r, O.diag(pr.g_g, Ko, pr.neq), 1e-10, 100000, stop=O.STOP_RELRES,
+                             dot_order=O.DOT_DEVICE)
+        assert conv and g.converged and g.iters == it, q.rid
+        assert np.array_equal(g.x, xo), q.rid
 
 
 def _ragged_laplace():
