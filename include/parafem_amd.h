@@ -40,6 +40,8 @@ extern "C" {
 #define PF_NOT_CONVERGED   5  /* PCG hit its iteration limit                  */
 #define PF_ERR_BREAKDOWN   6  /* non-finite or zero curvature p.u / up        */
 #define PF_ERR_NODEVICE    7  /* no HIP device / device index out of range    */
+#define PF_ERR_TOO_LARGE   8  /* the problem exceeds the fixed device scratch
+                                 of an allocation-free entry point            */
 
 /* Storage precision of element matrices. */
 #define PF_STORE_FP64      0  /* storkm_pp as REAL(iwp) (precision.f90:19)    */
@@ -286,6 +288,18 @@ int pf_batch_query(pf_batch *batch, pf_pcg_result *res, int *ready);
 /* y = K x over the assembled operator in FP64 (host vectors, neq), with the
  * p123 fixed-freedom rows applied.  Used to report true residuals. */
 int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);
+/* No counterpart in the reference: pf_apply for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Same operator, same bits as pf_apply, but the padded x, y
+ * and the element products live in a fixed device scratch array of
+ * PF_NOALLOC_SCRATCH_BYTES per GPU (held from library load until the process
+ * ends) instead of buffers allocated per call.  Needs
+ * 8*(ntot*nels) + 2*8*(neq+1) bytes (each part rounded up to 256) within
+ * that size, else PF_ERR_TOO_LARGE.  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (two host threads, or two streams
+ * with work in flight) race. */
+#define PF_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pf_apply_noalloc(pf_ctx *ctx, const double *x_host, double *y_host);
 /* The element products of p124's time loop without the fixed-freedom rows:
  * y = SCATTER(storka_pp . GATHER(x)) (which = 0, the PCG's matrices, in
  * FP64) or storkb_pp (which = 1) (p124.f90:158-162, 171-174, 191-194).

@@ -35,6 +35,7 @@ PF_ERR_STATE = 4
 PF_NOT_CONVERGED = 5
 PF_ERR_BREAKDOWN = 6
 PF_ERR_NODEVICE = 7
+PF_ERR_TOO_LARGE = 8
 
 STORE = {"fp64": 0, "fp32": 1}
 STOP = {"checon": 0, "relres": 1}
@@ -156,6 +157,7 @@ def lib():
         "pf_pcg_timing_get": (i32, [vp, pd, pi]),
         "pf_dev_copy": (i32, [vp, vp, vp, sz]),
         "pf_dot_order_set": (i32, [vp, i32]),
+        "pf_apply_noalloc": (i32, [vp, pd, pd]),
         "pf_batch_create": (i32, [i32, C.POINTER(vp)]),
         "pf_batch_destroy": (i32, [vp]),
         "pf_batch_run": (i32, [vp, i32, C.POINTER(vp), i32]),
@@ -666,6 +668,15 @@ class Context:
         xx = _f64(x)
         y = np.empty(self.neq)
         _check(lib().pf_apply(self.h, _pd(xx), _pd(y)), "pf_apply")
+        return y
+
+    def apply_noalloc(self, x) -> np.ndarray:
+        """apply() through the allocation-free entry point (the library's
+        fixed per-GPU scratch; PFError with PF_ERR_TOO_LARGE beyond it)."""
+        xx = _f64(x)
+        assert xx.size == self.neq
+        y = np.empty(self.neq)
+        _check(lib().pf_apply_noalloc(self.h, _pd(xx), _pd(y)), "pf_apply_noalloc")
         return y
 
 

@@ -248,6 +248,7 @@ const char *pf_status_string(int s)
     case PF_NOT_CONVERGED: return "PCG reached its iteration limit";
     case PF_ERR_BREAKDOWN: return "PCG breakdown (non-finite or zero curvature)";
     case PF_ERR_NODEVICE: return "no HIP device";
+    case PF_ERR_TOO_LARGE: return "problem exceeds the allocation-free scratch (PF_NOALLOC_SCRATCH_BYTES)";
     default: return "unknown status";
     }
 }
@@ -274,7 +275,8 @@ int pf_create(int device, pf_ctx **out)
         return pf_set_hip_error(e, "hipStreamCreate", __FILE__, __LINE__);
     }
     c->stream = c->own_stream;
-    int rc = pf_alloc(c, (void **)&c->state, sizeof(pf_state));
+    int rc = pf_scratch_init(device);   // device address of the allocation-free scratch
+    if (!rc) rc = pf_alloc(c, (void **)&c->state, sizeof(pf_state));
     if (!rc) rc = pf_alloc(c, (void **)&c->partials, sizeof(double) * kPartials);
     if (!rc && hipHostMalloc((void **)&c->state_host, sizeof(pf_state), hipHostMallocDefault) != hipSuccess) rc = PF_ERR_OOM;
     if (!rc && hipHostMalloc((void **)&c->scalar_host, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = PF_ERR_OOM;

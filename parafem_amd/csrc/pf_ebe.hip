@@ -34,6 +34,7 @@
 // reproducible bit for bit.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1185,6 +1186,32 @@ __global__ void kb_snapshot(const pf_bslot *__restrict__ T, int ns, pf_state *__
 
 }  // namespace
 
+// ------------------------------------------------ allocation-free K x
+// pf_apply's buffers (padded x, y, element products) in a file-scope device
+// array instead of allocations.  In host code `scratch` names a host-side
+// stand-in, not device memory: kernels and copies take the per-device
+// address that hipGetSymbolAddress returns (g_scratch_dev).
+static __device__ __align__(256) unsigned char scratch[PF_NOALLOC_SCRATCH_BYTES];
+static constexpr int kMaxDevices = 64;
+static void *g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+int pf_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return PF_ERR_NODEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return PF_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return pf_set_hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)",
+                                __FILE__, __LINE__);
+    }
+    g_scratch_dev[device] = ptr;
+    return PF_OK;
+}
+
 extern "C" {
 
 int pf_dev_copy(pf_ctx *c, void *dst, const void *src, size_t bytes)
@@ -1650,6 +1677,37 @@ int pf_apply(pf_ctx *c, const double *x_host, double *y_host)
     if (!c || !x_host || !y_host) return PF_ERR_ARG;
     if (!c->have_K) return PF_ERR_STATE;
     return apply_host(c, x_host, y_host, nullptr, true);
+}
+
+int pf_apply_noalloc(pf_ctx *c, const double *x_host, double *y_host)
+{
+    if (!c || !x_host || !y_host) return PF_ERR_ARG;
+    if (!c->have_K) return PF_ERR_STATE;
+    auto up256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t nv = (size_t)c->neq + 1, ns = (size_t)c->ntot * c->nels;
+    const size_t bv = up256(nv * sizeof(double)), bs = up256(ns * sizeof(double));
+    if (2 * bv + bs > PF_NOALLOC_SCRATCH_BYTES) return PF_ERR_TOO_LARGE;
+    unsigned char *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c->device >= 0 && c->device < kMaxDevices) base = (unsigned char *)g_scratch_dev[c->device];
+    }
+    if (!base) return PF_ERR_STATE;   // pf_create looks the address up; not reached after a good create
+    double *xe = (double *)base, *y = (double *)(base + bv), *ut = (double *)(base + 2 * bv);
+    PF_HIP(hipSetDevice(c->device));
+    // xe[0] must read 0 (a restrained freedom gathers slot 0); the scratch
+    // keeps the previous call's contents, so it is written on every call
+    static const double zero = 0.0;
+    PF_HIP(hipMemcpyAsync(xe, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PF_HIP(hipMemcpyAsync(xe + 1, x_host, c->neq * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = launch_ebe_exact<true>(c, xe, ut);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scatter_fix, dim3(pf_grid_for(c->neq)), dim3(256), 0, c->stream, c->csr_ptr, c->csr_slot,
+                       ut, (const double *)c->fixstore, xe, y, c->neq);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipMemcpyAsync(y_host, y, c->neq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PF_HIP(hipStreamSynchronize(c->stream));
+    return PF_OK;
 }
 
 int pf_apply_raw(pf_ctx *c, int which, const double *x_host, double *y_host)

@@ -191,6 +191,10 @@ struct pf_ctx {
     size_t ev_next = 0;
 };
 
+// Looks up, once per device (the device current), the address of the
+// allocation-free entry points' scratch array (pf_ebe.hip).
+int pf_scratch_init(int device);
+
 // p124's transient element matrices into K64 (storka) and K64b (storkb).
 int pf_launch_assemble_transient(pf_ctx *c, double theta, double dtim, const double *prop, const int *etype);
 
