@@ -331,6 +331,10 @@ def serve_bench(args):
         res_closed = srv.serve(closed, log=progress)
         cap = S.summarize(res_closed, time.perf_counter() - t0)
         log(f"closed loop: {cap}")
+        if args.closed_only:
+            print(json.dumps({"closed_loop": cap, "precision": args.serve_precision, "slots": args.slots,
+                              "requests": args.requests, "nmin": args.nmin, "nmax": args.nmax}), flush=True)
+            return
         rate = args.rate if args.rate > 0 else 0.8 * cap["solves_per_s"]
         stream = S.make_stream(args.requests, rate, seed=4, nmin=args.nmin, nmax=args.nmax)
         t0 = time.perf_counter()
@@ -390,6 +394,8 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serve: Poisson arrival rate (0 = 80%% of the measured closed-loop capacity)")
     ap.add_argument("--slots", type=int, default=32, help="serve: concurrent solves (one batch)")
+    ap.add_argument("--closed-only", action="store_true",
+                    help="serve: only the closed-loop capacity of the server (no timed stream, no FP64 baseline)")
     ap.add_argument("--serve-sample", type=int, default=48,
                     help="hex20: requests of the reduced serving stream reported beside the iteration rate "
                          "(0 = skip)")
