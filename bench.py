@@ -331,8 +331,10 @@ def serve_bench(args):
         res_closed = srv.serve(closed, log=progress)
         cap = S.summarize(res_closed, time.perf_counter() - t0)
         log(f"closed loop: {cap}")
+        log(f"server host time: {srv.stats}")
         if args.closed_only:
-            print(json.dumps({"closed_loop": cap, "precision": args.serve_precision, "slots": args.slots,
+            print(json.dumps({"closed_loop": cap, "host": srv.stats, "precision": args.serve_precision,
+                              "slots": args.slots,
                               "requests": args.requests, "nmin": args.nmin, "nmax": args.nmax}), flush=True)
             return
         rate = args.rate if args.rate > 0 else 0.8 * cap["solves_per_s"]
@@ -459,22 +461,26 @@ def main():
     log(f"problem built: nels={pr.nels} neq={pr.neq} ({time.perf_counter() - t0:.1f} s)")
     ctx = Context(0)
     paths = {}
-    order = ["fp64", "split", "mixed64", "mixed"] if args.precision == "mixed" else ["mixed", "mixed64", "split", "fp64"]
-    for prec in order:
+    order = ["fp64", "split", "mixed64", "mixed", "mixed_ranges"] if args.precision == "mixed" \
+        else ["mixed_ranges", "mixed", "mixed64", "split", "fp64"]
+    for name in order:
+        # mixed_ranges: the mixed solver with range-fused products and sums
+        prec = "mixed" if name == "mixed_ranges" else name
         store = "fp64" if prec == "fp64" else "fp32"
+        ctx.scatter_mode = "ranges" if name == "mixed_ranges" else "slots"
         t0 = time.perf_counter()
         P.setup(ctx, pr, store=store)
         ctx.sync()
         t_setup = time.perf_counter() - t0
         shape = T.tune_ebe(ctx, pr.r0, prec)
-        log(f"{prec}: autotuned elements per lane group {shape['ebe_epl']} "
+        log(f"{name}: autotuned elements per lane group {shape['ebe_epl']} "
             f"({'cache' if shape['cached'] else 'timed ' + str(shape['ms'])})")
         elapsed, tk, nprof = time_iterations(ctx, pr, prec, args.steps, args.warmup)
         ebe_ms = tk["ebe"][0] / max(tk["ebe"][1], 1)
         per_elem = ebe_bytes(pr.ntot, store, prec)
         achieved = pr.nels * per_elem / (ebe_ms * 1e-3) / 1e9
         iter_ms = sum(v[0] for v in tk.values()) / nprof
-        paths[prec] = {
+        paths[name] = {
             "iter_per_s": round(args.steps / elapsed, 3),
             "ms_per_step": round(elapsed / args.steps * 1e3, 4),
             "elapsed_s": elapsed,
@@ -487,22 +493,27 @@ def main():
             "setup_assembly_s": round(t_setup, 3),
             "device_gib": round(ctx.mem_used() / 2**30, 2),
             "tile_shape": {"ebe_epl": shape["ebe_epl"], "source": "autotuner"},
-            "kernel": ebe_kernel_name(pr.ntot, prec, shape["ebe_epl"]),
+            "kernel": (f"k_ebe_range<{pr.ntot}> (products and their sums into u, one launch per colour)"
+                       if name == "mixed_ranges" else ebe_kernel_name(pr.ntot, prec, shape["ebe_epl"])),
         }
-        log(f"{prec}: {paths[prec]['iter_per_s']} iter/s, element products {ebe_ms:.3f} ms "
+        log(f"{name}: {paths[name]['iter_per_s']} iter/s, element products {ebe_ms:.3f} ms "
             f"({achieved:.0f} GB/s), kernels " +
-            ", ".join(f"{k}={v}" for k, v in paths[prec]["kernel_ms_per_iter"].items()))
+            ", ".join(f"{k}={v}" for k, v in paths[name]["kernel_ms_per_iter"].items()))
+    ctx.scatter_mode = "slots"
 
     solves = None
     if args.solve_tol > 0:
         solves = {}
         x64, solves["fp64"] = solve(ctx, pr, "fp64", "fp64", args.solve_tol)
         log(f"fp64 solve: {solves['fp64']}")
-        for prec in ("mixed", "mixed64", "split"):
-            xm, solves[prec] = solve(ctx, pr, prec, "fp32", args.solve_tol)
-            solves[prec]["rel_err_vs_fp64"] = float(np.abs(xm - x64).max() / np.abs(x64).max())
-            solves[f"speedup_{prec}_vs_fp64"] = round(solves["fp64"]["time_s"] / solves[prec]["time_s"], 3)
-            log(f"{prec} solve: {solves[prec]}")
+        for name in ("mixed", "mixed_ranges", "mixed64", "split"):
+            prec = "mixed" if name == "mixed_ranges" else name
+            ctx.scatter_mode = "ranges" if name == "mixed_ranges" else "slots"
+            xm, solves[name] = solve(ctx, pr, prec, "fp32", args.solve_tol)
+            solves[name]["rel_err_vs_fp64"] = float(np.abs(xm - x64).max() / np.abs(x64).max())
+            solves[f"speedup_{name}_vs_fp64"] = round(solves["fp64"]["time_s"] / solves[name]["time_s"], 3)
+            log(f"{name} solve: {solves[name]}")
+        ctx.scatter_mode = "slots"
 
     serving = None
     if args.serve_sample > 0 and args.workload == "hex20":

@@ -467,6 +467,24 @@ int pf_reduce_grid(int64_t n);
 int pf_dot_order_set(pf_ctx *ctx, int order);
 int pf_dot_order_get(pf_ctx *ctx, int *order);
 
+/* How the mixed solver (PF_PREC_MIXED) sums the element products into u --
+ * SCATTER in p121.f90:89-97 (gather_scatter.f90:694-850):
+ *   PF_SCATTER_SLOTS (default): the products go to utemp and every
+ *     equation adds its element slots in ascending element order (the
+ *     reference's order; every other precision always does this);
+ *   PF_SCATTER_RANGES: one workgroup forms the products of a range of
+ *     consecutive elements and adds them per equation in LDS, then into u
+ *     once; ranges sharing an equation run in different launches (colours).
+ *     No utemp round trip.  Deterministic, but the sums' order is not the
+ *     reference's.  Built at pf_pcg_begin from the steering and fixed rows
+ *     (PF_ERR_ARG if the steering does not allow it: an equation listed
+ *     twice by one element, or more than 32 colours).  The batch
+ *     (pf_batch_*) refuses contexts in this mode. */
+#define PF_SCATTER_SLOTS   0
+#define PF_SCATTER_RANGES  1
+int pf_scatter_mode_set(pf_ctx *ctx, int mode);
+int pf_scatter_mode_get(pf_ctx *ctx, int *mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,7 @@ STORE = {"fp64": 0, "fp32": 1}
 STOP = {"checon": 0, "relres": 1}
 PREC = {"fp64": 0, "mixed": 1, "mixed64": 2, "split": 3}
 DOT_ORDER = {"tree": 0, "seq": 1}
+SCATTER_MODE = {"slots": 0, "ranges": 1}
 
 
 class LibraryNotBuilt(RuntimeError):
@@ -164,6 +165,8 @@ def lib():
         "pf_batch_run_graph": (i32, [vp, i32, C.POINTER(vp), i32]),
         "pf_batch_query": (i32, [vp, vp, pi]),
         "pf_dot_order_get": (i32, [vp, pi]),
+        "pf_scatter_mode_set": (i32, [vp, i32]),
+        "pf_scatter_mode_get": (i32, [vp, pi]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -315,6 +318,19 @@ class Context:
     @dot_order.setter
     def dot_order(self, order: str):
         _check(lib().pf_dot_order_set(self.h, DOT_ORDER[order]), "pf_dot_order_set")
+
+    @property
+    def scatter_mode(self) -> str:
+        """How the mixed solver sums element products into u: "slots"
+        (utemp, then each equation's slots in ascending element order;
+        default) or "ranges" (range-fused products and sums, no utemp)."""
+        o = C.c_int()
+        _check(lib().pf_scatter_mode_get(self.h, C.byref(o)), "pf_scatter_mode_get")
+        return {v: k for k, v in SCATTER_MODE.items()}[o.value]
+
+    @scatter_mode.setter
+    def scatter_mode(self, mode: str):
+        _check(lib().pf_scatter_mode_set(self.h, SCATTER_MODE[mode]), "pf_scatter_mode_set")
 
     def mem_used(self) -> int:
         n = C.c_size_t()

@@ -134,6 +134,7 @@ void pf_graph_drop(pf_ctx *c)
 static void free_problem_vectors(pf_ctx *c)
 {
     pf_graph_drop(c);
+    pf_ranges_drop(c);
     pf_block_free(c);
     size_t nv = (size_t)c->neq + 1;
     free_arr(c, c->x, nv); free_arr(c, c->r, nv); free_arr(c, c->p, nv);
@@ -496,6 +497,7 @@ static int finish_direct32(pf_ctx *c, const double *dg)
     PF_HIP(hipMemcpyAsync(c->diag, c->diag0, nv * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     c->have_K = true;
     c->pcg_ready = false;
+    pf_ranges_drop(c);   // fixed rows are cleared
     free_arr(c, c->fixstore, nv);
     c->nfix = 0;
     PF_HIP(hipStreamSynchronize(c->stream));
@@ -524,6 +526,7 @@ static int finish_matrices(pf_ctx *c)
     }
     c->have_K = true;
     c->pcg_ready = false;
+    pf_ranges_drop(c);   // fixed rows are cleared
     free_arr(c, c->fixstore, nv);
     c->nfix = 0;
     PF_HIP(hipStreamSynchronize(c->stream));
@@ -717,6 +720,7 @@ int pf_fixed_set(pf_ctx *c, int nfix, const int *no_f, double penalty, double *s
     for (int i = 0; i < nfix; ++i)
         if (no_f[i] < 1 || no_f[i] > c->neq) return PF_ERR_ARG;
     pf_graph_drop(c);
+    pf_ranges_drop(c);   // their fixed-row flags change
     PF_HIP(hipSetDevice(c->device));
     size_t nv = (size_t)c->neq + 1;
     free_arr(c, c->fixstore, nv);

@@ -39,126 +39,9 @@
 #include <vector>
 
 #include "pf_internal.h"
+#include "pf_device.h"
 
 namespace {
-
-constexpr int MAXG = PF_REDUCE_MAX_GRID;
-
-enum { KIND_F64 = 0, KIND_F32 = 1, KIND_ACC = 2, KIND_WIDE = 3 };   // K_e storage / arithmetic
-enum { RUN_ALWAYS = 0, RUN_LIVE = 1, RUN_REFINE = 2 };  // state gating
-
-__device__ __forceinline__ bool skip(const pf_state *st, int gate)
-{
-    if (!st || gate == RUN_ALWAYS) return false;
-    if (st->frozen) return true;
-    return gate == RUN_REFINE && !st->refine;
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = __dadd_rn(v, __shfl_down(v, off, 64));
-    return v;
-}
-
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
-    return v;
-}
-
-// Folds NQ per-thread values of this block (sum or max per quantity) and
-// stores the block results in partials[qid*MAXG + bidx].
-template <int NQ>
-__device__ void block_to_partials(double (&v)[NQ], const int (&qid)[NQ], const bool (&ismax)[NQ], double *partials,
-                                  int bidx)
-{
-    __shared__ double ws[NQ][PF_BLOCK / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) v[k] = ismax[k] ? wave_max(v[k]) : wave_sum(v[k]);
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) ws[k][wave] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) {
-            double s = ws[k][0];
-            for (int w = 1; w < PF_BLOCK / 64; ++w) s = ismax[k] ? fmax(s, ws[k][w]) : __dadd_rn(s, ws[k][w]);
-            partials[qid[k] * MAXG + bidx] = s;
-        }
-    }
-}
-
-// In a one-block kernel: folds the G block results of each quantity in the
-// fixed order; out[] is valid in thread 0.
-template <int NQ>
-__device__ void fold_partials(double (&out)[NQ], const int (&qid)[NQ], const bool (&ismax)[NQ],
-                              const double *partials, int G)
-{
-    __shared__ double ws[NQ][PF_BLOCK / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-        // Thread t adds entries t, t+256, ... in that order; the loads of a
-        // batch of 16 are issued before the dependent additions.
-        double s = 0.0;
-        for (int b0 = threadIdx.x; b0 < G; b0 += 16 * PF_BLOCK) {
-            double t[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int b = b0 + u * PF_BLOCK;
-                t[u] = b < G ? partials[qid[k] * MAXG + b] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 16; ++u)
-                if (b0 + u * PF_BLOCK < G) s = ismax[k] ? fmax(s, t[u]) : __dadd_rn(s, t[u]);
-        }
-        s = ismax[k] ? wave_max(s) : wave_sum(s);
-        if (lane == 0) ws[k][wave] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) {
-            double s = ws[k][0];
-            for (int w = 1; w < PF_BLOCK / 64; ++w) s = ismax[k] ? fmax(s, ws[k][w]) : __dadd_rn(s, ws[k][w]);
-            out[k] = s;
-        }
-    }
-}
-
-// Width of the lane group serving one element, elements per wavefront, and
-// elements per lane group (EPL: a group loads EPL element matrices before it
-// multiplies, to keep enough bytes in flight when an element is small).
-template <int NT, int KIND> struct Lanes {
-    static constexpr int W = NT > 32 ? 64 : (NT > 16 ? 32 : (NT > 8 ? 16 : 8));
-    static constexpr int EPW = 64 / W;
-    static constexpr int EPL = NT > 32 ? 1 : NT > 8 ? (KIND == 1 ? 2 : 1) : (KIND == 1 ? 8 : KIND == 0 ? 1 : 4);
-    static constexpr int EPB = EPW * EPL * (PF_BLOCK / 64);   // elements per block
-};
-
-// Workgroups are dealt to the 8 XCDs round-robin by index.  Renumber them so
-// that each XCD works on one contiguous range of logical blocks: neighbouring
-// elements share nodes, and their gathers then hit the same L2.
-__device__ __forceinline__ long xcd_block(int b, int nb)
-{
-    const int per = nb >> 3, rem = nb & 7, x = b & 7, i = b >> 3;
-    return (long)x * per + (x < rem ? x : rem) + i;
-}
-
-typedef float f4_t __attribute__((ext_vector_type(4)));
-typedef double d2_t __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ double bpermute64(int addr, double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)b);
-    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 
 // Element products.  GATHER: src is a neq+1 vector with src[0] = 0 and the
 // element vector is src[g_g]; otherwise src is pmul (ntot,nels).  Lane i of
@@ -1313,6 +1196,24 @@ int pf_dot_order_get(pf_ctx *c, int *order)
     return PF_OK;
 }
 
+int pf_scatter_mode_set(pf_ctx *c, int mode)
+{
+    if (!c || (mode != PF_SCATTER_SLOTS && mode != PF_SCATTER_RANGES)) return PF_ERR_ARG;
+    if (mode != c->scatter_mode) {
+        pf_graph_drop(c);
+        c->pcg_ready = false;   // pf_pcg_begin again (it builds the ranges)
+    }
+    c->scatter_mode = mode;
+    return PF_OK;
+}
+
+int pf_scatter_mode_get(pf_ctx *c, int *mode)
+{
+    if (!c || !mode) return PF_ERR_ARG;
+    *mode = c->scatter_mode;
+    return PF_OK;
+}
+
 int pf_checon_par(pf_ctx *c, const double *xnew, double tol, int *converged, double *x, int64_t n)
 {
     if (!c || !converged || n < 0 || (n && (!xnew || !x))) return PF_ERR_ARG;
@@ -1345,6 +1246,13 @@ static int pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const double
     if (opts->precision == PF_PREC_MIXED && c->ntot <= 24 && !c->p32) {
         pf_graph_drop(c);
         const int rc = pf_alloc(c, (void **)&c->p32, ((size_t)c->neq + 1) * sizeof(float));
+        if (rc) return rc;
+    }
+    const bool ranges = opts->precision == PF_PREC_MIXED && c->scatter_mode == PF_SCATTER_RANGES;
+    if (ranges && c->dot_order != PF_DOT_TREE) return PF_ERR_ARG;   // p.u is summed per range
+    if (ranges && !c->rng) {
+        pf_graph_drop(c);
+        const int rc = pf_ranges_build(c);
         if (rc) return rc;
     }
     c->opts = *opts;
@@ -1397,12 +1305,16 @@ int pf_pcg_run(pf_ctx *c, int n)
     // row-major writes cost the product more than the scatter saves.
     const int rm = c->csr_slot_t && !mixed ? 1 : 0;
     const int *slot = rm ? c->csr_slot_t : c->csr_slot;
+    const bool ranges = mixed && c->scatter_mode == PF_SCATTER_RANGES;
+    if (ranges && !c->rng) return PF_ERR_STATE;   // pf_pcg_begin builds them
     hipEvent_t t0, t1;
     for (int it = 0; it < n; ++it) {
         int rc;
         bool tm = pf_timing_begin(c, PF_K_EBE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
-        if (mixed)
+        if (ranges)
+            rc = pf_ranges_iteration(c, p32);   // products, their sums into u, p.u per range
+        else if (mixed)
             rc = launch_ebe<KIND_F32, true, float>(c, c->p, c->utemp32, c->state, RUN_LIVE, nullptr, p32, rm);
         else if (wide)
             rc = launch_ebe<KIND_WIDE, true, double>(c, c->p, c->utemp, c->state, RUN_LIVE, nullptr, nullptr, rm);
@@ -1414,7 +1326,10 @@ int pf_pcg_run(pf_ctx *c, int n)
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         tm = pf_timing_begin(c, PF_K_SCATTER, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
-        if (mixed)
+        if (ranges) {
+            rc = pf_ranges_fold(c);
+            if (rc) return rc;
+        } else if (mixed)
             hipLaunchKernelGGL(k_scatter_pu<float>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr, slot,
                                c->utemp32, c->fixstore, c->p, c->u, c->neq, c->fix_zero ? 1 : 0, c->partials,
                                c->state);
@@ -1422,9 +1337,11 @@ int pf_pcg_run(pf_ctx *c, int n)
             hipLaunchKernelGGL(k_scatter_pu<double>, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->csr_ptr,
                                slot, c->utemp, c->fixstore, c->p, c->u, c->neq, c->fix_zero ? 1 : 0,
                                c->partials, c->state);
-        const int Ga = seq_reduce(c, G, c->p + 1, c->u + 1, PF_Q_PU, nullptr, nullptr, 0, false, c->neq, c->state,
-                                  RUN_LIVE);
-        hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, Ga, c->state);
+        if (!ranges) {
+            const int Ga = seq_reduce(c, G, c->p + 1, c->u + 1, PF_Q_PU, nullptr, nullptr, 0, false, c->neq,
+                                      c->state, RUN_LIVE);
+            hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, Ga, c->state);
+        }
         if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         tm = pf_timing_begin(c, PF_K_UPDATE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
@@ -1474,7 +1391,7 @@ int pf_pcg_run_graph(pf_ctx *c, int n)
     if (!c->pcg_ready) return PF_ERR_STATE;
     if (c->timing) return pf_pcg_run(c, n);   // timing records an event per launch
     if (!c->graph_exec || c->graph_iters != n || c->graph_precision != c->opts.precision ||
-        c->graph_stop != c->opts.stop_mode || c->graph_dot != c->dot_order) {
+        c->graph_stop != c->opts.stop_mode || c->graph_dot != c->dot_order || c->graph_scatter != c->scatter_mode) {
         pf_graph_drop(c);
         PF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         const int rc = pf_pcg_run(c, n);
@@ -1495,6 +1412,7 @@ int pf_pcg_run_graph(pf_ctx *c, int n)
         c->graph_precision = c->opts.precision;
         c->graph_stop = c->opts.stop_mode;
         c->graph_dot = c->dot_order;
+        c->graph_scatter = c->scatter_mode;
     }
     PF_HIP(hipGraphLaunch(c->graph_exec, c->stream));
     return PF_OK;
@@ -1783,6 +1701,7 @@ static int batch_prepare(pf_batch *b, int nctx, pf_ctx *const *ctxs)
         if (c->dot_order != PF_DOT_TREE || c->ntot != c0->ntot || c->opts.precision != c0->opts.precision ||
             c->opts.stop_mode != c0->opts.stop_mode)
             return PF_ERR_ARG;
+        if (c->opts.precision == PF_PREC_MIXED && c->scatter_mode != PF_SCATTER_SLOTS) return PF_ERR_ARG;
         const bool mixed = c->opts.precision == PF_PREC_MIXED;
         const int rm = c->csr_slot_t && !mixed ? 1 : 0;
         pf_bslot &S = T[k];

@@ -118,6 +118,22 @@ struct pf_block {
     pf_blk_state *st = nullptr, *st_host = nullptr;
 };
 
+// Range-fused products and sums of the mixed solver (pf_fused.hip): ranges
+// of consecutive elements, each range's distinct equations (with first-writer
+// and fixed-row flags), the local index of every element slot, the rounds
+// inside a range and the ranges grouped by colour.
+struct pf_ranges {
+    unsigned short *lmap = nullptr;    // (ntot,nels) local index in the range's equation list, 0xFFFF restrained
+    unsigned char *lround = nullptr;   // (nels) round of the element inside its range
+    int *rptr = nullptr;               // (nrange+1) offsets into ue
+    unsigned *ue = nullptr;            // distinct equations per range, ascending, flag bits on top
+    int *cperm = nullptr;              // ranges by colour
+    double *rpart = nullptr;           // (nrange) p.u per range
+    long nslot = 0, nels = 0, nrange = 0, nue = 0;
+    int nround = 1, ncol = 1;
+    std::vector<int> cstart;           // (ncol+1) colour offsets into cperm
+};
+
 struct pf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;      // stream in use
@@ -179,6 +195,11 @@ struct pf_ctx {
     int graph_iters = 0, graph_precision = -1, graph_stop = -1, graph_dot = -1;
     // Summation order of dots and sums (PF_DOT_TREE or PF_DOT_SEQ).
     int dot_order = PF_DOT_TREE;
+    // How the mixed solver sums element products into u (PF_SCATTER_*), and
+    // the ranges of the range-fused mode (built at pf_pcg_begin).
+    int scatter_mode = PF_SCATTER_SLOTS;
+    int graph_scatter = -1;
+    pf_ranges *rng = nullptr;
     // Launch shapes (pf_tune_set); 0 = the built-in choice.
     int tune_epl = 0;                  // single-RHS products: elements per lane group
     int tune_blk_waves = 0;            // block products: elements per workgroup
@@ -190,6 +211,14 @@ struct pf_ctx {
     std::vector<std::pair<int, int>> ev_used[PF_NKERNELS];   // (start, stop) indices
     size_t ev_next = 0;
 };
+
+// Range-fused mixed iteration (pf_fused.hip): build / free the ranges of the
+// current steering and fixed rows; enqueue the products with their sums into
+// u and the p.u partials; fold alpha from them.
+int pf_ranges_build(pf_ctx *c);
+void pf_ranges_drop(pf_ctx *c);
+int pf_ranges_iteration(pf_ctx *c, const float *p32);
+int pf_ranges_fold(pf_ctx *c);
 
 // Looks up, once per device (the device current), the address of the
 // allocation-free entry points' scratch array (pf_ebe.hip).

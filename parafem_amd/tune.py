@@ -91,8 +91,11 @@ def tune_ebe(ctx, r0, precision: str, reps: int = 20, warm: int = 3, use_cache: 
     `precision` on the problem loaded in `ctx` (right-hand side r0), set it
     on the context and return {"ebe_epl", "ms", "cached"}.  The PCG state
     is left at a tol-0 run: call pcg_begin/pcg_solve afterwards."""
-    key = _key(ctx, "k_ebe", precision, 1)
     cur = ctx.tune_get()
+    if precision == "mixed" and ctx.scatter_mode == "ranges":
+        # the range-fused kernel has one shape (an element per lane group)
+        return {"ebe_epl": cur["ebe_epl"], "ms": {}, "cached": True}
+    key = _key(ctx, "k_ebe", precision, 1)
     hit = lookup(key) if use_cache else None
     if hit is not None:
         ctx.tune_set(hit["ebe_epl"], cur["blk_waves"], cur["blk_rows"])
