@@ -323,7 +323,8 @@ def serve_bench(args):
         if done[0] % 25 == 0:
             log(f"{done[0]} finished; last: {msg}")
 
-    srv = S.Server(slots=args.slots, cache=cache, precision=args.serve_precision)
+    srv = S.Server(slots=args.slots, cache=cache, precision=args.serve_precision, chunk=args.chunk,
+                   persistent_elems=args.persistent_elems)
     try:
         # warm-up: compile/capture paths once
         srv.serve(S.make_stream(args.slots, 0.0, seed=5, nmin=args.nmin, nmax=args.nmin + 2))
@@ -396,6 +397,9 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serve: Poisson arrival rate (0 = 80%% of the measured closed-loop capacity)")
     ap.add_argument("--slots", type=int, default=32, help="serve: concurrent solves (one batch)")
+    ap.add_argument("--chunk", type=int, default=16, help="serve: iterations per batch run")
+    ap.add_argument("--persistent-elems", type=int, default=0,
+                    help="serve: batches of at most this many elements run as persistent kernels")
     ap.add_argument("--closed-only", action="store_true",
                     help="serve: only the closed-loop capacity of the server (no timed stream, no FP64 baseline)")
     ap.add_argument("--serve-sample", type=int, default=48,

@@ -284,6 +284,16 @@ int pf_batch_create(int device, pf_batch **batch);
 int pf_batch_destroy(pf_batch *batch);
 int pf_batch_run(pf_batch *batch, int nctx, pf_ctx *const *ctxs, int n_iters);
 int pf_batch_run_graph(pf_batch *batch, int nctx, pf_ctx *const *ctxs, int n_iters);
+/*   pf_batch_run_persistent : the iterations of every solve given, in one
+ *                        cooperative launch of resident workgroups that
+ *                        runs until every solve has stopped or max_iters
+ *                        iterations (grid-wide barriers between the
+ *                        iteration's phases, the same work and bits as the
+ *                        launches per kernel).  Needs cooperative-launch
+ *                        support (PF_ERR_ARG otherwise).  A barrier that
+ *                        waits for seconds gives up, and the next
+ *                        pf_batch_query returns PF_ERR_HIP. */
+int pf_batch_run_persistent(pf_batch *batch, int nctx, pf_ctx *const *ctxs, int max_iters);
 int pf_batch_query(pf_batch *batch, pf_pcg_result *res, int *ready);
 /* y = K x over the assembled operator in FP64 (host vectors, neq), with the
  * p123 fixed-freedom rows applied.  Used to report true residuals. */

@@ -163,6 +163,7 @@ def lib():
         "pf_batch_destroy": (i32, [vp]),
         "pf_batch_run": (i32, [vp, i32, C.POINTER(vp), i32]),
         "pf_batch_run_graph": (i32, [vp, i32, C.POINTER(vp), i32]),
+        "pf_batch_run_persistent": (i32, [vp, i32, C.POINTER(vp), i32]),
         "pf_batch_query": (i32, [vp, vp, pi]),
         "pf_dot_order_get": (i32, [vp, pi]),
         "pf_scatter_mode_set": (i32, [vp, i32]),
@@ -733,12 +734,15 @@ class Batch:
         arr = (C.c_void_p * len(ctxs))(*[c.h for c in ctxs])
         return arr
 
-    def run(self, ctxs, n: int, graph: bool = True):
-        """Enqueue n iterations of every context in ctxs (no host sync)."""
+    def run(self, ctxs, n: int, graph: bool = True, persistent: bool = False):
+        """Enqueue n iterations of every context in ctxs (no host sync):
+        a replayed HIP graph (graph), direct launches, or one persistent
+        cooperative kernel that stops early once every solve has stopped."""
         if not 1 <= len(ctxs) <= BATCH_MAX:
             raise ValueError(f"1..{BATCH_MAX} contexts per batch")
         arr = self._handles(ctxs)
-        f = lib().pf_batch_run_graph if graph else lib().pf_batch_run
+        f = (lib().pf_batch_run_persistent if persistent else
+             lib().pf_batch_run_graph if graph else lib().pf_batch_run)
         _check(f(self.h, len(ctxs), arr, int(n)), "pf_batch_run")
         self._last = list(ctxs)
 

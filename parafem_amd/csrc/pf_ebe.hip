@@ -965,99 +965,241 @@ __device__ __forceinline__ int bslot_find(const I *__restrict__ start, int ns, l
     return __popcll(__ballot(le)) - 1;
 }
 
+// The work of block b of each batched launch (the bodies below are shared by
+// the launches per kernel and by the persistent kernel).
 // Element products of every solve: the iteration's product of p (gate
-// RUN_LIVE) or, with LOOP, the true-residual product of x (gate RUN_REFINE,
-// a grid smaller than the blocks, looping).
-template <int NT, int KIND, typename OUT, bool LOOP>
-__global__ __launch_bounds__(PF_BLOCK) void kb_ebe(const pf_bslot *__restrict__ T, const long *__restrict__ est,
-                                                  int ns, long total, int gate)
+// RUN_LIVE) or, with REF, the true-residual product of x (gate RUN_REFINE).
+template <int NT, int KIND, typename OUT, bool REF>
+__device__ __forceinline__ void bs_ebe(const pf_bslot *__restrict__ T, const long *__restrict__ est, int ns, long gb,
+                                       int gate)
 {
     constexpr int EPL = Lanes<NT, KIND>::EPL;
-    auto one = [&](long gb) {
-        const int s = bslot_find(est, ns, gb);
-        const pf_bslot &S = T[s];
-        if (skip(S.st, gate)) return;
-        const long lb = gb - est[s];
-        if constexpr (LOOP)
-            ebe_block<NT, KIND, true, OUT, EPL>(lb, S.K64, S.Khi, S.Klo, S.g_g, S.x, nullptr, (OUT *)S.utemp, S.nels,
-                                                S.kstride, S.rowmajor);
-        else if constexpr (KIND == KIND_F32)
-            ebe_block<NT, KIND, true, OUT, EPL>(lb, S.K64, S.Khi, S.Klo, S.g_g, S.p, S.p32, (OUT *)S.utemp32,
-                                                S.nels, S.kstride, S.rowmajor);
-        else
-            ebe_block<NT, KIND, true, OUT, EPL>(lb, S.K64, S.Khi, S.Klo, S.g_g, S.p, nullptr, (OUT *)S.utemp, S.nels,
-                                                S.kstride, S.rowmajor);
-    };
-    if constexpr (LOOP) {
-        // solve by solve, so that a solve not due for a replacement costs one
-        // state read per workgroup (the usual case: every solve skips)
-        for (int s = 0; s < ns; ++s) {
-            if (skip(T[s].st, gate)) continue;
-            const long end = s + 1 < ns ? est[s + 1] : total;
-            for (long gb = est[s] + blockIdx.x; gb < end; gb += gridDim.x) {
-                one(gb);
-                __builtin_amdgcn_wave_barrier();   // this wave's LDS reads finish before the next block's writes
-            }
+    const int s = bslot_find(est, ns, gb);
+    const pf_bslot &S = T[s];
+    if (skip(S.st, gate)) return;
+    const long lb = gb - est[s];
+    if constexpr (REF)
+        ebe_block<NT, KIND, true, OUT, EPL>(lb, S.K64, S.Khi, S.Klo, S.g_g, S.x, nullptr, (OUT *)S.utemp, S.nels,
+                                            S.kstride, S.rowmajor);
+    else if constexpr (KIND == KIND_F32)
+        ebe_block<NT, KIND, true, OUT, EPL>(lb, S.K64, S.Khi, S.Klo, S.g_g, S.p, S.p32, (OUT *)S.utemp32, S.nels,
+                                            S.kstride, S.rowmajor);
+    else
+        ebe_block<NT, KIND, true, OUT, EPL>(lb, S.K64, S.Khi, S.Klo, S.g_g, S.p, nullptr, (OUT *)S.utemp, S.nels,
+                                            S.kstride, S.rowmajor);
+}
+
+// The true-residual products of the solves due for a replacement, solve by
+// solve (a solve not due costs one state read per workgroup), blocks
+// b0, b0 + nb, ... of each.
+template <int NT>
+__device__ __forceinline__ void bs_ebe_refine(const pf_bslot *__restrict__ T, const long *__restrict__ est, int ns,
+                                              long total, long b0, long nb)
+{
+    for (int s = 0; s < ns; ++s) {
+        if (skip(T[s].st, RUN_REFINE)) continue;
+        const long end = s + 1 < ns ? est[s + 1] : total;
+        for (long gb = est[s] + b0; gb < end; gb += nb) {
+            bs_ebe<NT, KIND_ACC, double, true>(T, est, ns, gb, RUN_REFINE);
+            __builtin_amdgcn_wave_barrier();   // this wave's LDS reads finish before the next block's writes
         }
-    } else {
-        one(xcd_block(blockIdx.x, gridDim.x));
     }
+}
+
+template <typename UT>
+__device__ __forceinline__ void bs_scatter_pu(const pf_bslot *__restrict__ T, const int *__restrict__ qst, int ns,
+                                              int b)
+{
+    const int s = bslot_find(qst, ns, b);
+    const pf_bslot &S = T[s];
+    if (skip(S.st, RUN_LIVE)) return;
+    const UT *ut;
+    if constexpr (sizeof(UT) == 4) ut = (const UT *)S.utemp32;
+    else ut = (const UT *)S.utemp;
+    scatter_pu_block<UT>(b - qst[s], S.G, S.csr_ptr, S.slot, ut, S.fixstore, S.p, S.u, S.neq, S.fix_zero,
+                         S.partials);
+}
+
+template <bool CHECON>
+__device__ __forceinline__ void bs_update(const pf_bslot *__restrict__ T, const int *__restrict__ qst, int ns, int b)
+{
+    const int s = bslot_find(qst, ns, b);
+    const pf_bslot &S = T[s];
+    if (skip(S.st, RUN_LIVE)) return;
+    update_block<CHECON>(b - qst[s], S.G, S.x, S.r, S.d, S.p, S.u, S.dinv, S.neq, S.partials, S.st);
+}
+
+__device__ __forceinline__ void bs_scatter_res(const pf_bslot *__restrict__ T, const int *__restrict__ qst, int ns,
+                                               int b)
+{
+    const int s = bslot_find(qst, ns, b);
+    const pf_bslot &S = T[s];
+    if (skip(S.st, RUN_REFINE)) return;
+    scatter_res_block(b - qst[s], S.G, S.csr_ptr, S.slot, S.utemp, S.fixstore, S.x, S.f, S.r, S.d, S.dinv, S.neq,
+                      S.partials);
+}
+
+__device__ __forceinline__ void bs_pupdate(const pf_bslot *__restrict__ T, const int *__restrict__ qst, int ns, int b)
+{
+    const int s = bslot_find(qst, ns, b);
+    const pf_bslot &S = T[s];
+    if (skip(S.st, RUN_LIVE)) return;
+    pupdate_block(b - qst[s], S.G, S.p, S.p32, S.d, S.neq, S.st);
+}
+
+// The folds of solve s: 0 alpha, 1 update (checon), 2 update (residual),
+// 3 true-residual replacement.
+template <int WHICH>
+__device__ __forceinline__ void bs_fold(const pf_bslot *__restrict__ T, int s)
+{
+    const pf_bslot &S = T[s];
+    if (skip(S.st, WHICH == 3 ? RUN_REFINE : RUN_LIVE)) return;
+    if constexpr (WHICH == 0) fold_alpha_block(S.partials, S.G, S.st);
+    else if constexpr (WHICH == 1) fold_update_block<true>(S.partials, S.G, S.st);
+    else if constexpr (WHICH == 2) fold_update_block<false>(S.partials, S.G, S.st);
+    else fold_refine_block(S.partials, S.G, S.st);
+}
+
+// One launch per kernel of the iteration.
+template <int NT, int KIND, typename OUT>
+__global__ __launch_bounds__(PF_BLOCK) void kb_ebe(const pf_bslot *__restrict__ T, const long *__restrict__ est, int ns)
+{
+    bs_ebe<NT, KIND, OUT, false>(T, est, ns, xcd_block(blockIdx.x, gridDim.x), RUN_LIVE);
+}
+
+template <int NT>
+__global__ __launch_bounds__(PF_BLOCK) void kb_ebe_refine(const pf_bslot *__restrict__ T, const long *__restrict__ est,
+                                                         int ns, long total)
+{
+    bs_ebe_refine<NT>(T, est, ns, total, blockIdx.x, gridDim.x);
 }
 
 template <typename UT>
 __global__ __launch_bounds__(PF_BLOCK) void kb_scatter_pu(const pf_bslot *__restrict__ T, const int *__restrict__ qst,
                                                          int ns)
 {
-    const int s = bslot_find(qst, ns, blockIdx.x);
-    const pf_bslot &S = T[s];
-    if (skip(S.st, RUN_LIVE)) return;
-    const UT *ut;
-    if constexpr (sizeof(UT) == 4) ut = (const UT *)S.utemp32;
-    else ut = (const UT *)S.utemp;
-    scatter_pu_block<UT>(blockIdx.x - qst[s], S.G, S.csr_ptr, S.slot, ut, S.fixstore, S.p, S.u, S.neq, S.fix_zero,
-                         S.partials);
+    bs_scatter_pu<UT>(T, qst, ns, blockIdx.x);
 }
 
 template <bool CHECON>
 __global__ __launch_bounds__(PF_BLOCK) void kb_update(const pf_bslot *__restrict__ T, const int *__restrict__ qst,
                                                      int ns)
 {
-    const int s = bslot_find(qst, ns, blockIdx.x);
-    const pf_bslot &S = T[s];
-    if (skip(S.st, RUN_LIVE)) return;
-    update_block<CHECON>(blockIdx.x - qst[s], S.G, S.x, S.r, S.d, S.p, S.u, S.dinv, S.neq, S.partials, S.st);
+    bs_update<CHECON>(T, qst, ns, blockIdx.x);
 }
 
 __global__ __launch_bounds__(PF_BLOCK) void kb_scatter_res(const pf_bslot *__restrict__ T, const int *__restrict__ qst,
                                                           int ns)
 {
-    const int s = bslot_find(qst, ns, blockIdx.x);
-    const pf_bslot &S = T[s];
-    if (skip(S.st, RUN_REFINE)) return;
-    scatter_res_block(blockIdx.x - qst[s], S.G, S.csr_ptr, S.slot, S.utemp, S.fixstore, S.x, S.f, S.r, S.d, S.dinv,
-                      S.neq, S.partials);
+    bs_scatter_res(T, qst, ns, blockIdx.x);
 }
 
 __global__ __launch_bounds__(PF_BLOCK) void kb_pupdate(const pf_bslot *__restrict__ T, const int *__restrict__ qst,
                                                       int ns)
 {
-    const int s = bslot_find(qst, ns, blockIdx.x);
-    const pf_bslot &S = T[s];
-    if (skip(S.st, RUN_LIVE)) return;
-    pupdate_block(blockIdx.x - qst[s], S.G, S.p, S.p32, S.d, S.neq, S.st);
+    bs_pupdate(T, qst, ns, blockIdx.x);
 }
 
-// One block per solve: 0 alpha, 1 update (checon), 2 update (residual),
-// 3 true-residual replacement.
 template <int WHICH>
 __global__ __launch_bounds__(PF_BLOCK) void kb_fold(const pf_bslot *__restrict__ T)
 {
-    const pf_bslot &S = T[blockIdx.x];
-    if (skip(S.st, WHICH == 3 ? RUN_REFINE : RUN_LIVE)) return;
-    if constexpr (WHICH == 0) fold_alpha_block(S.partials, S.G, S.st);
-    else if constexpr (WHICH == 1) fold_update_block<true>(S.partials, S.G, S.st);
-    else if constexpr (WHICH == 2) fold_update_block<false>(S.partials, S.G, S.st);
-    else fold_refine_block(S.partials, S.G, S.st);
+    bs_fold<WHICH>(T, blockIdx.x);
+}
+
+// ------------------------------------------ persistent batched iterations
+// Grid-wide barrier of a cooperative launch (every workgroup resident): an
+// arrival counter and a generation word.  The last workgroup to arrive
+// resets the counter and advances the generation; the others wait for the
+// generation to move, at most about 2^26 polls (seconds).  A wait that runs
+// out sets the error word, which every other waiting workgroup sees, and the
+// barrier returns false: the kernel then ends in every workgroup.  Agent-
+// scope release/acquire make each workgroup's writes visible to the others
+// (the L2 of every XCD) across the barrier.
+struct pf_gbar {
+    unsigned count;
+    unsigned gen;
+    int error;
+    int iters;   // iterations the persistent kernel completed
+};
+
+__device__ bool grid_barrier(pf_gbar *g, unsigned nb)
+{
+    __shared__ int ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int good = 1;
+        const unsigned gen = __hip_atomic_load(&g->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned arrived =
+            __hip_atomic_fetch_add(&g->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (arrived == nb) {
+            __hip_atomic_store(&g->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&g->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long polls = 0;
+            while (__hip_atomic_load(&g->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (__hip_atomic_load(&g->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++polls > (1L << 26)) {
+                    __hip_atomic_store(&g->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    good = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+// The iteration of every solve, repeated until every solve has stopped or
+// max_iters iterations have run, in one cooperative launch: each phase's
+// blocks (the same blocks, with the same work, as the launches per kernel)
+// are dealt to the resident workgroups in rounds, XCD-contiguous within a
+// round, with a grid barrier between phases.  The true-residual phases run
+// only in iterations where some solve asks for a replacement.
+template <int NT, int KIND, typename OUT, bool CHECON, bool REF>
+__global__ __launch_bounds__(PF_BLOCK) void kb_persistent(const pf_bslot *__restrict__ T, const long *__restrict__ est,
+                                                         const int *__restrict__ qst, int ns, long total_eb,
+                                                         long total_ebr, int total_qb, int max_iters, pf_gbar *bar)
+{
+    const unsigned nb = gridDim.x;
+    const long lbx = xcd_block(blockIdx.x, nb);
+    int it = 0;
+    for (; it < max_iters; ++it) {
+        bool any = false;
+        for (int s = 0; s < ns; ++s) any |= !T[s].st->frozen;   // the same in every workgroup
+        if (!any) break;
+        for (long r0 = 0; r0 < total_eb; r0 += nb) {
+            const long gb = r0 + lbx;
+            if (gb < total_eb) bs_ebe<NT, KIND, OUT, false>(T, est, ns, gb, RUN_LIVE);
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (!grid_barrier(bar, nb)) return;
+        for (int b = blockIdx.x; b < total_qb; b += nb) bs_scatter_pu<OUT>(T, qst, ns, b);
+        if (!grid_barrier(bar, nb)) return;
+        for (int s = blockIdx.x; s < ns; s += nb) bs_fold<0>(T, s);
+        if (!grid_barrier(bar, nb)) return;
+        for (int b = blockIdx.x; b < total_qb; b += nb) bs_update<CHECON>(T, qst, ns, b);
+        if (!grid_barrier(bar, nb)) return;
+        for (int s = blockIdx.x; s < ns; s += nb) bs_fold<CHECON ? 1 : 2>(T, s);
+        if (!grid_barrier(bar, nb)) return;
+        if constexpr (REF) {
+            bool due = false;
+            for (int s = 0; s < ns; ++s) due |= !skip(T[s].st, RUN_REFINE);
+            if (due) {
+                bs_ebe_refine<NT>(T, est + PF_BATCH_MAX, ns, total_ebr, blockIdx.x, nb);
+                if (!grid_barrier(bar, nb)) return;
+                for (int b = blockIdx.x; b < total_qb; b += nb) bs_scatter_res(T, qst, ns, b);
+                if (!grid_barrier(bar, nb)) return;
+                for (int s = blockIdx.x; s < ns; s += nb) bs_fold<3>(T, s);
+                if (!grid_barrier(bar, nb)) return;
+            }
+        }
+        for (int b = blockIdx.x; b < total_qb; b += nb) bs_pupdate(T, qst, ns, b);
+        if (!grid_barrier(bar, nb)) return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) bar->iters = it;
 }
 
 // Copies of every solve's state (pf_batch_query's snapshot).
@@ -1666,6 +1808,11 @@ struct pf_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_iters = 0;
+    // persistent runs: the grid barrier (device) and its copy taken with
+    // each snapshot, the device's CU count and cooperative-launch support
+    pf_gbar *bar = nullptr, *bar_host = nullptr;
+    int cus = 0, coop = 0;
+    bool persistent_used = false;
 };
 
 static const size_t kBatchTable = sizeof(pf_bslot) * PF_BATCH_MAX + 2 * sizeof(long) * PF_BATCH_MAX +
@@ -1800,8 +1947,8 @@ static int batch_release(pf_batch *b, int nctx, pf_ctx *const *ctxs)
 template <int NT, int KIND, typename OUT>
 static void batch_launch_ebe(pf_batch *b)
 {
-    hipLaunchKernelGGL((kb_ebe<NT, KIND, OUT, false>), dim3((unsigned)b->total_eb), dim3(PF_BLOCK), 0, b->stream, b->T,
-                       b->est, b->ns, b->total_eb, (int)RUN_LIVE);
+    hipLaunchKernelGGL((kb_ebe<NT, KIND, OUT>), dim3((unsigned)b->total_eb), dim3(PF_BLOCK), 0, b->stream, b->T,
+                       b->est, b->ns);
 }
 
 template <int NT>
@@ -1809,8 +1956,8 @@ static void batch_launch_refine(pf_batch *b)
 {
     // usually does nothing: few workgroups that loop over every solve's blocks
     const int grid = (int)std::min<long>(b->total_ebr, 2048);
-    hipLaunchKernelGGL((kb_ebe<NT, KIND_ACC, double, true>), dim3(grid), dim3(PF_BLOCK), 0, b->stream, b->T,
-                       b->est + PF_BATCH_MAX, b->ns, b->total_ebr, (int)RUN_REFINE);
+    hipLaunchKernelGGL(kb_ebe_refine<NT>, dim3(grid), dim3(PF_BLOCK), 0, b->stream, b->T, b->est + PF_BATCH_MAX,
+                       b->ns, b->total_ebr);
 }
 
 template <int NT>
@@ -1840,6 +1987,51 @@ static void batch_iteration_nt(pf_batch *b)
         hipLaunchKernelGGL(kb_fold<3>, gs, blk, 0, b->stream, b->T);
     }
     hipLaunchKernelGGL(kb_pupdate, gq, blk, 0, b->stream, b->T, b->qst, b->ns);
+}
+
+template <int NT, int KIND, typename OUT, bool CHECON, bool REF>
+static int batch_launch_persistent(pf_batch *b, int max_iters)
+{
+    const void *kern = reinterpret_cast<const void *>(&kb_persistent<NT, KIND, OUT, CHECON, REF>);
+    int per_cu = 0;
+    PF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, PF_BLOCK, 0));
+    if (per_cu < 1) return PF_ERR_ARG;
+    // every workgroup resident (the cooperative launch refuses otherwise),
+    // no more than the largest phase has blocks
+    const long most = std::max<long>(std::max<long>(b->total_eb, b->total_qb), b->ns);
+    int grid = (int)std::min<long>((long)per_cu * b->cus, most);
+    if (grid < 1) grid = 1;
+    PF_HIP(hipMemsetAsync(b->bar, 0, sizeof(pf_gbar), b->stream));
+    const pf_bslot *T = b->T;
+    const long *est = b->est;
+    const int *qst = b->qst;
+    int ns = b->ns, tq = b->total_qb, mi = max_iters;
+    long te = b->total_eb, ter = b->total_ebr;
+    pf_gbar *bar = b->bar;
+    void *args[] = {(void *)&T, (void *)&est, (void *)&qst, (void *)&ns, (void *)&te, (void *)&ter, (void *)&tq,
+                    (void *)&mi, (void *)&bar};
+    PF_HIP(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(PF_BLOCK), args, 0, b->stream));
+    return PF_OK;
+}
+
+template <int NT>
+static int batch_persistent_nt(pf_batch *b, int max_iters)
+{
+    const bool ch = b->stop == PF_STOP_CHECON;
+    switch (b->precision) {
+    case PF_PREC_MIXED:
+        return ch ? batch_launch_persistent<NT, KIND_F32, float, true, true>(b, max_iters)
+                  : batch_launch_persistent<NT, KIND_F32, float, false, true>(b, max_iters);
+    case PF_PREC_MIXED64:
+        return ch ? batch_launch_persistent<NT, KIND_WIDE, double, true, true>(b, max_iters)
+                  : batch_launch_persistent<NT, KIND_WIDE, double, false, true>(b, max_iters);
+    case PF_PREC_SPLIT:
+        return ch ? batch_launch_persistent<NT, KIND_ACC, double, true, false>(b, max_iters)
+                  : batch_launch_persistent<NT, KIND_ACC, double, false, false>(b, max_iters);
+    default:
+        return ch ? batch_launch_persistent<NT, KIND_F64, double, true, false>(b, max_iters)
+                  : batch_launch_persistent<NT, KIND_F64, double, false, false>(b, max_iters);
+    }
 }
 
 static int batch_iterations(pf_batch *b, int n)
@@ -1881,6 +2073,10 @@ int pf_batch_create(int device, pf_batch **out)
     if (e == hipSuccess) e = hipMalloc(&dt, kBatchTable);
     if (e == hipSuccess) e = hipHostMalloc(&ht, kBatchTable, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void **)&b->snap, sizeof(pf_state) * PF_BATCH_MAX);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->bar, sizeof(pf_gbar));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b->bar_host, sizeof(pf_gbar), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&b->coop, hipDeviceAttributeCooperativeLaunch, device);
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->snap_host, sizeof(pf_state) * PF_BATCH_MAX, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventRecord(b->ev_table, b->stream);
     if (dt) {
@@ -1912,6 +2108,8 @@ int pf_batch_destroy(pf_batch *b)
     if (b->T_host) (void)hipHostFree(b->T_host);
     if (b->snap) (void)hipFree(b->snap);
     if (b->snap_host) (void)hipHostFree(b->snap_host);
+    if (b->bar) (void)hipFree(b->bar);
+    if (b->bar_host) (void)hipHostFree(b->bar_host);
     for (hipEvent_t e : b->ev_ctx)
         if (e) (void)hipEventDestroy(e);
     if (b->ev_done) (void)hipEventDestroy(b->ev_done);
@@ -1960,6 +2158,22 @@ int pf_batch_run_graph(pf_batch *b, int nctx, pf_ctx *const *ctxs, int n_iters)
     return batch_release(b, nctx, ctxs);
 }
 
+int pf_batch_run_persistent(pf_batch *b, int nctx, pf_ctx *const *ctxs, int max_iters)
+{
+    if (!b || max_iters < 1) return PF_ERR_ARG;
+    if (!b->coop) return PF_ERR_ARG;   // the device cannot guarantee a resident grid
+    int rc = batch_prepare(b, nctx, ctxs);
+    if (rc) return rc;
+    switch (b->ntot) {
+    case 60: rc = batch_persistent_nt<60>(b, max_iters); break;
+    case 24: rc = batch_persistent_nt<24>(b, max_iters); break;
+    default: rc = batch_persistent_nt<8>(b, max_iters); break;
+    }
+    if (rc) return rc;
+    b->persistent_used = true;
+    return batch_release(b, nctx, ctxs);
+}
+
 int pf_batch_query(pf_batch *b, pf_pcg_result *res, int *ready)
 {
     if (!b || !res || !ready) return PF_ERR_ARG;
@@ -1969,6 +2183,8 @@ int pf_batch_query(pf_batch *b, pf_pcg_result *res, int *ready)
         hipLaunchKernelGGL(kb_snapshot, dim3(b->ns), dim3(64), 0, b->stream, b->T, b->ns, b->snap);
         PF_HIP(hipGetLastError());
         PF_HIP(hipMemcpyAsync(b->snap_host, b->snap, sizeof(pf_state) * b->ns, hipMemcpyDeviceToHost, b->stream));
+        if (b->persistent_used)
+            PF_HIP(hipMemcpyAsync(b->bar_host, b->bar, sizeof(pf_gbar), hipMemcpyDeviceToHost, b->stream));
         PF_HIP(hipEventRecord(b->ev_snap, b->stream));
         b->snap_pending = true;
         b->snap_n = b->ns;
@@ -1977,6 +2193,11 @@ int pf_batch_query(pf_batch *b, pf_pcg_result *res, int *ready)
     if (q == hipErrorNotReady) return PF_OK;
     if (q != hipSuccess) return pf_set_hip_error(q, "hipEventQuery", __FILE__, __LINE__);
     b->snap_pending = false;
+    if (b->persistent_used && b->bar_host->error) {
+        // a persistent run's grid barrier ran out of time: its solves stopped
+        // part-way through an iteration
+        return pf_set_hip_error(hipErrorLaunchTimeOut, "persistent batch grid barrier", __FILE__, __LINE__);
+    }
     for (int k = 0; k < b->snap_n; ++k) fill_result(b->snap_host[k], &res[k]);
     *ready = 1;
     return PF_OK;

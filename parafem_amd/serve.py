@@ -28,6 +28,8 @@ load of the top face.
 """
 from __future__ import annotations
 
+import queue as queue_mod
+import threading
 import time
 from collections import OrderedDict, deque
 from dataclasses import dataclass, field
@@ -139,7 +141,10 @@ class _Slot:
 class Server:
     """Continuous batching: every admitted solve is a slot of one pf_batch,
     so each launch of the iteration's kernels advances all of them; solves
-    join and leave the batch between chunks of iterations."""
+    join and leave the batch between chunks of iterations.  Set-up of
+    arriving requests and read-back of finished ones run on a second host
+    thread (the library's calls release the interpreter lock), so the
+    thread driving the batch never waits for them."""
     slots: int = 32
     hbm_budget: float = 240e9
     chunk: int = 16
@@ -149,113 +154,173 @@ class Server:
     keep_x: bool = False
     device: int = 0
     cache: CubeCache = field(default_factory=CubeCache)
+    # batches of at most this many elements in total run each chunk as one
+    # persistent cooperative kernel (pf_batch_run_persistent) instead of a
+    # replayed graph of launches: small solves are launch-bound
+    persistent_elems: int = 0
 
     def __post_init__(self):
         if not 1 <= self.slots <= BATCH_MAX:
             raise ValueError(f"1..{BATCH_MAX} slots")
         self._slots = [_Slot(Context(self.device)) for _ in range(self.slots)]
         self._batch = Batch(self.device)
+        # host-side time of the last serve() on the worker thread: set-up of
+        # requests, read-backs of finished solves; cache trims (each waits
+        # for the device) and batch cycles on the driving thread
+        self.stats = {"admit_s": 0.0, "readback_s": 0.0, "trims": 0, "cycles": 0}
 
     def close(self):
         self._batch.close()
         for s in self._slots:
             s.ctx.close()
 
-    def _device_bytes(self) -> int:
+    def _device_bytes(self, with_worker: set) -> int:
         """Device memory the slots hold now: live buffers and each context's
-        cache of freed blocks."""
-        return sum(s.ctx.mem_used() for s in self._slots)
+        cache of freed blocks, read from the contexts this thread owns, and
+        the estimate of what a context on the worker holds (it is not
+        touched from two threads)."""
+        return sum(s.bytes if id(s) in with_worker else s.ctx.mem_used() for s in self._slots)
 
-    def _fits(self, slot: _Slot, need: int, busy: bool) -> bool:
+    def _fits(self, slot: _Slot, need: int, busy: bool, idle: list[_Slot], with_worker: set) -> bool:
         """Whether a request needing `need` bytes can start on `slot`:
-        within the HBM budget after handing back the slot's cached blocks,
-        then those of idle slots, if needed.  With nothing running a request
-        is always admitted."""
-        if self._device_bytes() + need <= self.hbm_budget:
+        within the HBM budget after handing back the cached blocks of the
+        slot, then of the other idle slots, if needed.  With nothing running
+        a request is always admitted."""
+        if self._device_bytes(with_worker) + need <= self.hbm_budget:
             return True
-        slot.ctx.trim()
-        if self._device_bytes() + need <= self.hbm_budget:
-            return True
-        for s in self._slots:
-            if s.req is None:
-                s.ctx.trim()
-        return self._device_bytes() + need <= self.hbm_budget or not busy
+        for s in [slot] + [t for t in idle if t is not slot]:
+            s.ctx.trim()
+            self.stats["trims"] += 1
+            if self._device_bytes(with_worker) + need <= self.hbm_budget:
+                return True
+        return not busy
 
-    def _admit(self, s: _Slot, req: Request, now: float):
+    def _admit(self, s: _Slot, req: Request):
         c = self.cache.get(req.n)
         ctx = s.ctx
         ctx.mesh_set(c["g_coord"])
         ctx.steer_build(c["g_num"], c["rest"], c["nn"])      # steering and maps built on the device
         ctx.assemble_elastic(req.e, req.v, None, store="fp64" if self.precision == "fp64" else "fp32")
         ctx.pcg_begin(c["r_unit"] * req.pressure, self.limit, self.tol, stop="relres", precision=self.precision)
-        s.req, s.t_admit, s.bytes = req, now, device_bytes(req.n, self.precision)
+
+    def _worker(self, jobs: queue_mod.Queue, results: queue_mod.Queue):
+        """Admissions and retirements, one at a time, on their slot's
+        context (no slot is in the batch while it is here)."""
+        while True:
+            job = jobs.get()
+            if job is None:
+                return
+            kind, s, payload, t0 = job
+            tb = time.perf_counter()
+            try:
+                if kind == "admit":
+                    self._admit(s, payload)
+                    s.t_admit = tb - t0
+                    self.stats["admit_s"] += time.perf_counter() - tb
+                    results.put(("admitted", s, None))
+                else:
+                    r = payload
+                    x = s.ctx.pcg_x()
+                    t = time.perf_counter() - t0
+                    q = s.req
+                    res = Result(q.rid, q.n, s.ctx.neq, r.iters, bool(r.converged), r.refinements, s.t_admit, t,
+                                 t - q.arrival, x if self.keep_x else None, float(np.abs(x).sum()))
+                    s.ctx.reset()
+                    self.stats["readback_s"] += time.perf_counter() - tb
+                    results.put(("retired", s, res))
+            except Exception as exc:              # handed to the driving thread
+                results.put(("error", s, exc))
 
     def serve(self, requests: list[Request], log=None) -> list[Result]:
         """Run the stream in real time (arrival times relative to the start).
 
-        Each cycle enqueues a chunk of iterations for the current slots,
-        takes a state snapshot behind it and enqueues one more chunk, so the
-        GPU keeps working while the host waits for the snapshot; when it
-        arrives, finished solves are read back and retired, waiting requests
-        admitted, and the next cycle runs the new set."""
+        The driving thread enqueues a chunk of iterations for the current
+        slots, takes a state snapshot behind it and enqueues one more chunk,
+        so the GPU keeps working while it waits for the snapshot; when it
+        arrives, finished solves leave the batch (the worker reads them back
+        and frees their buffers), requests the worker has set up join it,
+        and the next cycle runs the new set."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
-        queue: deque[Request] = deque()
+        waiting: deque[Request] = deque()
         free = list(self._slots)
         active: list[_Slot] = []
+        ready: list[_Slot] = []          # set up by the worker, not yet in the batch
+        busy_worker = 0                  # jobs handed to the worker and not back yet
+        with_worker: set = set()         # id() of the slots the worker holds
         out: list[Result] = []
         by_ctx = {s.ctx.h: s for s in self._slots}
-        done: list[tuple] = []           # (slot, result) finished, still to read back
         in_flight = False
+        self.stats = {"admit_s": 0.0, "readback_s": 0.0, "trims": 0, "cycles": 0}
+        jobs: queue_mod.Queue = queue_mod.Queue()
+        results: queue_mod.Queue = queue_mod.Queue()
+        worker = threading.Thread(target=self._worker, args=(jobs, results), daemon=True)
+        worker.start()
         t0 = time.perf_counter()
-        while pending or queue or active or done:
-            now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now:
-                queue.append(pending.popleft())
-            if not in_flight:
-                while queue and free:
-                    need = device_bytes(queue[0].n, self.precision)
-                    if not self._fits(free[-1], need, bool(active)):
+        try:
+            while pending or waiting or active or ready or busy_worker:
+                now = time.perf_counter() - t0
+                while pending and pending[0].arrival <= now:
+                    waiting.append(pending.popleft())
+                # hand arrived requests to the worker while slots and memory allow
+                while waiting and free:
+                    need = device_bytes(waiting[0].n, self.precision)
+                    if not self._fits(free[-1], need, bool(active or ready or busy_worker), free, with_worker):
                         break
                     s = free.pop()
-                    self._admit(s, queue.popleft(), time.perf_counter() - t0)
-                    active.append(s)
-                if active:
+                    s.req = waiting.popleft()
+                    s.bytes = need
+                    with_worker.add(id(s))
+                    jobs.put(("admit", s, s.req, t0))
+                    busy_worker += 1
+                # what the worker finished
+                while True:
+                    try:
+                        kind, s, res = results.get_nowait()
+                    except queue_mod.Empty:
+                        break
+                    busy_worker -= 1
+                    with_worker.discard(id(s))
+                    if kind == "error":
+                        raise res
+                    if kind == "admitted":
+                        ready.append(s)
+                    else:
+                        out.append(res)
+                        if log:
+                            log(f"request {res.rid} n={res.n} iters={res.iters} latency={res.latency:.3f}s")
+                        s.req = None
+                        free.append(s)
+                if not in_flight and (active or ready):
+                    active.extend(ready)
+                    ready = []
                     ctxs = [s.ctx for s in active]
-                    self._batch.run(ctxs, self.chunk)
+                    pers = sum(s.req.n ** 3 for s in active) <= self.persistent_elems
+                    self._batch.run(ctxs, self.chunk, persistent=pers)
                     self._batch.query()              # snapshot behind this chunk
-                    self._batch.run(ctxs, self.chunk)   # and one more chunk queued
+                    self._batch.run(ctxs, self.chunk, persistent=pers)   # and one more chunk queued
                     in_flight = True
-            # read back the solves that finished in the previous snapshot
-            # while the GPU works on the chunks just queued
-            for s, r in done:
-                x = s.ctx.pcg_x()
-                t = time.perf_counter() - t0
-                q = s.req
-                out.append(Result(q.rid, q.n, s.ctx.neq, r.iters, bool(r.converged), r.refinements,
-                                  s.t_admit, t, t - q.arrival, x if self.keep_x else None,
-                                  float(np.abs(x).sum())))
-                if log:
-                    log(f"request {q.rid} n={q.n} iters={r.iters} latency={t - q.arrival:.3f}s")
-                s.ctx.reset()
-                s.req = None
-                free.append(s)
-            done = []
-            got = self._batch.query() if in_flight else None
-            if got is None:
-                if not active and pending:
-                    time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
-                elif in_flight:
-                    time.sleep(20e-6)
-                continue
-            in_flight = False
-            snap, res = got
-            for ctx, r in zip(snap, res):
-                s = by_ctx[ctx.h]
-                # converged, broken down (status 6) or at the limit: the solve
-                # is frozen on the device; anything else is still running
-                if r.converged or r.status == 6 or r.iters >= self.limit:
-                    active.remove(s)
-                    done.append((s, r))
+                    self.stats["cycles"] += 1
+                got = self._batch.query() if in_flight else None
+                if got is None:
+                    if not in_flight and not busy_worker and pending and not waiting:
+                        time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
+                    else:
+                        time.sleep(20e-6)
+                    continue
+                in_flight = False
+                snap, res_list = got
+                for ctx, r in zip(snap, res_list):
+                    s = by_ctx[ctx.h]
+                    # converged, broken down (status 6) or at the limit: the
+                    # solve is frozen on the device; anything else still runs
+                    if r.converged or r.status == 6 or r.iters >= self.limit:
+                        active.remove(s)
+                        with_worker.add(id(s))
+                        jobs.put(("retire", s, r, t0))
+                        busy_worker += 1
+        finally:
+            jobs.put(None)
+            worker.join()
         return sorted(out, key=lambda r: r.rid)
 
 

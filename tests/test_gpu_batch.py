@@ -29,6 +29,8 @@ def _elastic_mix():
 
 
 def _run_batch(ctxs, rs, limit, tol, stop, precision, graph, chunk=7):
+    """graph: True (replayed graph), False (direct launches) or
+    "persistent" (one cooperative kernel per chunk)."""
     for c, r in zip(ctxs, rs):
         c.pcg_begin(r, limit, tol, stop=stop, precision=precision)
     b = Batch(0)
@@ -36,7 +38,7 @@ def _run_batch(ctxs, rs, limit, tol, stop, precision, graph, chunk=7):
         live = list(range(len(ctxs)))
         done = {}
         while live:
-            b.run([ctxs[k] for k in live], chunk, graph=graph)
+            b.run([ctxs[k] for k in live], chunk, graph=graph is True, persistent=graph == "persistent")
             got = None
             while got is None:
                 got = b.query()
@@ -56,7 +58,11 @@ def _run_batch(ctxs, rs, limit, tol, stop, precision, graph, chunk=7):
 
 @pytest.mark.parametrize("precision,stop,graph", [("fp64", "checon", True), ("fp64", "relres", False),
                                                   ("split", "relres", True), ("mixed64", "relres", True),
-                                                  ("mixed", "relres", True), ("mixed", "relres", False)])
+                                                  ("mixed", "relres", True), ("mixed", "relres", False),
+                                                  ("fp64", "checon", "persistent"),
+                                                  ("split", "relres", "persistent"),
+                                                  ("mixed64", "relres", "persistent"),
+                                                  ("mixed", "relres", "persistent")])
 def test_batch_equals_alone_hex20(precision, stop, graph):
     probs = _elastic_mix()
     store = "fp64" if precision == "fp64" else "fp32"
@@ -86,8 +92,9 @@ def test_batch_equals_alone_hex20(precision, stop, graph):
             c.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batch_equals_alone_hex8_laplace_penalty(precision):
+@pytest.mark.parametrize("precision,graph", [("fp64", True), ("mixed", True), ("fp64", "persistent"),
+                                             ("mixed", "persistent")])
+def test_batch_equals_alone_hex8_laplace_penalty(precision, graph):
     """hex8 Laplace cubes with penalty fixed freedoms (p123's fixed rows in
     the scatter) and, for the mixed path, the FP32 copy of p."""
     probs = [P.laplace_cube(n, lognormal_k=True, seed=s) for n, s in ((5, 2), (3, 7), (7, 9))]
@@ -97,8 +104,8 @@ def test_batch_equals_alone_hex8_laplace_penalty(precision):
         rs = [P.setup(c, pr, store=store) for c, pr in zip(ctxs, probs)]
         say(f"batch hex8 laplace {precision}: alone")
         alone = [c.pcg_solve(r, 3000, 1e-10, stop="relres", precision=precision) for c, r in zip(ctxs, rs)]
-        say(f"batch hex8 laplace {precision}: batched")
-        got = _run_batch(ctxs, rs, 3000, 1e-10, "relres", precision, True, chunk=5)
+        say(f"batch hex8 laplace {precision} {graph}: batched")
+        got = _run_batch(ctxs, rs, 3000, 1e-10, "relres", precision, graph, chunk=5)
         for (xa, ra), (ib, cb, xb) in zip(alone, got):
             assert ra.converged and cb and ra.iters == ib
             assert np.array_equal(xa, xb)
