@@ -56,7 +56,12 @@ class Mesh:
 
     def g_coord(self) -> np.ndarray:
         """Per-element coordinates in the reference ``g_coord_pp(nod,3,nels)``
-        memory order: returned as (nels, 3, nod) C-order float64."""
+        memory order: returned as (nels, 3, nod) C-order float64.  A
+        generator that formed them element by element keeps them (read-only)
+        for as long as ``coords`` is the array it made."""
+        gc = self.extra.get("g_coord")
+        if gc is not None and self.extra.get("g_coord_of") is self.coords:
+            return gc
         return element_coords(self.g_num, self.coords)
 
 
@@ -80,62 +85,61 @@ def box_hex20(nxe: int, nye: int, nze: int, aa: float = 1.0, bb: float = 1.0,
     """20-node brick box in ParaFEM's node numbering (S&G element order).
 
     Follows GEOMETRY_20BXZ (``geometry.f90:172-286``) for every element at
-    once; nodal coordinates are the element-local ones scattered to nodes.
+    once: five base node numbers per element, each local node one of them
+    plus a fixed offset; the element's coordinates pick, per local node and
+    axis, the lower face, the upper face or their midpoint (the same
+    expressions for every element sharing a node, so nodal coordinates are
+    the element-local ones scattered to nodes and the per-element array is
+    kept as ``g_coord``).
     """
     nels = nxe * nye * nze
     ip, iq, is_ = _elem_ijk(nels, nxe, nze)
     per_plane = (2 * nxe + 1) * (nze + 1) + (2 * nze + 1) * (nxe + 1)
     fac1 = per_plane * (iq - 1)
     fac2 = per_plane * iq
-    n = np.empty((nels, 20), dtype=np.int64)
-    n[:, 0] = fac1 + (3 * nxe + 2) * is_ + 2 * ip - 1
-    n[:, 1] = fac1 + (3 * nxe + 2) * is_ - nxe + ip - 1
-    n[:, 2] = n[:, 0] - 3 * nxe - 2
-    n[:, 3] = n[:, 2] + 1
-    n[:, 4] = n[:, 3] + 1
-    n[:, 5] = n[:, 1] + 1
-    n[:, 6] = n[:, 0] + 2
-    n[:, 7] = n[:, 0] + 1
-    n[:, 8] = fac2 - (nxe + 1) * (nze + 1) + (nxe + 1) * is_ + ip
-    n[:, 9] = n[:, 8] - nxe - 1
-    n[:, 10] = n[:, 9] + 1
-    n[:, 11] = n[:, 8] + 1
-    n[:, 12] = fac2 + (3 * nxe + 2) * is_ + 2 * ip - 1
-    n[:, 13] = fac2 + (3 * nxe + 2) * is_ - nxe + ip - 1
-    n[:, 14] = n[:, 12] - 3 * nxe - 2
-    n[:, 15] = n[:, 14] + 1
-    n[:, 16] = n[:, 15] + 1
-    n[:, 17] = n[:, 13] + 1
-    n[:, 18] = n[:, 12] + 2
-    n[:, 19] = n[:, 12] + 1
-
-    x0 = (ip - 1) * aa
-    x1 = ip * aa
-    y0 = (iq - 1) * bb
-    y1 = iq * bb
-    zb = -is_ * cc           # lower face of the element (more negative)
-    zt = -(is_ - 1) * cc     # upper face
-    c = np.empty((nels, 20, 3), dtype=np.float64)
-    for l in (0, 1, 2, 8, 9, 12, 13, 14):
-        c[:, l, 0] = x0
-    for l in (4, 5, 6, 10, 11, 16, 17, 18):
-        c[:, l, 0] = x1
-    for l in (3, 7, 15, 19):
-        c[:, l, 0] = 0.5 * (x0 + x1)
-    c[:, 0:8, 1] = y0[:, None]
-    c[:, 12:20, 1] = y1[:, None]
-    c[:, 8:12, 1] = 0.5 * (y0 + y1)[:, None]
-    for l in (0, 6, 7, 8, 11, 12, 18, 19):
-        c[:, l, 2] = zb
-    for l in (2, 3, 4, 9, 10, 14, 15, 16):
-        c[:, l, 2] = zt
-    for l in (1, 5, 13, 17):
-        c[:, l, 2] = 0.5 * (zb + zt)
     nn = ((2 * nxe + 1) * (nze + 1) + (nxe + 1) * nze) * (nye + 1) \
         + (nxe + 1) * (nze + 1) * nye
+    it = np.int32 if nn < 2 ** 31 - 1 else np.int64
+    base = np.empty((nels, 5), dtype=it)
+    base[:, 0] = fac1 + (3 * nxe + 2) * is_ + 2 * ip - 1       # local nodes 1-3, 4-8
+    base[:, 1] = fac1 + (3 * nxe + 2) * is_ - nxe + ip - 1     # 2, 6
+    base[:, 2] = fac2 - (nxe + 1) * (nze + 1) + (nxe + 1) * is_ + ip   # 9-12
+    base[:, 3] = fac2 + (3 * nxe + 2) * is_ + 2 * ip - 1       # 13, 15-17, 19, 20
+    base[:, 4] = fac2 + (3 * nxe + 2) * is_ - nxe + ip - 1     # 14, 18
+    which = np.array([0, 1, 0, 0, 0, 1, 0, 0, 2, 2, 2, 2, 3, 4, 3, 3, 3, 4, 3, 3])
+    step = np.array([0, 0, -3 * nxe - 2, -3 * nxe - 1, -3 * nxe, 1, 2, 1,
+                     0, -nxe - 1, -nxe, 1,
+                     0, 0, -3 * nxe - 2, -3 * nxe - 1, -3 * nxe, 1, 2, 1], dtype=it)
+    n = base[:, which]
+    n += step
+
+    # per element and axis: lower face, upper face, midpoint
+    def faces(lo, hi):
+        f = np.empty((nels, 3), dtype=np.float64)
+        f[:, 0] = lo
+        f[:, 1] = hi
+        f[:, 2] = 0.5 * (f[:, 0] + f[:, 1])
+        return f
+
+    fx = faces((ip - 1) * aa, ip * aa)
+    fy = faces((iq - 1) * bb, iq * bb)
+    fz = faces(-is_ * cc, -(is_ - 1) * cc)        # z decreases downwards
+    sx = np.array([0, 0, 0, 2, 1, 1, 1, 2, 0, 0, 1, 1, 0, 0, 0, 2, 1, 1, 1, 2])
+    sy = np.array([0] * 8 + [2] * 4 + [1] * 8)
+    sz = np.array([0, 2, 1, 1, 1, 2, 0, 0, 0, 1, 1, 0, 0, 2, 1, 1, 1, 2, 0, 0])
+    gc = np.empty((nels, 3, 20), dtype=np.float64)
+    gc[:, 0, :] = fx[:, sx]
+    gc[:, 1, :] = fy[:, sy]
+    gc[:, 2, :] = fz[:, sz]
     coords = np.zeros((nn, 3), dtype=np.float64)
-    coords[n.reshape(-1) - 1] = c.reshape(-1, 3)
-    return Mesh(20, n.astype(np.int32), coords, nxe, nye, nze, aa, bb, cc)
+    idx = n.reshape(-1) - 1
+    for k in range(3):
+        coords[idx, k] = gc[:, k, :].reshape(-1)
+    gc.flags.writeable = False
+    m = Mesh(20, n.astype(np.int32, copy=False), coords, nxe, nye, nze, aa, bb, cc)
+    m.extra["g_coord"] = gc
+    m.extra["g_coord_of"] = coords
+    return m
 
 
 def box_hex8(nxe: int, nye: int, nze: int, aa: float = 1.0, bb: float = 1.0,

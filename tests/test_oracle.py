@@ -419,6 +419,20 @@ def test_demo_deck_regenerated_exactly(golden):
     assert np.abs(val - np.array(meta["load_val"])[:, 2]).max() < 1e-7
 
 
+def test_box_hex20_kept_g_coord_is_the_gathered_one():
+    """The per-element coordinates the generator keeps are exactly
+    g_coord(g_num, :) of its nodal coordinates, and are dropped once the
+    coordinates are replaced."""
+    for dims, sc in (((3, 2, 4), (1.0, 1.0, 1.0)), ((5, 3, 2), (0.3, 1.7, 2.9))):
+        m = M.box_hex20(*dims, *sc)
+        kept = m.g_coord()
+        assert not kept.flags.writeable
+        assert np.array_equal(kept, M.element_coords(m.g_num, m.coords))
+        m.coords = m.coords * 2.0
+        assert np.array_equal(m.g_coord(), M.element_coords(m.g_num, m.coords))
+        assert not np.array_equal(m.g_coord(), kept)
+
+
 def _demo_problem(golden):
     meta = json.load(open(os.path.join(golden, "p121_demo.json")))
     g = meta["mg"]
