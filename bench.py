@@ -197,7 +197,7 @@ def block_bench(args):
     P.setup(ctx, pr, store="fp64")
     ctx.sync()
     shape = T.tune_block(ctx)
-    log(f"block: autotuned {shape['blk_waves']} elements per workgroup, {shape['blk_rows']} row tiles per pass "
+    log(f"block: autotuned {shape['blk_waves']} elements per workgroup "
         f"({'cache' if shape['cached'] else 'timed ' + str(shape['ms'])})")
     ctx.block_solve(F[:, :1], 1e-4, outer_limit=1)          # warm: packing, allocation, code paths
     t0 = time.perf_counter()
@@ -215,10 +215,10 @@ def block_bench(args):
     ebe_ms = tk["ebe"][0] / max(tk["ebe"][1], 1)
     log("block iteration kernels (ms per launch): " +
         ", ".join(f"{kk}={v[0] / max(v[1], 1):.4f}" for kk, v in tk.items() if v[1]))
-    btraffic, bsource = profiled_traffic("block", "k_blk_ebe<60, float, 4>") if n == 60 else (None, None)
+    btraffic, bsource = profiled_traffic("block", "k_blk_ebe_sym<60>") if n == 60 else (None, None)
     flops = 2.0 * pr.ntot * pr.ntot * k * pr.nels
     tflops = flops / (ebe_ms * 1e-3) / 1e12
-    kt_bytes = pr.nels * 4096 * 4
+    kt_bytes = pr.nels * 2560 * 4       # the 10 stored 16x16 tiles of the upper tile triangle
     ut_bytes = pr.nels * pr.ntot * k * 4
     log(f"FP32 MFMA element product: {ebe_ms:.3f} ms, {tflops:.1f} TFLOP/s useful, "
         f"{(kt_bytes + ut_bytes) / (ebe_ms * 1e-3) / 1e9:.0f} GB/s (K tiles + U_e)")
@@ -254,10 +254,10 @@ def block_bench(args):
             "fp64_sequential_s_estimated": t_seq, "fp64_sample_solves": len(t_one),
             "fp64_iterations": iters_one,
             "speedup_vs_fp64_sequential": round(t_seq / t_block, 3) if t_seq else None,
-            "tile_shape": {"blk_waves": shape["blk_waves"], "blk_rows": shape["blk_rows"], "source": "autotuner"},
+            "tile_shape": {"blk_waves": shape["blk_waves"], "source": "autotuner"},
         },
         "roofline": {
-            "bound": "mfma", "kernel": "k_blk_ebe<60, float> (v_mfma_f32_16x16x4f32)",
+            "bound": "mfma", "kernel": "k_blk_ebe_sym<60> (v_mfma_f32_16x16x4f32)",
             "achieved": round(tflops, 2), "peak": 157.3, "unit": "TFLOP/s", "frac": round(tflops / 157.3, 4),
             "traffic": btraffic, "traffic_source": f"profiles/{bsource}" if bsource else None,
             "ebe_ms": ebe_ms,

@@ -422,7 +422,8 @@ int pf_block_time_ebe(pf_ctx *ctx, int reps, double *ms);
  *              multiplies: 1 or 2 (ntot 60), 1, 2 or 4 (ntot 24), 1, 2, 4
  *              or 8 (ntot 8);
  *   blk_waves: block products, elements (wavefronts) per workgroup: 1, 2, 4;
- *   blk_rows:  FP32 block products, 16-row output tiles per pass: 1, 2, 4.
+ *   blk_rows:  FP64 block products, 16-row output tiles per pass: 1, 2, 4
+ *              (the FP32 block product forms every row tile in one pass).
  * Setting them drops a captured iteration chunk. */
 typedef struct {
     int ebe_epl;

@@ -10,7 +10,10 @@
 // Element matrices are stored pre-arranged as A operands: for tile row mt
 // and k-step quad ktq, lane l reads four consecutive values (k-steps
 // 4ktq..4ktq+3) -- one 16-byte (FP32) or 32-byte (FP64) load per lane,
-// 1 KB / 2 KB per wavefront, coalesced.  Block vectors keep column c at
+// 1 KB / 2 KB per wavefront, coalesced.  The FP32 matrices keep only the
+// 16x16 tiles on and above the diagonal (upper triangle, mirrored inside
+// the diagonal tiles): the product uses each off-diagonal tile as stored for
+// its own rows and, transposed through LDS, for the mirrored rows.  Block vectors keep column c at
 // position pf_blk_pos(c), so the B operand of lane l for all four column
 // tiles is one vector load from the gathered row.
 //
@@ -74,8 +77,8 @@ __global__ void k_blk_pack(const double *__restrict__ K, long kstride, T *__rest
     }
 }
 
-// U_e = K_e P_e for every element, all PF_BLK columns; MP output tile rows
-// per pass (the FP64 kernel makes two passes to bound its registers).
+// U_e = K_e P_e for every element, all PF_BLK columns, from full tiles (the
+// FP64 product of the true residuals); MP output tile rows per pass.
 template <int NT, typename T, int MP>
 __global__ __launch_bounds__(256) void k_blk_ebe(const T *__restrict__ Kt, long kstride, const int *__restrict__ gg,
                                                 const T *__restrict__ P, T *__restrict__ Ut, int nels,
@@ -129,6 +132,114 @@ __global__ __launch_bounds__(256) void k_blk_ebe(const T *__restrict__ Kt, long 
                                                 reinterpret_cast<T4 *>(Ut + (e * NT + row) * PF_BLK + 4 * cl));
             }
     }
+}
+
+// Symmetric FP32 tiles: the MT(MT+1)/2 tiles (a, b), a <= b, row-major over
+// the upper triangle of tiles; tile (a, b) lane l holds S[l%16][4j + l/16],
+// j = 0..3, with S = K[16a.., 16b..] and K(i, j) read from the upper
+// triangle (K(min, max)), so the stored operator is exactly symmetric.
+template <int NT> struct SymTiles {
+    static constexpr int MT = Tiles<NT>::MT;
+    static constexpr int NS = MT * (MT + 1) / 2;   // stored tiles
+    static constexpr int PER = NS * 256;           // floats per element
+};
+
+template <int NT>
+__global__ void k_blk_pack_sym(const double *__restrict__ K, long kstride, float *__restrict__ Ks, long nels)
+{
+    using ST = SymTiles<NT>;
+    const long n = nels * ST::PER;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const long e = q / ST::PER;
+        const int w = (int)(q - e * ST::PER);
+        const int j = w & 3, l = (w >> 2) & 63;
+        int t = w >> 8, a = 0;
+        while (t >= ST::MT - a) {                  // row a of the tile triangle holds MT - a tiles
+            t -= ST::MT - a;
+            ++a;
+        }
+        const int b = a + t;
+        const int row = 16 * a + (l & 15), col = 16 * b + 4 * j + (l >> 4);
+        const int lo = row < col ? row : col, hi = row < col ? col : row;
+        Ks[q] = (hi < NT) ? (float)K[pf_idx_pair(e * kstride, NT, lo, hi)] : 0.0f;
+    }
+}
+
+// U_e = K_e P_e in FP32 for every element, all PF_BLK columns, from the
+// symmetric tiles.  One wavefront per element loads its tiles, writes the
+// off-diagonal ones transposed into LDS, then walks the k-blocks in
+// ascending order: for k-block kb it gathers the B operands of those four
+// k-steps and adds into row tile r <= kb the stored tile (r, kb) and into
+// row tile r > kb the transposed tile (kb, r), so every output tile sums its
+// k-steps in ascending order.
+template <int NT>
+__global__ __launch_bounds__(256) void k_blk_ebe_sym(const float *__restrict__ Ks, long kstride,
+                                                    const int *__restrict__ gg, const float *__restrict__ P,
+                                                    float *__restrict__ Ut, int nels,
+                                                    const pf_blk_state *__restrict__ st)
+{
+    using TL = Tiles<NT>;
+    using ST = SymTiles<NT>;
+    constexpr int MT = TL::MT, KT = TL::KT, NS = ST::NS, NO = NS - MT;
+    __shared__ float tr[4][NO > 0 ? NO : 1][16][17];   // per wave, padded rows
+    if (st && st->done) return;
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long e = (long)blockIdx.x * (blockDim.x >> 6) + w;
+    if (e >= nels) return;                       // whole wavefronts only; no block barriers below
+    const int cl = l & 15, rq = l >> 4;
+    const int gl = l < NT ? gg[e * NT + l] : 0;
+    const f4_t *Ke = reinterpret_cast<const f4_t *>(Ks) + e * kstride * (NS * 64);
+    // tile (a, b), a <= b, sits at index a*MT - a(a-1)/2 + (b - a); off-diagonal
+    // tile (a, b) at LDS slot a*(MT-1) - a(a-1)/2 + (b - a - 1)
+    auto tix = [](int a, int b) { return a * MT - a * (a - 1) / 2 + (b - a); };
+    auto oix = [](int a, int b) { return a * (MT - 1) - a * (a - 1) / 2 + (b - a - 1); };
+    f4_t s[NS];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) s[t] = __builtin_nontemporal_load(Ke + t * 64 + l);
+    if constexpr (NO > 0) {
+#pragma unroll
+        for (int a = 0; a < MT; ++a)
+#pragma unroll
+            for (int bb = a + 1; bb < MT; ++bb)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) tr[w][oix(a, bb)][l & 15][4 * j + (l >> 4)] = s[tix(a, bb)][j];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    f4_t acc[MT][4];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[m][nt] = f4_t{0, 0, 0, 0};
+#pragma unroll
+    for (int kb = 0; kb < MT; ++kb) {
+        f4_t b[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = 16 * kb + 4 * j + rq;
+            const int g = __shfl(gl, row < NT ? row : 0, 64);
+            b[j] = row < NT ? *reinterpret_cast<const f4_t *>(P + (long)g * PF_BLK + 4 * cl) : f4_t{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int r = 0; r < MT; ++r)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (4 * kb + j >= KT) break;
+                const float av = r <= kb ? s[tix(r, kb)][j] : tr[w][oix(kb, r)][4 * j + (l >> 4)][l & 15];
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) acc[r][nt] = mfma(av, b[j][nt], acc[r][nt]);
+            }
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = 16 * m + 4 * rq + v;
+            if (row < NT)   // read back by the scatter only: non-temporal
+                __builtin_nontemporal_store(f4_t{acc[m][0][v], acc[m][1][v], acc[m][2][v], acc[m][3][v]},
+                                            reinterpret_cast<f4_t *>(Ut + (e * NT + row) * PF_BLK + 4 * cl));
+        }
 }
 
 // Per-column reduction of 16 rows x 4 waves of a block into partials[q][blk][pos].
@@ -224,7 +335,10 @@ __device__ void blk_fold(const double *partials, int G, double *out)
 
 __global__ __launch_bounds__(256) void k_blk_fold_alpha(const double *partials, int G, pf_blk_state *st)
 {
-    if (st->done) return;
+    if (st->done) {
+        if (threadIdx.x == 0) st->xlast = 0;   // the last p update has run
+        return;
+    }
     __shared__ double pu[PF_BLK];
     blk_fold(partials, G, pu);
     if (threadIdx.x < PF_BLK) {
@@ -234,10 +348,10 @@ __global__ __launch_bounds__(256) void k_blk_fold_alpha(const double *partials, 
     }
 }
 
-// x += alpha p, r -= alpha u; partials of (r, dinv r) and (r,r).  The
-// preconditioned residual d = dinv r is not stored: k_blk_pupdate forms it.
-__global__ __launch_bounds__(256) void k_blk_update(float *__restrict__ X, float *__restrict__ R,
-                                                   const float *__restrict__ P, const float *__restrict__ U,
+// r -= alpha u; partials of (r, dinv r) and (r,r).  The preconditioned
+// residual d = dinv r is not stored, and x += alpha p waits for
+// k_blk_pupdate, which reads p anyway: one vector pass fewer.
+__global__ __launch_bounds__(256) void k_blk_update(float *__restrict__ R, const float *__restrict__ U,
                                                    const float *__restrict__ dinv, int neq, double *partials,
                                                    const pf_blk_state *st)
 {
@@ -251,18 +365,16 @@ __global__ __launch_bounds__(256) void k_blk_update(float *__restrict__ X, float
     const int qe = ((int)blockIdx.x + 1) * qc < neq ? ((int)blockIdx.x + 1) * qc : neq;
     for (int q = (int)blockIdx.x * qc + rl; q < qe; q += 16) {
         const long o = (long)(q + 1) * PF_BLK + 4 * cl;
-        f4_t x = *reinterpret_cast<const f4_t *>(X + o), r = *reinterpret_cast<const f4_t *>(R + o);
-        const f4_t p = *reinterpret_cast<const f4_t *>(P + o), u = *reinterpret_cast<const f4_t *>(U + o);
+        f4_t r = *reinterpret_cast<const f4_t *>(R + o);
+        const f4_t u = *reinterpret_cast<const f4_t *>(U + o);
         const float di = dinv[q + 1];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            x[k] = __fmaf_rn(a[k], p[k], x[k]);
             r[k] = __fmaf_rn(-a[k], u[k], r[k]);
             const float d = di * r[k];
             rd[k] = __dadd_rn(rd[k], __dmul_rn((double)r[k], (double)d));
             rr[k] = __dadd_rn(rr[k], __dmul_rn((double)r[k], (double)r[k]));
         }
-        *reinterpret_cast<f4_t *>(X + o) = x;
         *reinterpret_cast<f4_t *>(R + o) = r;
     }
     blk_block_partials(rd, partials);
@@ -297,24 +409,37 @@ __global__ __launch_bounds__(256) void k_blk_fold_update(const double *partials,
     if (threadIdx.x == 0) {
         st->iters += 1;
         if (trig) st->trig = 1;
-        if (st->iters >= st->limit) st->done = 1;
+        if (st->iters >= st->limit) {
+            st->done = 1;
+            st->xlast = 1;
+        }
     }
 }
 
-// p = dinv r + beta p
-__global__ __launch_bounds__(256) void k_blk_pupdate(float *__restrict__ P, const float *__restrict__ R,
-                                                    const float *__restrict__ dinv, int neq, const pf_blk_state *st)
+// x += alpha p, then p = dinv r + beta p (only the x update on the
+// iteration that set done: the solve ends there)
+__global__ __launch_bounds__(256) void k_blk_pupdate(float *__restrict__ X, float *__restrict__ P,
+                                                    const float *__restrict__ R, const float *__restrict__ dinv,
+                                                    int neq, const pf_blk_state *st)
 {
-    if (st->done) return;
+    if (st->done && !st->xlast) return;
+    const bool last = st->done;
     const int cl = threadIdx.x & 15, rl = threadIdx.x >> 4;
-    float b[4];
+    float a[4], b[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) b[k] = (float)st->beta[4 * cl + k];
+    for (int k = 0; k < 4; ++k) {
+        a[k] = (float)st->alpha[4 * cl + k];
+        b[k] = (float)st->beta[4 * cl + k];
+    }
     const int qc = (neq + (int)gridDim.x - 1) / (int)gridDim.x;   // rows per block, one contiguous range
     const int qe = ((int)blockIdx.x + 1) * qc < neq ? ((int)blockIdx.x + 1) * qc : neq;
     for (int q = (int)blockIdx.x * qc + rl; q < qe; q += 16) {
         const long o = (long)(q + 1) * PF_BLK + 4 * cl;
-        f4_t p = *reinterpret_cast<const f4_t *>(P + o);
+        f4_t p = *reinterpret_cast<const f4_t *>(P + o), x = *reinterpret_cast<const f4_t *>(X + o);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = __fmaf_rn(a[k], p[k], x[k]);
+        *reinterpret_cast<f4_t *>(X + o) = x;
+        if (last) continue;
         const f4_t r = *reinterpret_cast<const f4_t *>(R + o);
         const float di = dinv[q + 1];
 #pragma unroll
@@ -466,6 +591,15 @@ int blk_tiles_per(int nt)
     }
 }
 
+int blk_sym_per(int nt)
+{
+    switch (nt) {
+    case 60: return SymTiles<60>::PER;
+    case 24: return SymTiles<24>::PER;
+    default: return SymTiles<8>::PER;
+    }
+}
+
 // Allocate (once per problem) and pack the element matrices.
 int blk_setup(pf_ctx *c, bool need32)
 {
@@ -488,12 +622,12 @@ int blk_setup(pf_ctx *c, bool need32)
         }
     }
     if (need32 && !b->Kt32) {
-        rc = blk_alloc(c, b->Kt32, nt);
+        rc = blk_alloc(c, b->Kt32, (size_t)ke * blk_sym_per(c->ntot));
         if (rc) return rc;
         switch (c->ntot) {
-        case 60: hipLaunchKernelGGL((k_blk_pack<60, float>), dim3(4096), dim3(256), 0, c->stream, c->K64, ks, b->Kt32, ke); break;
-        case 24: hipLaunchKernelGGL((k_blk_pack<24, float>), dim3(4096), dim3(256), 0, c->stream, c->K64, ks, b->Kt32, ke); break;
-        default: hipLaunchKernelGGL((k_blk_pack<8, float>), dim3(4096), dim3(256), 0, c->stream, c->K64, ks, b->Kt32, ke); break;
+        case 60: hipLaunchKernelGGL(k_blk_pack_sym<60>, dim3(4096), dim3(256), 0, c->stream, c->K64, ks, b->Kt32, ke); break;
+        case 24: hipLaunchKernelGGL(k_blk_pack_sym<24>, dim3(4096), dim3(256), 0, c->stream, c->K64, ks, b->Kt32, ke); break;
+        default: hipLaunchKernelGGL(k_blk_pack_sym<8>, dim3(4096), dim3(256), 0, c->stream, c->K64, ks, b->Kt32, ke); break;
         }
     }
     PF_HIP(hipGetLastError());
@@ -509,7 +643,6 @@ int blk_setup(pf_ctx *c, bool need32)
         rc = blk_alloc(c, b->P, nv);
         if (!rc) rc = blk_alloc(c, b->U, nv);
         if (!rc) rc = blk_alloc(c, b->R, nv);
-        if (!rc) rc = blk_alloc(c, b->D, nv);
         if (!rc) rc = blk_alloc(c, b->X, nv);
         if (!rc) rc = blk_alloc(c, b->dinv, (size_t)c->neq + 1);
         if (!rc) rc = blk_alloc(c, b->Ut32, ns);
@@ -526,33 +659,41 @@ int blk_setup(pf_ctx *c, bool need32)
     return rc;
 }
 
-template <typename T, int MP>
-void blk_launch_mp(pf_ctx *c, int waves, const T *Kt, const T *P, T *Ut, const pf_blk_state *st)
+template <int MP>
+void blk_launch_mp(pf_ctx *c, int waves, const double *Kt, const double *P, double *Ut, const pf_blk_state *st)
 {
     const int grid = (c->nels + waves - 1) / waves;
     const long ks = pf_k64_estride(c);
     const dim3 g(grid), b(64 * waves);
     switch (c->ntot) {
-    case 60: hipLaunchKernelGGL((k_blk_ebe<60, T, MP>), g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
-    case 24: hipLaunchKernelGGL((k_blk_ebe<24, T, MP>), g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
-    default: hipLaunchKernelGGL((k_blk_ebe<8, T, MP>), g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
+    case 60: hipLaunchKernelGGL((k_blk_ebe<60, double, MP>), g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
+    case 24: hipLaunchKernelGGL((k_blk_ebe<24, double, MP>), g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
+    default: hipLaunchKernelGGL((k_blk_ebe<8, double, MP>), g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
     }
 }
 
 // The element product in the context's launch shape: elements per workgroup
-// (tune_blk_waves, default 4) and, for FP32, 16-row output tiles per pass
-// (tune_blk_rows, default 4); the FP64 product makes passes of 2 tiles.
+// (tune_blk_waves, default 4) and, for the FP64 product, 16-row output
+// tiles per pass (tune_blk_rows, default 2); the FP32 product forms every
+// row tile in one pass over the symmetric tiles.
 template <typename T>
 int blk_launch_ebe(pf_ctx *c, const T *Kt, const T *P, T *Ut, const pf_blk_state *st)
 {
     const int waves = c->tune_blk_waves ? c->tune_blk_waves : 4;
     if constexpr (sizeof(T) == 8) {
-        blk_launch_mp<T, 2>(c, waves, Kt, P, Ut, st);
+        const int rows = c->tune_blk_rows ? c->tune_blk_rows : 2;
+        if (rows == 1) blk_launch_mp<1>(c, waves, Kt, P, Ut, st);
+        else if (rows == 2) blk_launch_mp<2>(c, waves, Kt, P, Ut, st);
+        else blk_launch_mp<4>(c, waves, Kt, P, Ut, st);
     } else {
-        const int rows = c->tune_blk_rows ? c->tune_blk_rows : 4;
-        if (rows == 1) blk_launch_mp<T, 1>(c, waves, Kt, P, Ut, st);
-        else if (rows == 2) blk_launch_mp<T, 2>(c, waves, Kt, P, Ut, st);
-        else blk_launch_mp<T, 4>(c, waves, Kt, P, Ut, st);
+        const int grid = (c->nels + waves - 1) / waves;
+        const long ks = pf_k64_estride(c);
+        const dim3 g(grid), b(64 * waves);
+        switch (c->ntot) {
+        case 60: hipLaunchKernelGGL(k_blk_ebe_sym<60>, g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
+        case 24: hipLaunchKernelGGL(k_blk_ebe_sym<24>, g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
+        default: hipLaunchKernelGGL(k_blk_ebe_sym<8>, g, b, 0, c->stream, Kt, ks, c->g_g, P, Ut, c->nels, st); break;
+        }
     }
     PF_HIP(hipGetLastError());
     return PF_OK;
@@ -587,14 +728,15 @@ void pf_block_free(pf_ctx *c)
     if (!b) return;
     const size_t nv = (size_t)(c->neq + 1) * PF_BLK;
     const size_t ns = (size_t)c->nels * c->ntot * PF_BLK;
-    const size_t nt = (size_t)(c->k_shared ? 1 : c->nels) * blk_tiles_per(c->ntot);
+    const size_t ke = (size_t)(c->k_shared ? 1 : c->nels);
+    const size_t nt = ke * blk_tiles_per(c->ntot);
     (void)hipStreamSynchronize(c->stream);
     auto fr = [&](auto *&p, size_t n) {
         if (p) pf_free(c, p, n * sizeof(*p));
         p = nullptr;
     };
-    fr(b->Kt32, nt); fr(b->Kt64, nt);
-    fr(b->P, nv); fr(b->U, nv); fr(b->R, nv); fr(b->D, nv); fr(b->X, nv);
+    fr(b->Kt32, ke * blk_sym_per(c->ntot)); fr(b->Kt64, nt);
+    fr(b->P, nv); fr(b->U, nv); fr(b->R, nv); fr(b->X, nv);
     fr(b->dinv, (size_t)c->neq + 1);
     fr(b->Ut32, ns); fr(b->Ut64, ns);
     fr(b->X64, nv); fr(b->R64, nv); fr(b->F64, nv); fr(b->U64, nv);
@@ -739,6 +881,7 @@ int pf_block_solve(pf_ctx *c, int k, const double *f, double *x, const pf_block_
         s0.tol2 = o->tol * o->tol;
         s0.delta2 = delta * delta;
         s0.trig = 0;
+        s0.xlast = 0;
         *b->st_host = s0;
         PF_HIP(hipMemcpyAsync(b->st, b->st_host, sizeof(pf_blk_state), hipMemcpyHostToDevice, c->stream));
         first = false;
@@ -760,14 +903,14 @@ int pf_block_solve(pf_ctx *c, int k, const double *f, double *x, const pf_block_
                 if (tm) PF_HIP(hipEventRecord(t1, c->stream));
                 tm = pf_timing_begin(c, PF_K_UPDATE, &t0, &t1);
                 if (tm) PF_HIP(hipEventRecord(t0, c->stream));
-                hipLaunchKernelGGL(k_blk_update, dim3(G), dim3(256), 0, c->stream, b->X, b->R, b->P, b->U, b->dinv,
-                                   c->neq, b->partials, b->st);
+                hipLaunchKernelGGL(k_blk_update, dim3(G), dim3(256), 0, c->stream, b->R, b->U, b->dinv, c->neq,
+                                   b->partials, b->st);
                 hipLaunchKernelGGL(k_blk_fold_update, dim3(1), dim3(256), 0, c->stream, b->partials, G, b->st);
                 if (tm) PF_HIP(hipEventRecord(t1, c->stream));
                 tm = pf_timing_begin(c, PF_K_PUPDATE, &t0, &t1);
                 if (tm) PF_HIP(hipEventRecord(t0, c->stream));
-                hipLaunchKernelGGL(k_blk_pupdate, dim3(G), dim3(256), 0, c->stream, b->P, b->R, b->dinv, c->neq,
-                                   b->st);
+                hipLaunchKernelGGL(k_blk_pupdate, dim3(G), dim3(256), 0, c->stream, b->X, b->P, b->R, b->dinv,
+                                   c->neq, b->st);
                 if (tm) PF_HIP(hipEventRecord(t1, c->stream));
             }
             PF_HIP(hipGetLastError());

@@ -102,13 +102,14 @@ struct pf_blk_state {
     double ff[PF_BLK];       // ||f||^2
     int frozen[PF_BLK];      // converged (true residual) or unused column
     int iters, limit, done, trig;   // trig: some column asks for a replacement
+    int xlast;               // done was set this iteration: its p update still adds alpha p into x
     double tol2, delta2;
 };
 
 struct pf_block {
-    float *Kt32 = nullptr;     // element matrices as FP32 MFMA A-operand tiles (nels * 4096)
+    float *Kt32 = nullptr;     // element matrices as FP32 MFMA A-operand tiles, upper tile triangle (nels * 2560 for hex20)
     double *Kt64 = nullptr;    // the same in FP64
-    float *P = nullptr, *U = nullptr, *R = nullptr, *D = nullptr, *X = nullptr;   // FP32 inner vectors
+    float *P = nullptr, *U = nullptr, *R = nullptr, *X = nullptr;   // FP32 inner vectors
     float *dinv = nullptr;     // (neq+1) Jacobi, FP32
     float *Ut32 = nullptr;     // (nels*ntot) x PF_BLK element products, FP32
     double *Ut64 = nullptr;    // the same in FP64 (true residual)
@@ -203,7 +204,7 @@ struct pf_ctx {
     // Launch shapes (pf_tune_set); 0 = the built-in choice.
     int tune_epl = 0;                  // single-RHS products: elements per lane group
     int tune_blk_waves = 0;            // block products: elements per workgroup
-    int tune_blk_rows = 0;             // block FP32 products: 16-row tiles per pass
+    int tune_blk_rows = 0;             // block FP64 products: 16-row tiles per pass
 
     // Optional per-kernel timing of pf_pcg_run: one event pair per launch.
     bool timing = false;

@@ -5,9 +5,11 @@ The shapes (C ABI ``pf_tune``):
 * single-RHS product ``k_ebe``: ``ebe_epl``, the number of elements a lane
   group loads before it multiplies (more bytes in flight per wavefront
   against more registers and fewer resident wavefronts);
-* block product ``k_blk_ebe``: ``blk_waves``, elements (one wavefront each)
-  per workgroup, and ``blk_rows``, 16-row output tiles per pass over the
-  gathered block (more accumulators against more re-reads of the block).
+* block products ``k_blk_ebe_sym`` (FP32) and ``k_blk_ebe`` (FP64):
+  ``blk_waves``, elements (one wavefront each) per workgroup, tuned on the
+  FP32 product; ``blk_rows``, 16-row output tiles per pass of the FP64
+  product (more accumulators against more re-reads of the gathered block;
+  the FP32 product forms all row tiles in one pass), left at its default.
 
 No shape changes a result: every lane forms its row in the same order
 whatever the shape, so the tuner only measures time.  Each candidate is
@@ -27,7 +29,9 @@ import os
 import numpy as np
 
 EPL_CANDIDATES = {60: (1, 2), 24: (1, 2, 4), 8: (1, 2, 4, 8)}
-BLOCK_CANDIDATES = tuple((w, r) for w in (1, 2, 4) for r in (1, 2, 4))
+BLOCK_WAVES = (1, 2, 4)
+# every block shape the library accepts (waves x FP64 rows per pass)
+BLOCK_SHAPES = tuple((w, r) for w in BLOCK_WAVES for r in (1, 2, 4))
 
 _memory: dict[str, dict] = {}
 
@@ -119,25 +123,24 @@ def tune_ebe(ctx, r0, precision: str, reps: int = 20, warm: int = 3, use_cache: 
 
 
 def tune_block(ctx, reps: int = 10, use_cache: bool = True) -> dict:
-    """Pick ``blk_waves`` and ``blk_rows`` for the FP32 block element
-    product on the problem loaded in `ctx` (FP64 element matrices), set them
-    and return {"blk_waves", "blk_rows", "ms", "cached"}."""
-    key = _key(ctx, "k_blk_ebe", "fp32", 64)
+    """Pick ``blk_waves`` for the FP32 block element product on the problem
+    loaded in `ctx` (FP64 element matrices), set it (``blk_rows`` keeps the
+    context's value) and return {"blk_waves", "ms", "cached"}."""
+    key = _key(ctx, "k_blk_ebe_sym", "fp32", 64)
     cur = ctx.tune_get()
     hit = lookup(key) if use_cache else None
     if hit is not None:
-        ctx.tune_set(cur["ebe_epl"], hit["blk_waves"], hit["blk_rows"])
+        ctx.tune_set(cur["ebe_epl"], hit["blk_waves"], cur["blk_rows"])
         return {**hit, "cached": True}
     # block buffers, FP32 tiles and a block of operands with full mantissas
     # (the matrix cores draw less power, and clock higher, on zeros)
     ctx.block_apply(np.random.default_rng(0).uniform(-1.0, 1.0, (ctx.neq, 64)), "mixed")
     times = {}
-    for w, r in BLOCK_CANDIDATES:
-        ctx.tune_set(cur["ebe_epl"], w, r)
-        times[(w, r)] = ctx.block_time_ebe(reps)
-    w, r = pick(times)
-    ctx.tune_set(cur["ebe_epl"], w, r)
-    entry = {"blk_waves": int(w), "blk_rows": int(r),
-             "ms": {f"{a}x{b}": round(t, 5) for (a, b), t in times.items()}}
+    for w in BLOCK_WAVES:
+        ctx.tune_set(cur["ebe_epl"], w, cur["blk_rows"])
+        times[w] = ctx.block_time_ebe(reps)
+    w = pick(times)
+    ctx.tune_set(cur["ebe_epl"], w, cur["blk_rows"])
+    entry = {"blk_waves": int(w), "ms": {str(a): round(t, 5) for a, t in times.items()}}
     _store(key, entry)
     return {**entry, "cached": False}

@@ -719,10 +719,10 @@ def test_launch_shapes_same_bits(ctx, case):
 
 
 def test_block_shapes_and_autotuner(ctx, tmp_path, monkeypatch):
-    """Every block launch shape (elements per workgroup x row tiles per
+    """Every block launch shape (elements per workgroup x FP64 row tiles per
     pass) gives the same bits for the FP32 and FP64 block products; the
-    autotuner picks one of them, caches it, and a cached lookup sets the
-    same shape without timing."""
+    autotuner picks the FP32 product's elements per workgroup, caches it,
+    and a cached lookup sets the same shape without timing."""
     from parafem_amd import tune as T
     monkeypatch.setenv("PARAFEM_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
     monkeypatch.setattr(T, "_memory", {})
@@ -735,7 +735,7 @@ def test_block_shapes_and_autotuner(ctx, tmp_path, monkeypatch):
         ctx.storkm_set(Ko)
         X = np.random.default_rng(5).normal(size=(pr.neq, 64))
         ref32 = ref64 = None
-        for w, rows in T.BLOCK_CANDIDATES:
+        for w, rows in T.BLOCK_SHAPES:
             ctx.tune_set(blk_waves=w, blk_rows=rows)
             Y32, Y64 = ctx.block_apply(X, "mixed"), ctx.block_apply(X, "fp64")
             if ref32 is None:
@@ -743,12 +743,12 @@ def test_block_shapes_and_autotuner(ctx, tmp_path, monkeypatch):
             assert np.array_equal(Y32, ref32) and np.array_equal(Y64, ref64), (w, rows)
         ctx.tune_set()
         got = T.tune_block(ctx, reps=3)
-        assert not got["cached"] and (got["blk_waves"], got["blk_rows"]) in T.BLOCK_CANDIDATES
+        assert not got["cached"] and got["blk_waves"] in T.BLOCK_WAVES
         assert ctx.tune_get()["blk_waves"] == got["blk_waves"]
         monkeypatch.setattr(T, "_memory", {})
         ctx.tune_set()
         again = T.tune_block(ctx)
-        assert again["cached"] and ctx.tune_get()["blk_rows"] == got["blk_rows"]
+        assert again["cached"] and ctx.tune_get()["blk_waves"] == got["blk_waves"]
         r = P.start_r(ctx, pr)
         e = T.tune_ebe(ctx, r, "fp64", reps=3, warm=1)
         assert e["ebe_epl"] in T.EPL_CANDIDATES[60] and ctx.tune_get()["ebe_epl"] == e["ebe_epl"]

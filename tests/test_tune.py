@@ -10,8 +10,9 @@ from parafem_amd import tune as T
 def test_candidates_match_the_library_limits():
     # pf_tune_set accepts powers of two up to 2 (ntot 60), 4 (ntot 24), 8 (ntot 8)
     assert T.EPL_CANDIDATES == {60: (1, 2), 24: (1, 2, 4), 8: (1, 2, 4, 8)}
-    assert len(T.BLOCK_CANDIDATES) == 9
-    assert all(w in (1, 2, 4) and r in (1, 2, 4) for w, r in T.BLOCK_CANDIDATES)
+    assert T.BLOCK_WAVES == (1, 2, 4)
+    assert len(T.BLOCK_SHAPES) == 9
+    assert all(w in (1, 2, 4) and r in (1, 2, 4) for w, r in T.BLOCK_SHAPES)
 
 
 def test_pick_prefers_the_earlier_of_near_ties():
@@ -46,8 +47,7 @@ class _Ctx:
 
     def block_time_ebe(self, reps):
         self.block_calls += 1
-        w, r = self.shape["blk_waves"], self.shape["blk_rows"]
-        return 1.0 + abs(w - 2) * 0.1 + abs(r - 4) * 0.05     # fastest at 2 x 4
+        return 1.0 + abs(self.shape["blk_waves"] - 2) * 0.1     # fastest at 2
 
 
 def test_block_tuning_and_cache(tmp_path, monkeypatch):
@@ -55,12 +55,13 @@ def test_block_tuning_and_cache(tmp_path, monkeypatch):
     monkeypatch.setenv("PARAFEM_AMD_TUNE_CACHE", str(path))
     monkeypatch.setattr(T, "_memory", {})
     c = _Ctx()
+    c.shape["blk_rows"] = 1                       # the FP64 passes are left as they are
     got = T.tune_block(c)
-    assert (got["blk_waves"], got["blk_rows"]) == (2, 4) and not got["cached"]
-    assert c.shape["blk_waves"] == 2 and c.block_calls == 9
+    assert got["blk_waves"] == 2 and not got["cached"]
+    assert c.shape["blk_waves"] == 2 and c.shape["blk_rows"] == 1 and c.block_calls == 3
     data = json.loads(path.read_text())
     (key, entry), = data.items()
-    assert "k_blk_ebe" in key and entry["blk_rows"] == 4
+    assert "k_blk_ebe_sym" in key and entry["blk_waves"] == 2 and "blk_rows" not in entry
     monkeypatch.setattr(T, "_memory", {})
     c2 = _Ctx()
     again = T.tune_block(c2)
