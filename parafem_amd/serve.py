@@ -21,7 +21,9 @@ deterministic, a solve's blocks do the work of its own launch and its state
 is its own, so a result is bit-identical alone and in the mix.
 
 Stream generator (``make_stream``): numpy ``default_rng(seed)``; arrival gaps
-exponential with the given rate (the first request arrives at 0); n =
+exponential with the given rate (unit-rate draws scaled by 1/rate; all zero
+for rate 0, the closed loop, which therefore holds the same requests; the
+first request arrives at 0); n =
 round(exp(U(ln nmin, ln nmax))); E ~ U[1,200]; nu ~ U[0.2,0.4]; pressure ~
 U[0.1,10].  The load is pressure times the unit-pressure consistent nodal
 load of the top face.
@@ -67,7 +69,10 @@ class Result:
 
 def make_stream(count: int, rate: float, seed: int = 4, nmin: int = 10, nmax: int = 80) -> list[Request]:
     rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate, count) if rate > 0 else np.zeros(count)
+    # unit-rate gaps are drawn whatever the rate, so one seed gives the same
+    # requests closed-loop (rate 0) and at any arrival rate
+    unit = rng.exponential(1.0, count)
+    gaps = unit / rate if rate > 0 else np.zeros(count)
     arrival = np.concatenate([[0.0], np.cumsum(gaps[1:])])
     n = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), count))).astype(int)
     e = rng.uniform(1.0, 200.0, count)
