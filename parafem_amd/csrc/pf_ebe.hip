@@ -955,6 +955,32 @@ struct pf_bslot {
     int nels, neq, G, fix_zero, rowmajor;
 };
 
+// A pointer read from memory (the table) is generic to the compiler, which
+// then emits flat loads and stores -- counted against the LDS wait counter
+// too, so every LDS exchange of the element product would also wait for the
+// matrix loads in flight.  Re-forming each one as a global-address-space
+// pointer (through its integer value: a direct round trip of casts folds
+// back to the generic pointer) lets them be global loads and stores, as with
+// kernel arguments.
+template <typename P>
+__device__ __forceinline__ P *as_global(P *p)
+{
+    return (P *)(__attribute__((address_space(1))) P *)(unsigned long)p;
+}
+
+__device__ __forceinline__ pf_bslot slot_of(const pf_bslot *__restrict__ T, int s)
+{
+    const pf_bslot &R = T[s];
+    pf_bslot S = R;
+    S.K64 = as_global(R.K64); S.Khi = as_global(R.Khi); S.Klo = as_global(R.Klo);
+    S.g_g = as_global(R.g_g); S.csr_ptr = as_global(R.csr_ptr); S.slot = as_global(R.slot);
+    S.x = as_global(R.x); S.r = as_global(R.r); S.p = as_global(R.p); S.d = as_global(R.d);
+    S.u = as_global(R.u); S.dinv = as_global(R.dinv); S.f = as_global(R.f); S.utemp = as_global(R.utemp);
+    S.p32 = as_global(R.p32); S.utemp32 = as_global(R.utemp32); S.fixstore = as_global(R.fixstore);
+    S.partials = as_global(R.partials); S.st = as_global(R.st);
+    return S;
+}
+
 // The solve owning block b: the last solve whose first block is <= b (at
 // most 64 solves, one lane each).
 template <typename I>
@@ -975,7 +1001,7 @@ __device__ __forceinline__ void bs_ebe(const pf_bslot *__restrict__ T, const lon
 {
     constexpr int EPL = Lanes<NT, KIND>::EPL;
     const int s = bslot_find(est, ns, gb);
-    const pf_bslot &S = T[s];
+    const pf_bslot S = slot_of(T, s);
     if (skip(S.st, gate)) return;
     const long lb = gb - est[s];
     if constexpr (REF)
@@ -997,7 +1023,7 @@ __device__ __forceinline__ void bs_ebe_refine(const pf_bslot *__restrict__ T, co
                                               long total, long b0, long nb)
 {
     for (int s = 0; s < ns; ++s) {
-        if (skip(T[s].st, RUN_REFINE)) continue;
+        if (skip(as_global(T[s].st), RUN_REFINE)) continue;
         const long end = s + 1 < ns ? est[s + 1] : total;
         for (long gb = est[s] + b0; gb < end; gb += nb) {
             bs_ebe<NT, KIND_ACC, double, true>(T, est, ns, gb, RUN_REFINE);
@@ -1011,7 +1037,7 @@ __device__ __forceinline__ void bs_scatter_pu(const pf_bslot *__restrict__ T, co
                                               int b)
 {
     const int s = bslot_find(qst, ns, b);
-    const pf_bslot &S = T[s];
+    const pf_bslot S = slot_of(T, s);
     if (skip(S.st, RUN_LIVE)) return;
     const UT *ut;
     if constexpr (sizeof(UT) == 4) ut = (const UT *)S.utemp32;
@@ -1024,7 +1050,7 @@ template <bool CHECON>
 __device__ __forceinline__ void bs_update(const pf_bslot *__restrict__ T, const int *__restrict__ qst, int ns, int b)
 {
     const int s = bslot_find(qst, ns, b);
-    const pf_bslot &S = T[s];
+    const pf_bslot S = slot_of(T, s);
     if (skip(S.st, RUN_LIVE)) return;
     update_block<CHECON>(b - qst[s], S.G, S.x, S.r, S.d, S.p, S.u, S.dinv, S.neq, S.partials, S.st);
 }
@@ -1033,7 +1059,7 @@ __device__ __forceinline__ void bs_scatter_res(const pf_bslot *__restrict__ T, c
                                                int b)
 {
     const int s = bslot_find(qst, ns, b);
-    const pf_bslot &S = T[s];
+    const pf_bslot S = slot_of(T, s);
     if (skip(S.st, RUN_REFINE)) return;
     scatter_res_block(b - qst[s], S.G, S.csr_ptr, S.slot, S.utemp, S.fixstore, S.x, S.f, S.r, S.d, S.dinv, S.neq,
                       S.partials);
@@ -1042,7 +1068,7 @@ __device__ __forceinline__ void bs_scatter_res(const pf_bslot *__restrict__ T, c
 __device__ __forceinline__ void bs_pupdate(const pf_bslot *__restrict__ T, const int *__restrict__ qst, int ns, int b)
 {
     const int s = bslot_find(qst, ns, b);
-    const pf_bslot &S = T[s];
+    const pf_bslot S = slot_of(T, s);
     if (skip(S.st, RUN_LIVE)) return;
     pupdate_block(b - qst[s], S.G, S.p, S.p32, S.d, S.neq, S.st);
 }
@@ -1052,7 +1078,7 @@ __device__ __forceinline__ void bs_pupdate(const pf_bslot *__restrict__ T, const
 template <int WHICH>
 __device__ __forceinline__ void bs_fold(const pf_bslot *__restrict__ T, int s)
 {
-    const pf_bslot &S = T[s];
+    const pf_bslot S = slot_of(T, s);
     if (skip(S.st, WHICH == 3 ? RUN_REFINE : RUN_LIVE)) return;
     if constexpr (WHICH == 0) fold_alpha_block(S.partials, S.G, S.st);
     else if constexpr (WHICH == 1) fold_update_block<true>(S.partials, S.G, S.st);
@@ -1168,7 +1194,7 @@ __global__ __launch_bounds__(PF_BLOCK) void kb_persistent(const pf_bslot *__rest
     int it = 0;
     for (; it < max_iters; ++it) {
         bool any = false;
-        for (int s = 0; s < ns; ++s) any |= !T[s].st->frozen;   // the same in every workgroup
+        for (int s = 0; s < ns; ++s) any |= !as_global(T[s].st)->frozen;   // the same in every workgroup
         if (!any) break;
         for (long r0 = 0; r0 < total_eb; r0 += nb) {
             const long gb = r0 + lbx;
@@ -1186,7 +1212,7 @@ __global__ __launch_bounds__(PF_BLOCK) void kb_persistent(const pf_bslot *__rest
         if (!grid_barrier(bar, nb)) return;
         if constexpr (REF) {
             bool due = false;
-            for (int s = 0; s < ns; ++s) due |= !skip(T[s].st, RUN_REFINE);
+            for (int s = 0; s < ns; ++s) due |= !skip(as_global(T[s].st), RUN_REFINE);
             if (due) {
                 bs_ebe_refine<NT>(T, est + PF_BATCH_MAX, ns, total_ebr, blockIdx.x, nb);
                 if (!grid_barrier(bar, nb)) return;
@@ -1206,7 +1232,7 @@ __global__ __launch_bounds__(PF_BLOCK) void kb_persistent(const pf_bslot *__rest
 __global__ void kb_snapshot(const pf_bslot *__restrict__ T, int ns, pf_state *__restrict__ out)
 {
     const int s = blockIdx.x;
-    if (s < ns && threadIdx.x == 0) out[s] = *T[s].st;
+    if (s < ns && threadIdx.x == 0) out[s] = *as_global(T[s].st);
 }
 
 }  // namespace
