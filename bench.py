@@ -308,13 +308,19 @@ def serve_sample(requests: int, nmin: int, nmax: int, precision: str, slots: int
 def serve_bench(args):
     """The serving loop on a request stream: closed-loop capacity, then the
     stream at a fixed Poisson rate through the mixed-precision server, then
-    the same stream one request at a time on the plain FP64 path."""
+    the same stream one request at a time on the plain FP64 path (each phase
+    can be left out; the FP64 phase can take the first N requests only)."""
     from parafem_amd import serve as S
-    cache = S.CubeCache(keep=64)
+    phases = set(args.phases.split(","))
+    if not phases <= {"closed", "stream", "fp64"} or not phases:
+        raise SystemExit("--phases: a comma-separated subset of closed,stream,fp64")
+    if "stream" in phases and "closed" not in phases and args.rate <= 0:
+        raise SystemExit("the stream phase without the closed loop needs --rate")
+    cache = S.CubeCache(keep=80)
     closed = S.make_stream(args.requests, 0.0, seed=4, nmin=args.nmin, nmax=args.nmax)
     for r in closed:
         cache.get(r.n)
-    log(f"serve: {args.requests} requests, n in [{args.nmin},{args.nmax}], sizes cached")
+    log(f"serve: {args.requests} requests, n in [{args.nmin},{args.nmax}], sizes cached, phases {sorted(phases)}")
     done = [0]
 
     def progress(msg):
@@ -323,41 +329,52 @@ def serve_bench(args):
         if done[0] % 25 == 0:
             log(f"{done[0]} finished; last: {msg}")
 
-    srv = S.Server(slots=args.slots, cache=cache, precision=args.serve_precision, chunk=args.chunk,
-                   persistent_elems=args.persistent_elems)
-    try:
-        # warm-up: compile/capture paths once
-        srv.serve(S.make_stream(args.slots, 0.0, seed=5, nmin=args.nmin, nmax=args.nmin + 2))
+    cap = mix = seq = None
+    rate = args.rate
+    if phases & {"closed", "stream"}:
+        srv = S.Server(slots=args.slots, cache=cache, precision=args.serve_precision, chunk=args.chunk,
+                       persistent_elems=args.persistent_elems)
+        try:
+            # warm-up: compile/capture paths once
+            srv.serve(S.make_stream(args.slots, 0.0, seed=5, nmin=args.nmin, nmax=args.nmin + 2))
+            if "closed" in phases:
+                done[0] = 0
+                t0 = time.perf_counter()
+                res_closed = srv.serve(closed, log=progress)
+                cap = S.summarize(res_closed, time.perf_counter() - t0)
+                log(f"closed loop: {cap}")
+                log(f"server host time: {srv.stats}")
+                if rate <= 0:
+                    rate = 0.8 * cap["solves_per_s"]
+            if "stream" in phases:
+                stream = S.make_stream(args.requests, rate, seed=4, nmin=args.nmin, nmax=args.nmax)
+                done[0] = 0
+                t0 = time.perf_counter()
+                res_mix = srv.serve(stream, log=progress)
+                mix = S.summarize(res_mix, time.perf_counter() - t0)
+                log(f"stream ({args.serve_precision} server, {args.slots} slots, {rate:.3f}/s): {mix}")
+        finally:
+            srv.close()
+    if "fp64" in phases:
+        n64 = args.fp64_requests if args.fp64_requests > 0 else args.requests
+        # the same requests (a seed gives the same ones at every rate), at
+        # the stream's arrival times when there is a rate
+        part = S.make_stream(args.requests, rate if rate > 0 else 0.0, seed=4, nmin=args.nmin,
+                             nmax=args.nmax)[:n64]
+        done[0] = 0
         t0 = time.perf_counter()
-        res_closed = srv.serve(closed, log=progress)
-        cap = S.summarize(res_closed, time.perf_counter() - t0)
-        log(f"closed loop: {cap}")
-        log(f"server host time: {srv.stats}")
-        if args.closed_only:
-            print(json.dumps({"closed_loop": cap, "host": srv.stats, "precision": args.serve_precision,
-                              "slots": args.slots,
-                              "requests": args.requests, "nmin": args.nmin, "nmax": args.nmax}), flush=True)
-            return
-        rate = args.rate if args.rate > 0 else 0.8 * cap["solves_per_s"]
-        stream = S.make_stream(args.requests, rate, seed=4, nmin=args.nmin, nmax=args.nmax)
-        t0 = time.perf_counter()
-        res_mix = srv.serve(stream, log=progress)
-        mix = S.summarize(res_mix, time.perf_counter() - t0)
-        log(f"stream ({args.serve_precision} server, {args.slots} slots): {mix}")
-    finally:
-        srv.close()
-    t0 = time.perf_counter()
-    res_seq = S.serve_sequential(stream, precision="fp64", cache=cache, log=progress)
-    seq = S.summarize(res_seq, time.perf_counter() - t0)
-    log(f"stream (FP64 one by one): {seq}")
+        res_seq = S.serve_sequential(part, precision="fp64", cache=cache, log=progress)
+        seq = S.summarize(res_seq, time.perf_counter() - t0)
+        log(f"FP64 one by one ({len(part)} requests): {seq}")
+    head = mix or cap or seq
     out = {
         "metric": METRIC,
-        "value": round(mix["solves_per_s"], 3),
+        "value": round(head["solves_per_s"], 3),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.requests,
         "warmup": args.slots,
-        "ms_per_step": round(1e3 / mix["solves_per_s"], 3),
+        "ms_per_step": round(1e3 / head["solves_per_s"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -366,13 +383,19 @@ def serve_bench(args):
         "config": {
             "workload": f"serving stream: {args.requests} hex20 elastic cube requests, n log-uniform in "
                         f"[{args.nmin},{args.nmax}], E U[1,200], nu U[0.2,0.4], pressure U[0.1,10] (seed 4), "
-                        f"Poisson arrivals at {rate:.2f}/s, {args.serve_precision} PCG (FP32-stored element "
-                        f"matrices, FP64 vectors, reliable updates) to relres {1e-10}, {args.slots} slots",
-            "rate_per_s": rate,
+                        + (f"Poisson arrivals at {rate:.2f}/s, " if rate > 0 else "closed loop, ")
+                        + f"{args.serve_precision} PCG (FP32-stored element matrices, FP64 vectors, reliable "
+                        f"updates) to relres {1e-10}, {args.slots} slots",
+            "phases": sorted(phases),
+            "value_from": "stream" if mix else ("closed" if cap else "fp64"),
+            "rate_per_s": rate if rate > 0 else None,
             "closed_loop": cap,
             "server": mix,
             "fp64_one_by_one": seq,
-            "p99_speedup_vs_fp64": round(seq["p99_latency_s"] / mix["p99_latency_s"], 3),
+            "capacity_vs_fp64_one_by_one": round(cap["solves_per_s"] / seq["solves_per_s"], 3)
+            if cap and seq else None,
+            "p99_speedup_vs_fp64": round(seq["p99_latency_s"] / mix["p99_latency_s"], 3)
+            if mix and seq and seq["requests"] == mix["requests"] else None,
         },
         "roofline": None,
         "cpu_baseline": None,
@@ -400,8 +423,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=16, help="serve: iterations per batch run")
     ap.add_argument("--persistent-elems", type=int, default=0,
                     help="serve: batches of at most this many elements run as persistent kernels")
-    ap.add_argument("--closed-only", action="store_true",
-                    help="serve: only the closed-loop capacity of the server (no timed stream, no FP64 baseline)")
+    ap.add_argument("--phases", default="closed,stream,fp64",
+                    help="serve: which of closed (closed-loop capacity), stream (Poisson arrivals) and fp64 "
+                         "(the same stream one by one on the FP64 path) to run")
+    ap.add_argument("--fp64-requests", type=int, default=0,
+                    help="serve: FP64 one-by-one phase on the first N requests only (0 = all)")
     ap.add_argument("--serve-sample", type=int, default=48,
                     help="hex20: requests of the reduced serving stream reported beside the iteration rate "
                          "(0 = skip)")
