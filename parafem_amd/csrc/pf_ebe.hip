@@ -714,10 +714,20 @@ __global__ __launch_bounds__(64) void k_seq_reduce(pf_seq_args A, long n, int G,
 // ---------------------------------------------------------- primitives
 typedef unsigned u4_t __attribute__((ext_vector_type(4)));
 
+// Device copy in 16-byte words: four loads in flight per lane before their
+// stores, then the remainder one word at a time.
 __global__ __launch_bounds__(256) void k_copy(const u4_t *__restrict__ src, u4_t *__restrict__ dst, long n16)
 {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const long stride = (long)gridDim.x * blockDim.x;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const u4_t v0 = src[i], v1 = src[i + stride], v2 = src[i + 2 * stride], v3 = src[i + 3 * stride];
+        dst[i] = v0;
+        dst[i + stride] = v1;
+        dst[i + 2 * stride] = v2;
+        dst[i + 3 * stride] = v3;
+    }
+    for (; i < n16; i += stride) dst[i] = src[i];
 }
 
 // Whole-array vector statements: 0 fill, 1 w = x + y*a, 2 z = x*y, 3 y = 1/x.

@@ -273,7 +273,10 @@ class Server:
                         break
                     s = free.pop()
                     s.req = waiting.popleft()
-                    s.bytes = need
+                    # while the worker builds the problem the context still
+                    # holds its cache of freed blocks, which the new buffers
+                    # may not reuse: count both
+                    s.bytes = s.ctx.mem_used() + need
                     with_worker.add(id(s))
                     jobs.put(("admit", s, s.req, t0))
                     busy_worker += 1
@@ -320,6 +323,7 @@ class Server:
                     # solve is frozen on the device; anything else still runs
                     if r.converged or r.status == 6 or r.iters >= self.limit:
                         active.remove(s)
+                        s.bytes = s.ctx.mem_used()       # freeing keeps the blocks in its cache
                         with_worker.add(id(s))
                         jobs.put(("retire", s, r, t0))
                         busy_worker += 1

@@ -345,6 +345,22 @@ def test_dot_and_sum_long_vectors_bitexact(ctx):
     db.free()
 
 
+def test_device_copy_long_and_ragged(ctx):
+    """The device copy with every lane through its four-words-in-flight loop
+    and then its one-word tail (8192 x 256 lanes of 16-byte words: 4 x 2^21
+    words and 12,345 more), and a copy shorter than one pass."""
+    rng = np.random.default_rng(13)
+    for nwords in (4 * 2**21 + 12345, 1001):
+        a = rng.normal(size=2 * nwords)          # two doubles per 16-byte word
+        say(f"device copy words={nwords}")
+        da = DeviceArray.from_host(ctx, a)
+        db = DeviceArray.from_host(ctx, np.zeros_like(a))
+        ctx.dev_copy(db, da)
+        assert np.array_equal(db.download(), a)
+        da.free()
+        db.free()
+
+
 def test_run_to_run_bitwise(ctx):
     pr = _distorted(4, seed=5)
     say("determinism")

@@ -33,3 +33,122 @@ def test_device_bytes_estimate():
     assert S.device_bytes(80, "mixed64") > 8 * S.device_bytes(40, "mixed64") * 0.9
     nels = 80 ** 3
     assert S.device_bytes(80, "fp64") > nels * 28800
+
+
+class _Res:
+    def __init__(self, iters, converged):
+        self.iters, self.converged, self.status, self.refinements = iters, converged, 0, 0
+
+
+class _Device:
+    """Device memory of the mock contexts: what each holds live and in its
+    cache of freed blocks (never reused, the worst case), against a fixed
+    capacity."""
+
+    def __init__(self, capacity):
+        import threading
+        self.capacity, self.lock, self.peak, self.ctxs = capacity, threading.Lock(), 0, []
+
+    def total(self):
+        return sum(c.live + c.cache for c in self.ctxs)
+
+
+def _mock_classes(dev):
+    import itertools
+
+    ids = itertools.count(1)
+
+    class Ctx:
+        def __init__(self, device=0):
+            self.h, self.neq, self.it, self.target, self.live, self.cache, self.n = next(ids), 0, 0, None, 0, 0, 0
+            dev.ctxs.append(self)
+
+        def mesh_set(self, g):
+            pass
+
+        def steer_build(self, g_num, rest, nn):
+            self.neq = nn
+            return nn, None
+
+        def assemble_elastic(self, e, v, ee, store="fp64"):
+            with dev.lock:
+                need = S.device_bytes(self.n, "mixed64")
+                if dev.total() + need > dev.capacity:
+                    raise MemoryError("device out of memory")
+                self.live = need
+                dev.peak = max(dev.peak, dev.total())
+
+        def pcg_begin(self, r, limit, tol, **kw):
+            self.it, self.target = 0, 20 + 3 * self.n
+
+        def pcg_x(self):
+            return np.ones(self.neq)
+
+        def reset(self):
+            with dev.lock:
+                self.cache += self.live
+                self.live, self.target = 0, None
+
+        def trim(self):
+            with dev.lock:
+                self.cache = 0
+
+        def mem_used(self):
+            return self.live + self.cache
+
+        def close(self):
+            pass
+
+    class Batch:
+        def __init__(self, device=0):
+            self._last, self._snap = [], None
+
+        def close(self):
+            pass
+
+        def run(self, ctxs, n, graph=True, persistent=False):
+            for c in ctxs:
+                assert c.target is not None, "a slot in the batch that is not set up"
+                c.it = min(c.target, c.it + n)
+            self._last = list(ctxs)
+
+        def query(self):
+            if self._snap is None:
+                self._snap = (self._last, [_Res(c.it, c.it >= c.target) for c in self._last])
+                return None
+            snap, self._snap = self._snap, None
+            return snap
+
+    class Cache(S.CubeCache):
+        def get(self, n):
+            return {"g_coord": None, "g_num": None, "rest": None, "nn": n, "r_unit": np.ones(n)}
+
+    return Ctx, Batch, Cache
+
+
+def test_admission_counts_cached_blocks_of_slots_being_set_up(monkeypatch):
+    """A slot handed to the set-up thread still holds its cache of freed
+    blocks while the new problem is allocated: admission must count both, so
+    the device never overflows, whatever order the sizes come in (mock
+    contexts whose freed blocks are never reused)."""
+    budget = 3 * S.device_bytes(80, "mixed64")
+    dev = _Device(budget)
+    Ctx, Batch, Cache = _mock_classes(dev)
+    monkeypatch.setattr(S, "Context", Ctx)
+    monkeypatch.setattr(S, "Batch", Batch)
+    reqs = S.make_stream(120, 0.0, seed=9, nmin=10, nmax=80)
+    for r in reqs:
+        r.n = int(r.n)
+    srv = S.Server(slots=8, hbm_budget=budget, chunk=16, cache=Cache())
+
+    orig_admit = S.Server._admit
+
+    def admit(self, s, req):
+        s.ctx.n = req.n
+        orig_admit(self, s, req)
+
+    monkeypatch.setattr(S.Server, "_admit", admit)
+    out = srv.serve(reqs)
+    assert sorted(r.rid for r in out) == list(range(120))
+    assert dev.peak <= budget
+    assert srv.stats["trims"] > 0          # the budget was reached and caches handed back
