@@ -189,15 +189,22 @@ class Server:
     def _fits(self, slot: _Slot, need: int, busy: bool, idle: list[_Slot], with_worker: set) -> bool:
         """Whether a request needing `need` bytes can start on `slot`:
         within the HBM budget after handing back the cached blocks of the
-        slot, then of the other idle slots, if needed.  With nothing running
-        a request is always admitted."""
-        if self._device_bytes(with_worker) + need <= self.hbm_budget:
+        slot, then of the other idle slots, if needed (an idle context holds
+        only cached blocks; nothing is handed back when that cannot make
+        room).  With nothing running a request is always admitted."""
+        used = self._device_bytes(with_worker)
+        if used + need <= self.hbm_budget:
             return True
-        for s in [slot] + [t for t in idle if t is not slot]:
-            s.ctx.trim()
-            self.stats["trims"] += 1
-            if self._device_bytes(with_worker) + need <= self.hbm_budget:
-                return True
+        order = [slot] + [t for t in idle if t is not slot]
+        held = [s.ctx.mem_used() for s in order]
+        if used - sum(held) + need <= self.hbm_budget:
+            for s, h in zip(order, held):
+                if not h:
+                    continue
+                s.ctx.trim()
+                self.stats["trims"] += 1
+                if self._device_bytes(with_worker) + need <= self.hbm_budget:
+                    return True
         return not busy
 
     def _admit(self, s: _Slot, req: Request):
