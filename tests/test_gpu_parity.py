@@ -346,9 +346,10 @@ def test_dot_and_sum_long_vectors_bitexact(ctx):
 
 
 def test_device_copy_long_and_ragged(ctx):
-    """The device copy with every lane through its four-words-in-flight loop
-    and then its one-word tail (8192 x 256 lanes of 16-byte words: 4 x 2^21
-    words and 12,345 more), and a copy shorter than one pass."""
+    """The device copy over many 16384-word chunks per workgroup, each lane
+    through its four-words-in-flight loop and then its one-word tail, the
+    last chunk partial (4 x 2^21 words and 12,345 more), and a copy shorter
+    than one chunk."""
     rng = np.random.default_rng(13)
     for nwords in (4 * 2**21 + 12345, 1001):
         a = rng.normal(size=2 * nwords)          # two doubles per 16-byte word
