@@ -295,6 +295,13 @@ int pf_batch_run_graph(pf_batch *batch, int nctx, pf_ctx *const *ctxs, int n_ite
  *                        pf_batch_query returns PF_ERR_HIP. */
 int pf_batch_run_persistent(pf_batch *batch, int nctx, pf_ctx *const *ctxs, int max_iters);
 int pf_batch_query(pf_batch *batch, pf_pcg_result *res, int *ready);
+/*   pf_batch_solo_set : solves of at least min_elems elements (0: none, the
+ *                        default) run their own iteration launches -- those
+ *                        of pf_pcg_run -- in the batch's stream and graph
+ *                        instead of a share of the batched launches (a large
+ *                        solve fills the device by itself); the same bits
+ *                        either way.  pf_batch_run_persistent ignores it. */
+int pf_batch_solo_set(pf_batch *batch, long min_elems);
 /* y = K x over the assembled operator in FP64 (host vectors, neq), with the
  * p123 fixed-freedom rows applied.  Used to report true residuals. */
 int pf_apply(pf_ctx *ctx, const double *x_host, double *y_host);

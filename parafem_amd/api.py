@@ -165,6 +165,7 @@ def lib():
         "pf_batch_run_graph": (i32, [vp, i32, C.POINTER(vp), i32]),
         "pf_batch_run_persistent": (i32, [vp, i32, C.POINTER(vp), i32]),
         "pf_batch_query": (i32, [vp, vp, pi]),
+        "pf_batch_solo_set": (i32, [vp, C.c_long]),
         "pf_dot_order_get": (i32, [vp, pi]),
         "pf_scatter_mode_set": (i32, [vp, i32]),
         "pf_scatter_mode_get": (i32, [vp, pi]),
@@ -733,6 +734,11 @@ class Batch:
     def _handles(ctxs):
         arr = (C.c_void_p * len(ctxs))(*[c.h for c in ctxs])
         return arr
+
+    def solo_set(self, min_elems: int):
+        """Solves of at least min_elems elements (0: none) run their own
+        iteration launches inside the batch's runs (pf_batch_solo_set)."""
+        _check(lib().pf_batch_solo_set(self.h, int(min_elems)), "pf_batch_solo_set")
 
     def run(self, ctxs, n: int, graph: bool = True, persistent: bool = False):
         """Enqueue n iterations of every context in ctxs (no host sync):

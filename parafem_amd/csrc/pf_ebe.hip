@@ -970,6 +970,7 @@ struct pf_bslot {
     pf_state *st;
     long kstride;
     int nels, neq, G, fix_zero, rowmajor;
+    int solo;   // runs its own launches inside the batch's run: no blocks here
 };
 
 // A pointer read from memory (the table) is generic to the compiler, which
@@ -1096,7 +1097,7 @@ template <int WHICH>
 __device__ __forceinline__ void bs_fold(const pf_bslot *__restrict__ T, int s)
 {
     const pf_bslot S = slot_of(T, s);
-    if (skip(S.st, WHICH == 3 ? RUN_REFINE : RUN_LIVE)) return;
+    if (S.solo || skip(S.st, WHICH == 3 ? RUN_REFINE : RUN_LIVE)) return;
     if constexpr (WHICH == 0) fold_alpha_block(S.partials, S.G, S.st);
     else if constexpr (WHICH == 1) fold_update_block<true>(S.partials, S.G, S.st);
     else if constexpr (WHICH == 2) fold_update_block<false>(S.partials, S.G, S.st);
@@ -1857,6 +1858,13 @@ struct pf_batch {
     pf_gbar *bar = nullptr, *bar_host = nullptr;
     int cus = 0, coop = 0;
     bool persistent_used = false;
+    // solves of at least solo_elems elements (0: none) run their own
+    // launches, in the batch's stream and graph, instead of a share of the
+    // batched ones: a large solve fills the device by itself, and its own
+    // kernels skip the slot lookup the batched ones do per workgroup
+    long solo_elems = 0;
+    std::vector<pf_ctx *> solo;
+    bool stale = false;   // the next run rebuilds the table (and drops the graph)
 };
 
 static const size_t kBatchTable = sizeof(pf_bslot) * PF_BATCH_MAX + 2 * sizeof(long) * PF_BATCH_MAX +
@@ -1873,7 +1881,7 @@ static void batch_graph_drop(pf_batch *b)
 
 // Builds the table of the given contexts, uploads it when it changed and
 // orders the batch stream after each context's stream.
-static int batch_prepare(pf_batch *b, int nctx, pf_ctx *const *ctxs)
+static int batch_prepare(pf_batch *b, int nctx, pf_ctx *const *ctxs, bool allow_solo = true)
 {
     if (!b || nctx < 1 || nctx > PF_BATCH_MAX || !ctxs) return PF_ERR_ARG;
     const pf_ctx *c0 = ctxs[0];
@@ -1915,8 +1923,9 @@ static int batch_prepare(pf_batch *b, int nctx, pf_ctx *const *ctxs)
         S.G = pf_grid_for(c->neq);
         S.fix_zero = c->fix_zero ? 1 : 0;
         S.rowmajor = rm;
+        S.solo = allow_solo && b->solo_elems > 0 && c->nels >= b->solo_elems ? 1 : 0;
         qst[k] = qb;
-        qb += S.G;
+        if (!S.solo) qb += S.G;
         if (mixed && !c->utemp32) return PF_ERR_STATE;
     }
     // elements per logical block of the iteration's product kind and of the
@@ -1937,14 +1946,20 @@ static int batch_prepare(pf_batch *b, int nctx, pf_ctx *const *ctxs)
     default: return PF_ERR_ARG;
     }
     long ebr = 0;
+    std::vector<pf_ctx *> solo;
     for (int k = 0; k < nctx; ++k) {
         est[k] = eb;
         est[nctx + k] = ebr;
+        if (T[k].solo) {
+            solo.push_back(ctxs[k]);
+            continue;
+        }
         eb += (T[k].nels + EPB - 1) / EPB;
         ebr += (T[k].nels + EPBR - 1) / EPBR;
     }
     PF_HIP(hipSetDevice(b->device));
-    const bool same = nctx == b->ns && std::memcmp(T.data(), b->cur.data(), sizeof(pf_bslot) * nctx) == 0 &&
+    const bool same = !b->stale && nctx == b->ns &&
+                      std::memcmp(T.data(), b->cur.data(), sizeof(pf_bslot) * nctx) == 0 &&
                       est == b->cur_est && qst == b->cur_qst;
     if (!same) {
         PF_HIP(hipEventSynchronize(b->ev_table));   // the staging buffer is free again
@@ -1954,9 +1969,14 @@ static int batch_prepare(pf_batch *b, int nctx, pf_ctx *const *ctxs)
         std::memcpy(b->qst_host, qst.data(), sizeof(int) * nctx);
         PF_HIP(hipMemcpyAsync(b->T, b->T_host, kBatchTable, hipMemcpyHostToDevice, b->stream));
         PF_HIP(hipEventRecord(b->ev_table, b->stream));
+        // a captured graph reads the batched solves from the table, but
+        // holds the solo solves' launches with their own buffers
         if (eb != b->total_eb || ebr != b->total_ebr || qb != b->total_qb || nctx != b->ns || c0->ntot != b->ntot ||
-            c0->opts.precision != b->precision || c0->opts.stop_mode != b->stop)
+            c0->opts.precision != b->precision || c0->opts.stop_mode != b->stop || !solo.empty() ||
+            !b->solo.empty() || b->stale)
             batch_graph_drop(b);
+        b->solo = solo;
+        b->stale = false;
         b->cur = T;
         b->cur_est = est;
         b->cur_qst = qst;
@@ -2081,10 +2101,21 @@ static int batch_persistent_nt(pf_batch *b, int max_iters)
 static int batch_iterations(pf_batch *b, int n)
 {
     for (int it = 0; it < n; ++it) {
-        switch (b->ntot) {
-        case 60: batch_iteration_nt<60>(b); break;
-        case 24: batch_iteration_nt<24>(b); break;
-        default: batch_iteration_nt<8>(b); break;
+        // the solo solves: their own iteration (pf_pcg_run's launches) in
+        // the batch's stream; no other work of theirs runs meanwhile
+        for (pf_ctx *c : b->solo) {
+            hipStream_t own = c->stream;
+            c->stream = b->stream;
+            const int rc = pf_pcg_run(c, 1);
+            c->stream = own;
+            if (rc) return rc;
+        }
+        if (b->total_eb > 0) {
+            switch (b->ntot) {
+            case 60: batch_iteration_nt<60>(b); break;
+            case 24: batch_iteration_nt<24>(b); break;
+            default: batch_iteration_nt<8>(b); break;
+            }
         }
         PF_HIP(hipGetLastError());
     }
@@ -2164,6 +2195,16 @@ int pf_batch_destroy(pf_batch *b)
     return PF_OK;
 }
 
+int pf_batch_solo_set(pf_batch *b, long min_elems)
+{
+    if (!b || min_elems < 0) return PF_ERR_ARG;
+    if (min_elems != b->solo_elems) {
+        b->solo_elems = min_elems;
+        b->stale = true;         // the next run rebuilds the table and its graph
+    }
+    return PF_OK;
+}
+
 int pf_batch_run(pf_batch *b, int nctx, pf_ctx *const *ctxs, int n_iters)
 {
     if (!b || n_iters < 0) return PF_ERR_ARG;
@@ -2206,7 +2247,7 @@ int pf_batch_run_persistent(pf_batch *b, int nctx, pf_ctx *const *ctxs, int max_
 {
     if (!b || max_iters < 1) return PF_ERR_ARG;
     if (!b->coop) return PF_ERR_ARG;   // the device cannot guarantee a resident grid
-    int rc = batch_prepare(b, nctx, ctxs);
+    int rc = batch_prepare(b, nctx, ctxs, false);   // every solve in the one kernel
     if (rc) return rc;
     switch (b->ntot) {
     case 60: rc = batch_persistent_nt<60>(b, max_iters); break;

@@ -163,12 +163,19 @@ class Server:
     # persistent cooperative kernel (pf_batch_run_persistent) instead of a
     # replayed graph of launches: small solves are launch-bound
     persistent_elems: int = 0
+    # solves of at least this many elements run their own launches inside
+    # the batch's runs (pf_batch_solo_set; 0: every solve shares the batched
+    # launches): a large solve fills the device alone, and its own kernels
+    # skip the per-workgroup slot lookup.  Off by default: on the [10, 80]
+    # stream it changed the capacity by less than the run-to-run spread.
+    solo_elems: int = 0
 
     def __post_init__(self):
         if not 1 <= self.slots <= BATCH_MAX:
             raise ValueError(f"1..{BATCH_MAX} slots")
         self._slots = [_Slot(Context(self.device)) for _ in range(self.slots)]
         self._batch = Batch(self.device)
+        self._batch.solo_set(self.solo_elems)
         # host-side time of the last serve() on the worker thread: set-up of
         # requests, read-backs of finished solves; cache trims (each waits
         # for the device) and batch cycles on the driving thread

@@ -28,13 +28,15 @@ def _elastic_mix():
     return out
 
 
-def _run_batch(ctxs, rs, limit, tol, stop, precision, graph, chunk=7):
+def _run_batch(ctxs, rs, limit, tol, stop, precision, graph, chunk=7, solo=0):
     """graph: True (replayed graph), False (direct launches) or
-    "persistent" (one cooperative kernel per chunk)."""
+    "persistent" (one cooperative kernel per chunk); solo: solves of at
+    least that many elements run their own launches in the batch."""
     for c, r in zip(ctxs, rs):
         c.pcg_begin(r, limit, tol, stop=stop, precision=precision)
     b = Batch(0)
     try:
+        b.solo_set(solo)
         live = list(range(len(ctxs)))
         done = {}
         while live:
@@ -108,6 +110,33 @@ def test_batch_equals_alone_hex8_laplace_penalty(precision, graph):
         got = _run_batch(ctxs, rs, 3000, 1e-10, "relres", precision, graph, chunk=5)
         for (xa, ra), (ib, cb, xb) in zip(alone, got):
             assert ra.converged and cb and ra.iters == ib
+            assert np.array_equal(xa, xb)
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+@pytest.mark.parametrize("precision,stop,graph,solo", [("mixed64", "relres", True, 60), ("fp64", "checon", False, 60),
+                                                       ("mixed", "relres", True, 1), ("split", "relres", True, 100)])
+def test_batch_with_solo_solves_equals_alone(precision, stop, graph, solo):
+    """Solves above the solo threshold run their own launches inside the
+    batch's runs (all of them for threshold 1), the rest share the batched
+    launches: every solve still bit-identical to itself alone."""
+    probs = _elastic_mix()
+    store = "fp64" if precision == "fp64" else "fp32"
+    tol = 1e-7 if stop == "checon" else 1e-10
+    ctxs = [Context(0) for _ in probs]
+    try:
+        rs = [P.setup(c, pr, store=store) for c, pr in zip(ctxs, probs)]
+        say(f"batch solo>={solo} {precision} {stop} graph={graph}: alone")
+        alone = []
+        for c, r in zip(ctxs, rs):
+            x, res = c.pcg_solve(r, 3000, tol, stop=stop, precision=precision)
+            alone.append((res.iters, res.converged, x))
+        say(f"batch solo>={solo} {precision} {stop} graph={graph}: batched")
+        got = _run_batch(ctxs, rs, 3000, tol, stop, precision, graph, solo=solo)
+        for (ia, ca, xa), (ib, cb, xb) in zip(alone, got):
+            assert ca and cb and ia == ib
             assert np.array_equal(xa, xb)
     finally:
         for c in ctxs:

@@ -571,12 +571,14 @@ def test_serving_mix_equals_alone():
 def test_serving_fp64_matches_oracle():
     """The serving loop on the FP64 path: every served request (its own n,
     E, nu and pressure) equals the oracle's p121 PCG in the device's
-    reduction order bit for bit, with the same iteration count."""
+    reduction order bit for bit, with the same iteration count -- the cubes
+    of 64 elements and more running their own launches inside the batch
+    (solo), the smaller ones sharing the batched launches."""
     from parafem_amd import serve as S
     reqs = S.make_stream(6, rate=0.0, seed=4, nmin=2, nmax=5)
     say(f"serving fp64 vs oracle sizes={[r.n for r in reqs]}")
     cache = S.CubeCache()
-    srv = S.Server(slots=3, chunk=8, keep_x=True, precision="fp64", cache=cache)
+    srv = S.Server(slots=3, chunk=8, keep_x=True, precision="fp64", cache=cache, solo_elems=64)
     try:
         got = srv.serve(reqs)
     finally:
