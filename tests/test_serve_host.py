@@ -106,6 +106,9 @@ def _mock_classes(dev):
         def close(self):
             pass
 
+        def solo_set(self, min_elems):
+            pass
+
         def run(self, ctxs, n, graph=True, persistent=False):
             for c in ctxs:
                 assert c.target is not None, "a slot in the batch that is not set up"
