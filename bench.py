@@ -333,7 +333,7 @@ def serve_bench(args):
     rate = args.rate
     if phases & {"closed", "stream"}:
         srv = S.Server(slots=args.slots, cache=cache, precision=args.serve_precision, chunk=args.chunk,
-                       persistent_elems=args.persistent_elems, solo_elems=args.solo_elems)
+                       persistent_elems=args.persistent_elems, solo_elems=args.solo_elems, lanes=args.lanes)
         try:
             # warm-up: compile/capture paths once
             srv.serve(S.make_stream(args.slots, 0.0, seed=5, nmin=args.nmin, nmax=args.nmin + 2))
@@ -421,6 +421,8 @@ def main():
                     help="serve: Poisson arrival rate (0 = 80%% of the measured closed-loop capacity)")
     ap.add_argument("--slots", type=int, default=32, help="serve: concurrent solves (one batch)")
     ap.add_argument("--chunk", type=int, default=16, help="serve: iterations per batch run")
+    ap.add_argument("--lanes", type=int, default=1,
+                    help="serve: independent batches on their own streams")
     ap.add_argument("--solo-elems", type=int, default=0,
                     help="serve: solves of at least this many elements run their own launches inside the batch "
                          "(0: none)")

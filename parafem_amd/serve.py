@@ -169,20 +169,29 @@ class Server:
     # skip the per-workgroup slot lookup.  Off by default: on the [10, 80]
     # stream it changed the capacity by less than the run-to-run spread.
     solo_elems: int = 0
+    # independent batches, each on its own stream, the admitted solves dealt
+    # to the one with the fewest elements (one batch measured fastest: two
+    # or three lost 2-12% of the closed-loop capacity on [10, 80] and
+    # [10, 40] streams)
+    lanes: int = 1
 
     def __post_init__(self):
         if not 1 <= self.slots <= BATCH_MAX:
             raise ValueError(f"1..{BATCH_MAX} slots")
         self._slots = [_Slot(Context(self.device)) for _ in range(self.slots)]
-        self._batch = Batch(self.device)
-        self._batch.solo_set(self.solo_elems)
+        if self.lanes < 1:
+            raise ValueError("at least one lane")
+        self._batches = [Batch(self.device) for _ in range(self.lanes)]
+        for b in self._batches:
+            b.solo_set(self.solo_elems)
         # host-side time of the last serve() on the worker thread: set-up of
         # requests, read-backs of finished solves; cache trims (each waits
         # for the device) and batch cycles on the driving thread
         self.stats = {"admit_s": 0.0, "readback_s": 0.0, "trims": 0, "cycles": 0}
 
     def close(self):
-        self._batch.close()
+        for b in self._batches:
+            b.close()
         for s in self._slots:
             s.ctx.close()
 
@@ -253,37 +262,43 @@ class Server:
     def serve(self, requests: list[Request], log=None) -> list[Result]:
         """Run the stream in real time (arrival times relative to the start).
 
-        The driving thread enqueues a chunk of iterations for the current
-        slots, takes a state snapshot behind it and enqueues one more chunk,
-        so the GPU keeps working while it waits for the snapshot; when it
-        arrives, finished solves leave the batch (the worker reads them back
-        and frees their buffers), requests the worker has set up join it,
-        and the next cycle runs the new set."""
+        For each batch the driving thread enqueues a chunk of iterations for
+        its current slots, takes a state snapshot behind it and enqueues one
+        more chunk, so the GPU keeps working while it waits for the
+        snapshot; when it arrives, finished solves leave the batch (the
+        worker reads them back and frees their buffers), requests the worker
+        has set up join the batch with the fewest elements, and the next
+        cycle runs the new set."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         waiting: deque[Request] = deque()
         free = list(self._slots)
-        active: list[_Slot] = []
-        ready: list[_Slot] = []          # set up by the worker, not yet in the batch
+        nb = len(self._batches)
+        active: list[list[_Slot]] = [[] for _ in range(nb)]
+        ready: list[_Slot] = []          # set up by the worker, not yet in a batch
         busy_worker = 0                  # jobs handed to the worker and not back yet
         with_worker: set = set()         # id() of the slots the worker holds
         out: list[Result] = []
         by_ctx = {s.ctx.h: s for s in self._slots}
-        in_flight = False
+        in_flight = [False] * nb
         self.stats = {"admit_s": 0.0, "readback_s": 0.0, "trims": 0, "cycles": 0}
         jobs: queue_mod.Queue = queue_mod.Queue()
         results: queue_mod.Queue = queue_mod.Queue()
         worker = threading.Thread(target=self._worker, args=(jobs, results), daemon=True)
         worker.start()
         t0 = time.perf_counter()
+
+        def elems(lane):
+            return sum(s.req.n ** 3 for s in active[lane])
+
         try:
-            while pending or waiting or active or ready or busy_worker:
+            while pending or waiting or any(active) or ready or busy_worker:
                 now = time.perf_counter() - t0
                 while pending and pending[0].arrival <= now:
                     waiting.append(pending.popleft())
                 # hand arrived requests to the worker while slots and memory allow
                 while waiting and free:
                     need = device_bytes(waiting[0].n, self.precision)
-                    if not self._fits(free[-1], need, bool(active or ready or busy_worker), free, with_worker):
+                    if not self._fits(free[-1], need, bool(any(active) or ready or busy_worker), free, with_worker):
                         break
                     s = free.pop()
                     s.req = waiting.popleft()
@@ -312,35 +327,46 @@ class Server:
                             log(f"request {res.rid} n={res.n} iters={res.iters} latency={res.latency:.3f}s")
                         s.req = None
                         free.append(s)
-                if not in_flight and (active or ready):
-                    active.extend(ready)
-                    ready = []
-                    ctxs = [s.ctx for s in active]
-                    pers = sum(s.req.n ** 3 for s in active) <= self.persistent_elems
-                    self._batch.run(ctxs, self.chunk, persistent=pers)
-                    self._batch.query()              # snapshot behind this chunk
-                    self._batch.run(ctxs, self.chunk, persistent=pers)   # and one more chunk queued
-                    in_flight = True
+                # set-up solves join the idle batches, the smallest first
+                idle = [k for k in range(nb) if not in_flight[k]]
+                while ready and idle:
+                    k = min(idle, key=elems)
+                    active[k].append(ready.pop(0))
+                progressed = False
+                for k in range(nb):
+                    if in_flight[k] or not active[k]:
+                        continue
+                    ctxs = [s.ctx for s in active[k]]
+                    pers = elems(k) <= self.persistent_elems
+                    b = self._batches[k]
+                    b.run(ctxs, self.chunk, persistent=pers)
+                    b.query()                        # snapshot behind this chunk
+                    b.run(ctxs, self.chunk, persistent=pers)   # and one more chunk queued
+                    in_flight[k] = True
                     self.stats["cycles"] += 1
-                got = self._batch.query() if in_flight else None
-                if got is None:
-                    if not in_flight and not busy_worker and pending and not waiting:
+                for k in range(nb):
+                    got = self._batches[k].query() if in_flight[k] else None
+                    if got is None:
+                        continue
+                    progressed = True
+                    in_flight[k] = False
+                    snap, res_list = got
+                    for ctx, r in zip(snap, res_list):
+                        s = by_ctx[ctx.h]
+                        # converged, broken down (status 6) or at the limit:
+                        # the solve is frozen on the device; anything else
+                        # still runs
+                        if r.converged or r.status == 6 or r.iters >= self.limit:
+                            active[k].remove(s)
+                            s.bytes = s.ctx.mem_used()       # freeing keeps the blocks in its cache
+                            with_worker.add(id(s))
+                            jobs.put(("retire", s, r, t0))
+                            busy_worker += 1
+                if not progressed:
+                    if not any(in_flight) and not busy_worker and pending and not waiting:
                         time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
                     else:
                         time.sleep(20e-6)
-                    continue
-                in_flight = False
-                snap, res_list = got
-                for ctx, r in zip(snap, res_list):
-                    s = by_ctx[ctx.h]
-                    # converged, broken down (status 6) or at the limit: the
-                    # solve is frozen on the device; anything else still runs
-                    if r.converged or r.status == 6 or r.iters >= self.limit:
-                        active.remove(s)
-                        s.bytes = s.ctx.mem_used()       # freeing keeps the blocks in its cache
-                        with_worker.add(id(s))
-                        jobs.put(("retire", s, r, t0))
-                        busy_worker += 1
         finally:
             jobs.put(None)
             worker.join()

@@ -547,15 +547,16 @@ def test_graph_replay_matches_eager(ctx):
             assert np.array_equal(ctx.pcg_x(), eager[k][0]), (precision, k)
 
 
-def test_serving_mix_equals_alone():
-    """Continuous batching: requests solved as slots of one batch (joining
-    and leaving it as slots free up) give bit-identical displacements and
-    iteration counts to the same requests solved alone; the FP64 one-by-one
-    path agrees within 1e-8."""
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_serving_mix_equals_alone(lanes):
+    """Continuous batching: requests solved as slots of one batch, or of two
+    batches on their own streams (joining and leaving them as slots free
+    up), give bit-identical displacements and iteration counts to the same
+    requests solved alone; the FP64 one-by-one path agrees within 1e-8."""
     from parafem_amd import serve as S
     reqs = S.make_stream(10, rate=0.0, seed=4, nmin=3, nmax=9)
-    say(f"serving mix sizes={[r.n for r in reqs]}")
-    srv = S.Server(slots=4, chunk=16, keep_x=True)
+    say(f"serving mix lanes={lanes} sizes={[r.n for r in reqs]}")
+    srv = S.Server(slots=4, chunk=16, keep_x=True, lanes=lanes)
     try:
         mix = srv.serve(reqs)
     finally:

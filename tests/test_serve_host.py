@@ -1,6 +1,7 @@
 """Host logic of the serving loop (no device): the request stream and the
 admission estimate."""
 import numpy as np
+import pytest
 
 from parafem_amd import serve as S
 
@@ -129,11 +130,13 @@ def _mock_classes(dev):
     return Ctx, Batch, Cache
 
 
-def test_admission_counts_cached_blocks_of_slots_being_set_up(monkeypatch):
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_admission_counts_cached_blocks_of_slots_being_set_up(monkeypatch, lanes):
     """A slot handed to the set-up thread still holds its cache of freed
     blocks while the new problem is allocated: admission must count both, so
     the device never overflows, whatever order the sizes come in (mock
-    contexts whose freed blocks are never reused)."""
+    contexts whose freed blocks are never reused); with one batch or two,
+    every request is served once."""
     budget = 3 * S.device_bytes(80, "mixed64")
     dev = _Device(budget)
     Ctx, Batch, Cache = _mock_classes(dev)
@@ -142,7 +145,7 @@ def test_admission_counts_cached_blocks_of_slots_being_set_up(monkeypatch):
     reqs = S.make_stream(120, 0.0, seed=9, nmin=10, nmax=80)
     for r in reqs:
         r.n = int(r.n)
-    srv = S.Server(slots=8, hbm_budget=budget, chunk=16, cache=Cache())
+    srv = S.Server(slots=8, hbm_budget=budget, chunk=16, cache=Cache(), lanes=lanes)
 
     orig_admit = S.Server._admit
 
