@@ -24,7 +24,6 @@
 // Reference: the product and SCATTER of p121.f90:89-97 (gather_scatter.f90:
 // 694-850) and p123.f90:135-145 (fixed rows).
 #include <algorithm>
-#include <cstring>
 #include <vector>
 
 #include "pf_internal.h"
@@ -187,21 +186,161 @@ void launch_ranges_nt(pf_ctx *c, const float *p32)
     }
 }
 
+// ---- building the ranges on the device ------------------------------------
+//
+// A workgroup per range sorts the range's slots as keys equation << 8 | slot
+// (slot = position among the range's R*NT slots): equal equations become
+// neighbours, in slot order.  k_rng_count counts the distinct equations;
+// after a scan of the counts k_rng_fill writes each range's ascending list,
+// the local index of every slot (the rank of its equation in the list) and
+// the rounds.  The colours of the ranges are chosen on the host, greedily in
+// range order (each sequentially depends on the ranges before it), from the
+// lists copied back; k_rng_mask and k_rng_flags then mark the first range
+// (lowest colour) of every equation and the fixed freedoms.
+static_assert(PF_BLOCK == 256, "a range's slot index takes the low 8 bits of a key");
+constexpr unsigned long long KEY_NONE = ~0ull;
+
+template <int NT>
+__device__ __forceinline__ void range_keys_sorted(const int *__restrict__ gg, long nslot, long r,
+                                                  unsigned long long *key)
+{
+    constexpr int W = RangeShape<NT>::R * NT;
+    const int t = threadIdx.x;
+    const long s = r * W + t;
+    unsigned long long kv = KEY_NONE;
+    if (t < W && s < nslot) {
+        const int g = gg[s];
+        if (g > 0) kv = ((unsigned long long)(unsigned)g << 8) | (unsigned)t;
+    }
+    key[t] = kv;
+    __syncthreads();
+    // bitonic sort of PF_BLOCK keys, ascending
+    for (int k = 2; k <= PF_BLOCK; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int x = t ^ j;
+            if (x > t) {
+                const unsigned long long a = key[t], b = key[x];
+                if ((a > b) == ((t & k) == 0)) {
+                    key[t] = b;
+                    key[x] = a;
+                }
+            }
+            __syncthreads();
+        }
+}
+
+template <int NT>
+__global__ __launch_bounds__(PF_BLOCK) void k_rng_count(const int *__restrict__ gg, long nslot,
+                                                       int *__restrict__ cnt, int *__restrict__ err)
+{
+    __shared__ unsigned long long key[PF_BLOCK];
+    const long r = blockIdx.x;
+    range_keys_sorted<NT>(gg, nslot, r, key);
+    const int t = threadIdx.x;
+    const unsigned long long kv = key[t], prev = t ? key[t - 1] : KEY_NONE;
+    const bool valid = kv != KEY_NONE;
+    const bool repeat = valid && t && (prev >> 8) == (kv >> 8);
+    // the same equation in two slots of one element
+    if (repeat && (int)(prev & 0xFF) / NT == (int)(kv & 0xFF) / NT) atomicOr(err, 1);
+    const int n = __syncthreads_count(valid && !repeat);
+    if (t == 0) cnt[r] = n;
+}
+
+template <int NT>
+__global__ __launch_bounds__(PF_BLOCK) void k_rng_fill(const int *__restrict__ gg, long nslot, long nels,
+                                                      const int *__restrict__ rptr, unsigned *__restrict__ ue,
+                                                      unsigned short *__restrict__ lmap,
+                                                      unsigned char *__restrict__ lround, int *__restrict__ nround)
+{
+    constexpr int R = RangeShape<NT>::R, W = R * NT;
+    static_assert(R <= 32, "rounds are bits of a 32-bit mask");
+    __shared__ unsigned long long key[PF_BLOCK];
+    __shared__ unsigned short lm[PF_BLOCK];
+    __shared__ unsigned rmask[PF_BLOCK];
+    __shared__ int wsum[PF_BLOCK / 64];
+    __shared__ unsigned busy;
+    const long r = blockIdx.x;
+    range_keys_sorted<NT>(gg, nslot, r, key);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const unsigned long long kv = key[t];
+    const bool valid = kv != KEY_NONE;
+    const bool first = valid && (t == 0 || (key[t - 1] >> 8) != (kv >> 8));
+    const unsigned long long bal = __ballot(first);
+    const int upto = __popcll(bal & ((2ull << lane) - 1ull));   // lanes 0..lane (all 64 at lane 63)
+    if (lane == 63) wsum[wave] = __popcll(bal);
+    lm[t] = LM_NONE;
+    rmask[t] = 0u;
+    __syncthreads();
+    int grp = upto - 1;   // rank of this key's equation among the range's distinct ones
+    for (int w = 0; w < wave; ++w) grp += wsum[w];
+    if (first) ue[rptr[r] + grp] = (unsigned)(kv >> 8);
+    if (valid) lm[kv & 0xFF] = (unsigned short)grp;
+    __syncthreads();
+    if (t < W && r * W + t < nslot) lmap[r * W + t] = lm[t];
+    // rounds: each element takes the lowest round that no earlier element of
+    // the range sharing an equation with it has taken (at most e bits are
+    // set before element e, so a round below 32 is free)
+    const long e0 = r * R;
+    int top = 0;
+    for (int e = 0; e < R && e0 + e < nels; ++e) {
+        if (t == 0) busy = 0u;
+        __syncthreads();
+        const unsigned short k = t < NT ? lm[e * NT + t] : LM_NONE;
+        if (k != LM_NONE) atomicOr(&busy, rmask[k]);
+        __syncthreads();
+        const int rd = __builtin_ctz(~busy);
+        if (k != LM_NONE) rmask[k] |= 1u << rd;
+        if (t == 0) {
+            lround[e0 + e] = (unsigned char)rd;
+            top = max(top, rd + 1);
+        }
+        __syncthreads();
+    }
+    if (t == 0) atomicMax(nround, top);
+}
+
+// mask[q] |= 1 << colour of every range listing q.
+__global__ __launch_bounds__(PF_BLOCK) void k_rng_mask(const int *__restrict__ rptr, const unsigned *__restrict__ ue,
+                                                      const int *__restrict__ rcol, unsigned *__restrict__ mask)
+{
+    const long r = blockIdx.x;
+    const unsigned bit = 1u << rcol[r];
+    for (int a = rptr[r] + threadIdx.x; a < rptr[r + 1]; a += PF_BLOCK) atomicOr(&mask[ue[a]], bit);
+}
+
+__global__ __launch_bounds__(PF_BLOCK) void k_rng_flags(const int *__restrict__ rptr, unsigned *__restrict__ ue,
+                                                       const int *__restrict__ rcol, const unsigned *__restrict__ mask,
+                                                       const double *__restrict__ fixstore)
+{
+    const long r = blockIdx.x;
+    const int col = rcol[r];
+    for (int a = rptr[r] + threadIdx.x; a < rptr[r + 1]; a += PF_BLOCK) {
+        const unsigned q = ue[a];
+        unsigned v = q;
+        if (__builtin_ctz(mask[q]) == col) v |= UE_FIRST;
+        if (fixstore && fixstore[q] != 0.0) v |= UE_FIXED;
+        ue[a] = v;
+    }
+}
+
+template <int NT>
+void launch_rng_count(pf_ctx *c, long nrange, long nslot, int *cnt, int *err)
+{
+    hipLaunchKernelGGL(k_rng_count<NT>, dim3(nrange), dim3(PF_BLOCK), 0, c->stream, c->g_g, nslot, cnt, err);
+}
+
+template <int NT>
+void launch_rng_fill(pf_ctx *c, long nrange, long nslot, const pf_ranges &F, int *nround)
+{
+    hipLaunchKernelGGL(k_rng_fill<NT>, dim3(nrange), dim3(PF_BLOCK), 0, c->stream, c->g_g, nslot, (long)c->nels,
+                       F.rptr, F.ue, F.lmap, F.lround, nround);
+}
+
 template <class T>
 void free_dev(pf_ctx *c, T *&p, size_t n)
 {
     pf_free(c, p, n * sizeof(T));
     p = nullptr;
-}
-
-template <class T>
-int upload(pf_ctx *c, T *&dst, const std::vector<T> &src)
-{
-    int rc = pf_alloc(c, (void **)&dst, std::max<size_t>(src.size(), 1) * sizeof(T));
-    if (rc) return rc;
-    if (!src.empty())
-        PF_HIP(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
-    return PF_OK;
 }
 
 }  // namespace
@@ -221,10 +360,11 @@ void pf_ranges_drop(pf_ctx *c)
     c->rng = nullptr;
 }
 
-// Builds the ranges of the current steering and fixed rows on the host (g_g
-// and fixstore copied back): distinct equations per range, the local index
-// of every element slot, the rounds inside a range, the colours of the
-// ranges and the first range of every equation.  Returns PF_ERR_ARG when the
+// Builds the ranges of the current steering and fixed rows: distinct
+// equations per range, the local index of every element slot and the rounds
+// inside a range on the device; the colours of the ranges on the host from
+// the lists (greedy in range order); the first range of every equation and
+// the fixed freedoms on the device again.  Returns PF_ERR_ARG when the
 // steering does not fit the scheme (an element listing an equation twice, or
 // more than 32 colours): the solve then keeps the slot-list scatter.
 int pf_ranges_build(pf_ctx *c)
@@ -240,104 +380,113 @@ int pf_ranges_build(pf_ctx *c)
     }
     const long nels = c->nels, nslot = (long)NT * nels;
     const long nrange = (nels + R - 1) / R;
-    std::vector<int> gg(nslot);
-    std::vector<double> fs;
-    PF_HIP(hipMemcpyAsync(gg.data(), c->g_g, nslot * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    if (c->fixstore) {
-        fs.resize((size_t)c->neq + 1);
-        PF_HIP(hipMemcpyAsync(fs.data(), c->fixstore, fs.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    }
-    PF_HIP(hipStreamSynchronize(c->stream));
-    std::vector<unsigned short> lmap(nslot, LM_NONE);
-    std::vector<unsigned char> lround(nels, 0);
-    std::vector<int> rptr(nrange + 1, 0);
-    std::vector<unsigned> ue;
-    ue.reserve((size_t)nslot / 2 + 16);
-    std::vector<int> rcol(nrange, 0);
-    std::vector<unsigned> mask((size_t)c->neq + 1, 0u);   // colours of the ranges touching each equation
-    std::vector<int> tmp;
-    std::vector<unsigned> rmask;
-    std::vector<long> stamp;
-    int nround = 1, ncol = 1;
-    for (long r = 0; r < nrange; ++r) {
-        const long e0 = r * R, e1 = std::min(nels, e0 + R);
-        tmp.clear();
-        for (long s = e0 * NT; s < e1 * NT; ++s)
-            if (gg[s] > 0) tmp.push_back(gg[s]);
-        std::sort(tmp.begin(), tmp.end());
-        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-        rptr[r] = (int)ue.size();
-        unsigned used = 0;
-        for (int q : tmp) {
-            ue.push_back((unsigned)q);
-            used |= mask[q];
-        }
-        if (~used == 0u) return PF_ERR_ARG;                  // more than 32 colours
-        const int col = __builtin_ctz(~used);
-        rcol[r] = col;
-        ncol = std::max(ncol, col + 1);
-        for (int q : tmp) mask[q] |= 1u << col;
-        // local indices and rounds: an element takes the lowest round no
-        // earlier element of the range sharing an equation has taken
-        rmask.assign(tmp.size(), 0u);
-        stamp.assign(tmp.size(), -1L);
-        for (long e = e0; e < e1; ++e) {
-            unsigned busy = 0;
-            for (int i = 0; i < NT; ++i) {
-                const int q = gg[e * NT + i];
-                if (q <= 0) continue;
-                const int kk = (int)(std::lower_bound(tmp.begin(), tmp.end(), q) - tmp.begin());
-                if (stamp[kk] == e) return PF_ERR_ARG;   // an equation twice in one element
-                stamp[kk] = e;
-                lmap[e * NT + i] = (unsigned short)kk;
-                busy |= rmask[kk];
-            }
-            if (~busy == 0u) return PF_ERR_ARG;
-            const int rd = __builtin_ctz(~busy);
-            if (rd > 254) return PF_ERR_ARG;
-            lround[e] = (unsigned char)rd;
-            nround = std::max(nround, rd + 1);
-            for (int i = 0; i < NT; ++i)
-                if (lmap[e * NT + i] != LM_NONE) rmask[lmap[e * NT + i]] |= 1u << rd;
-        }
-    }
-    rptr[nrange] = (int)ue.size();
-    // flags: the lowest colour touching an equation writes it; fixed rows
-    for (long r = 0; r < nrange; ++r)
-        for (int a = rptr[r]; a < rptr[r + 1]; ++a) {
-            const int q = (int)ue[a];
-            if (__builtin_ctz(mask[q]) == rcol[r]) ue[a] |= UE_FIRST;
-            if (!fs.empty() && fs[q] != 0.0) ue[a] |= UE_FIXED;
-        }
-    // ranges by colour, ascending range order inside a colour
-    std::vector<int> cstart(ncol + 1, 0), cperm(nrange);
-    for (long r = 0; r < nrange; ++r) cstart[rcol[r] + 1]++;
-    for (int k2 = 0; k2 < ncol; ++k2) cstart[k2 + 1] += cstart[k2];
-    {
-        std::vector<int> fill(cstart.begin(), cstart.end() - 1);
-        for (long r = 0; r < nrange; ++r) cperm[fill[rcol[r]]++] = (int)r;
-    }
+    if (nrange >= INT32_MAX) return PF_ERR_ARG;
     c->rng = new pf_ranges();
     pf_ranges &F = *c->rng;
     F.nslot = nslot;
     F.nels = nels;
     F.nrange = nrange;
-    F.nue = (long)ue.size();
-    F.nround = nround;
+    const int nb = pf_scan_blocks(nrange);
+    int *cnt = nullptr, *bsum = nullptr, *flags = nullptr, *rcol = nullptr;
+    unsigned *mask = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(c->stream);
+        free_dev(c, cnt, (size_t)nrange);
+        free_dev(c, bsum, (size_t)nb);
+        free_dev(c, flags, 4);
+        free_dev(c, rcol, (size_t)nrange);
+        free_dev(c, mask, (size_t)c->neq + 1);
+    };
+    auto fail = [&](int rc) {
+        release();
+        pf_ranges_drop(c);
+        return rc;
+    };
+    int rc = pf_alloc(c, (void **)&cnt, (size_t)nrange * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&bsum, (size_t)nb * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&flags, 4 * sizeof(int));   // error, list entries, rounds, zero counts
+    if (!rc) rc = pf_alloc(c, (void **)&F.rptr, ((size_t)nrange + 1) * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&F.lmap, (size_t)nslot * sizeof(unsigned short));
+    if (!rc) rc = pf_alloc(c, (void **)&F.lround, (size_t)nels);
+    if (rc) return fail(rc);
+    int host[3] = {0, 0, 0};
+    hipError_t he = hipMemsetAsync(flags, 0, 4 * sizeof(int), c->stream);
+    if (he == hipSuccess) he = hipMemsetAsync(F.rptr, 0, sizeof(int), c->stream);
+    if (he == hipSuccess) {
+        switch (NT) {
+        case 60: launch_rng_count<60>(c, nrange, nslot, cnt, flags); break;
+        case 24: launch_rng_count<24>(c, nrange, nslot, cnt, flags); break;
+        default: launch_rng_count<8>(c, nrange, nslot, cnt, flags); break;
+        }
+        pf_scan_launch(c, cnt, (int)nrange, bsum, F.rptr, flags + 1, flags + 3);
+        he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = hipMemcpyAsync(host, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    if (he != hipSuccess) return fail(pf_set_hip_error(he, "range lists", __FILE__, __LINE__));
+    if (host[0]) return fail(PF_ERR_ARG);   // an equation twice in one element
+    F.nue = host[1];
+    rc = pf_alloc(c, (void **)&F.ue, (size_t)std::max<long>(F.nue, 1) * sizeof(unsigned));
+    if (!rc) rc = pf_alloc(c, (void **)&rcol, (size_t)nrange * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&mask, ((size_t)c->neq + 1) * sizeof(unsigned));
+    if (!rc) rc = pf_alloc(c, (void **)&F.cperm, (size_t)nrange * sizeof(int));
+    if (!rc) rc = pf_alloc(c, (void **)&F.rpart, ((size_t)nrange + NF) * sizeof(double));
+    if (rc) return fail(rc);
+    switch (NT) {
+    case 60: launch_rng_fill<60>(c, nrange, nslot, F, flags + 2); break;
+    case 24: launch_rng_fill<24>(c, nrange, nslot, F, flags + 2); break;
+    default: launch_rng_fill<8>(c, nrange, nslot, F, flags + 2); break;
+    }
+    he = hipGetLastError();
+    // the lists for the colouring
+    std::vector<int> rptr(nrange + 1);
+    std::vector<unsigned> ue((size_t)F.nue);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(rptr.data(), F.rptr, rptr.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (he == hipSuccess && F.nue)
+        he = hipMemcpyAsync(ue.data(), F.ue, ue.size() * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(host + 2, flags + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    if (he != hipSuccess) return fail(pf_set_hip_error(he, "range lists", __FILE__, __LINE__));
+    F.nround = std::max(host[2], 1);
+    // colours: a range takes the lowest colour no earlier range sharing an
+    // equation has taken
+    std::vector<int> col(nrange);
+    std::vector<unsigned> used_by((size_t)c->neq + 1, 0u);
+    int ncol = 1;
+    for (long r = 0; r < nrange; ++r) {
+        unsigned used = 0;
+        for (int a = rptr[r]; a < rptr[r + 1]; ++a) used |= used_by[ue[a]];
+        if (~used == 0u) return fail(PF_ERR_ARG);   // more than 32 colours
+        const int k = __builtin_ctz(~used);
+        col[r] = k;
+        ncol = std::max(ncol, k + 1);
+        for (int a = rptr[r]; a < rptr[r + 1]; ++a) used_by[ue[a]] |= 1u << k;
+    }
+    // ranges by colour, ascending range order inside a colour
+    std::vector<int> cstart(ncol + 1, 0), cperm(nrange);
+    for (long r = 0; r < nrange; ++r) cstart[col[r] + 1]++;
+    for (int k = 0; k < ncol; ++k) cstart[k + 1] += cstart[k];
+    {
+        std::vector<int> fill(cstart.begin(), cstart.end() - 1);
+        for (long r = 0; r < nrange; ++r) cperm[fill[col[r]]++] = (int)r;
+    }
     F.ncol = ncol;
     F.cstart = cstart;
-    int rc = upload(c, F.lmap, lmap);
-    if (!rc) rc = upload(c, F.lround, lround);
-    if (!rc) rc = upload(c, F.rptr, rptr);
-    if (!rc) rc = upload(c, F.ue, ue);
-    if (!rc) rc = upload(c, F.cperm, cperm);
-    if (!rc) rc = pf_alloc(c, (void **)&F.rpart, ((size_t)nrange + NF) * sizeof(double));
-    if (!rc) {
-        const hipError_t e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) rc = pf_set_hip_error(e, "range upload", __FILE__, __LINE__);
+    he = hipMemcpyAsync(rcol, col.data(), nrange * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(F.cperm, cperm.data(), nrange * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (he == hipSuccess) he = hipMemsetAsync(mask, 0, ((size_t)c->neq + 1) * sizeof(unsigned), c->stream);
+    if (he == hipSuccess) {
+        hipLaunchKernelGGL(k_rng_mask, dim3(nrange), dim3(PF_BLOCK), 0, c->stream, F.rptr, F.ue, rcol, mask);
+        hipLaunchKernelGGL(k_rng_flags, dim3(nrange), dim3(PF_BLOCK), 0, c->stream, F.rptr, F.ue, rcol, mask,
+                           c->fixstore);
+        he = hipGetLastError();
     }
-    if (rc) pf_ranges_drop(c);
-    return rc;
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    if (he != hipSuccess) return fail(pf_set_hip_error(he, "range flags", __FILE__, __LINE__));
+    release();
+    return PF_OK;
 }
 
 // The mixed iteration's product, sums into u and alpha (replaces k_ebe,

@@ -221,6 +221,13 @@ void pf_ranges_drop(pf_ctx *c);
 int pf_ranges_iteration(pf_ctx *c, const float *p32);
 int pf_ranges_fold(pf_ctx *c);
 
+// Exclusive scan of n integer counts on the context's stream (pf_setup.hip):
+// ptr[q + 1] = cnt[0] + ... + cnt[q] (ptr[0] is the caller's), the sum into
+// *total, bit 2 into *zero_flag where a count is zero; bsum holds
+// pf_scan_blocks(n) ints.
+int pf_scan_blocks(long n);
+void pf_scan_launch(pf_ctx *c, const int *cnt, int n, int *bsum, int *ptr, int *total, int *zero_flag);
+
 // Looks up, once per device (the device current), the address of the
 // allocation-free entry points' scratch array (pf_ebe.hip).
 int pf_scratch_init(int device);

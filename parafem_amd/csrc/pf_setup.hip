@@ -164,6 +164,16 @@ int grid_for_stride(long n) { return (int)std::max<long>(1, std::min<long>((n + 
 
 }  // namespace
 
+int pf_scan_blocks(long n) { return (int)std::max<long>(1, (n + SCAN_CHUNK - 1) / SCAN_CHUNK); }
+
+void pf_scan_launch(pf_ctx *c, const int *cnt, int n, int *bsum, int *ptr, int *total, int *zero_flag)
+{
+    const int nb = pf_scan_blocks(n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, c->stream, cnt, n, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream, bsum, nb, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, c->stream, cnt, n, bsum, ptr, zero_flag);
+}
+
 // Transposed steering of c->g_g (already on the device, c->neq and c->ntot,
 // c->nels set) into c->csr_ptr / c->csr_slot; PF_ERR_ARG when an entry is
 // outside 0..neq or an equation has no slot.
@@ -193,9 +203,7 @@ int pf_csr_build(pf_ctx *c)
     if (he == hipSuccess) {
         const int gs = grid_for_stride(nslot);
         hipLaunchKernelGGL(k_csr_count, dim3(gs), dim3(256), 0, c->stream, c->g_g, nslot, neq, cnt, flags);
-        hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, c->stream, cnt, neq, bsum);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream, bsum, nb, flags + 1);
-        hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, c->stream, cnt, neq, bsum, c->csr_ptr, flags);
+        pf_scan_launch(c, cnt, neq, bsum, c->csr_ptr, flags + 1, flags);
         he = hipGetLastError();
     }
     if (he == hipSuccess) he = hipMemcpyAsync(host, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream);

@@ -92,3 +92,28 @@ def test_ranges_mode_rules():
         ctx.scatter_mode = "slots"
         xt, rt = ctx.pcg_solve(r, 500, 1e-9, stop="relres", precision="split")
         assert rs.iters == rt.iters and np.array_equal(xs, xt)
+
+
+def test_ranges_refuse_an_equation_twice_in_one_element():
+    """The device build of the ranges checks that no element lists an
+    equation twice (its rows would be summed in one round); the solve is
+    refused in range mode and runs with the slot-list sums."""
+    pr = P.elastic_cube(3)
+    g = pr.g_g.copy()
+    # the last element: slot j's equation listed again in slot i, where j's
+    # own equation is listed by another element too (every equation keeps a slot)
+    e = pr.nels - 1
+    uses = np.bincount(g.ravel(), minlength=pr.neq + 1)
+    free = np.flatnonzero(g[e] > 0)
+    i = int(free[-1])
+    j = int(next(k for k in free if k != i and uses[g[e, k]] > 1))
+    g[e, j] = g[e, i]
+    bad = P.Problem(pr.kind, pr.mesh, g, pr.neq, pr.r0, pr.nf, e=pr.e, v=pr.v)
+    with Context(0) as ctx:
+        r = P.setup(ctx, bad, store="fp32")
+        ctx.scatter_mode = "ranges"
+        with pytest.raises(PFError):
+            ctx.pcg_begin(r, 10, 1e-8, stop="relres", precision="mixed")
+        ctx.scatter_mode = "slots"
+        ctx.pcg_begin(r, 10, 1e-8, stop="relres", precision="mixed")
+        ctx.pcg_run(2)
