@@ -333,7 +333,8 @@ def serve_bench(args):
     rate = args.rate
     if phases & {"closed", "stream"}:
         srv = S.Server(slots=args.slots, cache=cache, precision=args.serve_precision, chunk=args.chunk,
-                       persistent_elems=args.persistent_elems, solo_elems=args.solo_elems, lanes=args.lanes)
+                       persistent_elems=args.persistent_elems, solo_elems=args.solo_elems, lanes=args.lanes,
+                       workers=args.workers, refine_delta=args.refine_delta)
         try:
             # warm-up: compile/capture paths once
             srv.serve(S.make_stream(args.slots, 0.0, seed=5, nmin=args.nmin, nmax=args.nmin + 2))
@@ -423,6 +424,10 @@ def main():
     ap.add_argument("--chunk", type=int, default=16, help="serve: iterations per batch run")
     ap.add_argument("--lanes", type=int, default=1,
                     help="serve: independent batches on their own streams")
+    ap.add_argument("--refine-delta", type=float, default=0.1,
+                    help="serve: reliable-update threshold of the mixed server (0: the library's 1e-2)")
+    ap.add_argument("--workers", type=int, default=1,
+                    help="serve: host threads setting up requests and reading back results")
     ap.add_argument("--solo-elems", type=int, default=0,
                     help="serve: solves of at least this many elements run their own launches inside the batch "
                          "(0: none)")

@@ -89,11 +89,14 @@ class CubeCache:
     def __init__(self, keep: int = 24):
         self.keep = keep
         self._d: OrderedDict[int, dict] = OrderedDict()
+        self._lock = threading.Lock()     # set-up threads share the cache
 
     def get(self, n: int) -> dict:
-        if n in self._d:
-            self._d.move_to_end(n)
-            return self._d[n]
+        with self._lock:
+            if n in self._d:
+                self._d.move_to_end(n)
+                return self._d[n]
+        # built outside the lock (two threads missing the same size both build it)
         m = M.box_hex20(n, n, n, 1.0, 1.0, 1.0)
         rest = M.base_fixed_rest(m)
         nf, neq = M.steering(m.nn, 3, rest)
@@ -101,9 +104,10 @@ class CubeCache:
         r_unit = M.nodal_loads_to_rhs(nf, nodes, vals, neq)
         entry = {"g_coord": m.g_coord(), "g_num": m.g_num, "rest": rest, "nn": m.nn, "neq": neq,
                  "nels": m.nels, "r_unit": r_unit}
-        self._d[n] = entry
-        while len(self._d) > self.keep:
-            self._d.popitem(last=False)
+        with self._lock:
+            self._d[n] = entry
+            while len(self._d) > self.keep:
+                self._d.popitem(last=False)
         return entry
 
 
@@ -122,14 +126,14 @@ def device_bytes(n: int, precision: str) -> int:
 
 
 def solve_alone(ctx: Context, cache: CubeCache, req: Request, precision: str, tol: float,
-                limit: int = 100000):
+                limit: int = 100000, refine_delta: float = 0.0):
     """One request by itself on `ctx` (blocking): (x, PcgResult)."""
     c = cache.get(req.n)
     ctx.mesh_set(c["g_coord"])
     ctx.steer_build(c["g_num"], c["rest"], c["nn"])
     ctx.assemble_elastic(req.e, req.v, None, store="fp64" if precision == "fp64" else "fp32")
     x, res = ctx.pcg_solve(c["r_unit"] * req.pressure, limit, tol, stop="relres", precision=precision,
-                           check_every=64)
+                           check_every=64, refine_delta=refine_delta)
     ctx.reset()
     return x, res
 
@@ -147,8 +151,8 @@ class Server:
     """Continuous batching: every admitted solve is a slot of one pf_batch,
     so each launch of the iteration's kernels advances all of them; solves
     join and leave the batch between chunks of iterations.  Set-up of
-    arriving requests and read-back of finished ones run on a second host
-    thread (the library's calls release the interpreter lock), so the
+    arriving requests and read-back of finished ones run on other host
+    threads (the library's calls release the interpreter lock), so the
     thread driving the batch never waits for them."""
     slots: int = 32
     hbm_budget: float = 240e9
@@ -174,6 +178,14 @@ class Server:
     # or three lost 2-12% of the closed-loop capacity on [10, 80] and
     # [10, 40] streams)
     lanes: int = 1
+    # host threads setting up arriving requests and reading back finished
+    # ones, each on its own slot's context and stream
+    workers: int = 1
+    # mixed solvers: replace the recurrence residual by the true one each
+    # time it has fallen by this factor (the library's default is 1e-2; on
+    # uniform-material cubes 0.1 took 5-11% fewer iterations at n = 40-60,
+    # and 1e-4 up to 24 times more)
+    refine_delta: float = 0.1
 
     def __post_init__(self):
         if not 1 <= self.slots <= BATCH_MAX:
@@ -181,12 +193,16 @@ class Server:
         self._slots = [_Slot(Context(self.device)) for _ in range(self.slots)]
         if self.lanes < 1:
             raise ValueError("at least one lane")
+        if self.workers < 1:
+            raise ValueError("at least one set-up thread")
+        self._stats_lock = threading.Lock()
         self._batches = [Batch(self.device) for _ in range(self.lanes)]
         for b in self._batches:
             b.solo_set(self.solo_elems)
-        # host-side time of the last serve() on the worker thread: set-up of
-        # requests, read-backs of finished solves; cache trims (each waits
-        # for the device) and batch cycles on the driving thread
+        # host-side time of the last serve() on the set-up threads (summed
+        # over them): set-up of requests, read-backs of finished solves;
+        # cache trims (each waits for the device) and batch cycles on the
+        # driving thread
         self.stats = {"admit_s": 0.0, "readback_s": 0.0, "trims": 0, "cycles": 0}
 
     def close(self):
@@ -229,7 +245,8 @@ class Server:
         ctx.mesh_set(c["g_coord"])
         ctx.steer_build(c["g_num"], c["rest"], c["nn"])      # steering and maps built on the device
         ctx.assemble_elastic(req.e, req.v, None, store="fp64" if self.precision == "fp64" else "fp32")
-        ctx.pcg_begin(c["r_unit"] * req.pressure, self.limit, self.tol, stop="relres", precision=self.precision)
+        ctx.pcg_begin(c["r_unit"] * req.pressure, self.limit, self.tol, stop="relres", precision=self.precision,
+                      refine_delta=self.refine_delta)
 
     def _worker(self, jobs: queue_mod.Queue, results: queue_mod.Queue):
         """Admissions and retirements, one at a time, on their slot's
@@ -244,7 +261,8 @@ class Server:
                 if kind == "admit":
                     self._admit(s, payload)
                     s.t_admit = tb - t0
-                    self.stats["admit_s"] += time.perf_counter() - tb
+                    with self._stats_lock:
+                        self.stats["admit_s"] += time.perf_counter() - tb
                     results.put(("admitted", s, None))
                 else:
                     r = payload
@@ -254,7 +272,8 @@ class Server:
                     res = Result(q.rid, q.n, s.ctx.neq, r.iters, bool(r.converged), r.refinements, s.t_admit, t,
                                  t - q.arrival, x if self.keep_x else None, float(np.abs(x).sum()))
                     s.ctx.reset()
-                    self.stats["readback_s"] += time.perf_counter() - tb
+                    with self._stats_lock:
+                        self.stats["readback_s"] += time.perf_counter() - tb
                     results.put(("retired", s, res))
             except Exception as exc:              # handed to the driving thread
                 results.put(("error", s, exc))
@@ -283,8 +302,10 @@ class Server:
         self.stats = {"admit_s": 0.0, "readback_s": 0.0, "trims": 0, "cycles": 0}
         jobs: queue_mod.Queue = queue_mod.Queue()
         results: queue_mod.Queue = queue_mod.Queue()
-        worker = threading.Thread(target=self._worker, args=(jobs, results), daemon=True)
-        worker.start()
+        workers = [threading.Thread(target=self._worker, args=(jobs, results), daemon=True)
+                   for _ in range(self.workers)]
+        for w in workers:
+            w.start()
         t0 = time.perf_counter()
 
         def elems(lane):
@@ -368,14 +389,16 @@ class Server:
                     else:
                         time.sleep(20e-6)
         finally:
-            jobs.put(None)
-            worker.join()
+            for _ in workers:
+                jobs.put(None)
+            for w in workers:
+                w.join()
         return sorted(out, key=lambda r: r.rid)
 
 
 def serve_sequential(requests: list[Request], precision: str = "fp64", tol: float = 1e-10,
                      device: int = 0, keep_x: bool = False, cache: CubeCache | None = None,
-                     log=None) -> list[Result]:
+                     log=None, refine_delta: float = 0.0) -> list[Result]:
     """The same stream one request at a time (blocking solves, arrival order,
     real time)."""
     cache = cache or CubeCache()
@@ -388,7 +411,7 @@ def serve_sequential(requests: list[Request], precision: str = "fp64", tol: floa
             if r.arrival > now:
                 time.sleep(r.arrival - now)
             ta = time.perf_counter() - t0
-            x, res = solve_alone(ctx, cache, r, precision, tol)
+            x, res = solve_alone(ctx, cache, r, precision, tol, refine_delta=refine_delta)
             t = time.perf_counter() - t0
             out.append(Result(r.rid, r.n, x.size, res.iters, bool(res.converged), res.refinements, ta, t,
                               t - r.arrival, x if keep_x else None, float(np.abs(x).sum())))

@@ -562,7 +562,7 @@ def test_serving_mix_equals_alone(lanes):
     finally:
         srv.close()
     assert [m.rid for m in mix] == list(range(10)) and all(m.converged for m in mix)
-    seq = S.serve_sequential(reqs, precision=srv.precision, keep_x=True)
+    seq = S.serve_sequential(reqs, precision=srv.precision, keep_x=True, refine_delta=srv.refine_delta)
     f64 = S.serve_sequential(reqs, precision="fp64", keep_x=True)
     for a, b, c in zip(mix, seq, f64):
         assert a.iters == b.iters and np.array_equal(a.x, b.x), a.rid

@@ -130,13 +130,13 @@ def _mock_classes(dev):
     return Ctx, Batch, Cache
 
 
-@pytest.mark.parametrize("lanes", [1, 2])
-def test_admission_counts_cached_blocks_of_slots_being_set_up(monkeypatch, lanes):
-    """A slot handed to the set-up thread still holds its cache of freed
+@pytest.mark.parametrize("lanes,workers", [(1, 1), (2, 1), (1, 3)])
+def test_admission_counts_cached_blocks_of_slots_being_set_up(monkeypatch, lanes, workers):
+    """A slot handed to a set-up thread still holds its cache of freed
     blocks while the new problem is allocated: admission must count both, so
     the device never overflows, whatever order the sizes come in (mock
     contexts whose freed blocks are never reused); with one batch or two,
-    every request is served once."""
+    one set-up thread or several, every request is served once."""
     budget = 3 * S.device_bytes(80, "mixed64")
     dev = _Device(budget)
     Ctx, Batch, Cache = _mock_classes(dev)
@@ -145,7 +145,7 @@ def test_admission_counts_cached_blocks_of_slots_being_set_up(monkeypatch, lanes
     reqs = S.make_stream(120, 0.0, seed=9, nmin=10, nmax=80)
     for r in reqs:
         r.n = int(r.n)
-    srv = S.Server(slots=8, hbm_budget=budget, chunk=16, cache=Cache(), lanes=lanes)
+    srv = S.Server(slots=8, hbm_budget=budget, chunk=16, cache=Cache(), lanes=lanes, workers=workers)
 
     orig_admit = S.Server._admit
 
