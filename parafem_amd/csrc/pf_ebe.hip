@@ -446,6 +446,7 @@ __device__ void finish_iteration(pf_state *st, double rd, double rr, double maxx
     st->rr = rr;
     st->crit = crit;
     st->iters += 1;
+    st->pfresh = 0;
     st->converged = crit <= st->tol;
     if (!isfinite(beta) || !isfinite(rd)) st->breakdown = 1;
     if (st->converged || st->iters >= st->limit || st->breakdown) st->frozen = 1;
@@ -1453,6 +1454,7 @@ static int pcg_begin(pf_ctx *c, const double *r0, int r0_on_device, const double
     const double delta = opts->refine_delta > 0.0 ? opts->refine_delta : 1e-2;
     s.delta2 = delta * delta;
     s.refine_every = opts->refine_every;
+    s.pfresh = 1;
     *c->state_host = s;
     PF_HIP(hipMemcpyAsync(c->state, c->state_host, sizeof(pf_state), hipMemcpyHostToDevice, c->stream));
     const int G = pf_grid_for(c->neq);
@@ -1500,7 +1502,7 @@ int pf_pcg_run(pf_ctx *c, int n)
         bool tm = pf_timing_begin(c, PF_K_EBE, &t0, &t1);
         if (tm) PF_HIP(hipEventRecord(t0, c->stream));
         if (ranges)
-            rc = pf_ranges_iteration(c, p32);   // products, their sums into u, p.u per range
+            rc = pf_ranges_iteration(c);   // p, the products, their sums into u, p.u per range
         else if (mixed)
             rc = launch_ebe<KIND_F32, true, float>(c, c->p, c->utemp32, c->state, RUN_LIVE, nullptr, p32, rm);
         else if (wide)
@@ -1563,10 +1565,12 @@ int pf_pcg_run(pf_ctx *c, int n)
             hipLaunchKernelGGL(k_fold_refine, dim3(1), dim3(PF_BLOCK), 0, c->stream, c->partials, Gr, c->state);
             if (tm) PF_HIP(hipEventRecord(t1, c->stream));
         }
-        tm = pf_timing_begin(c, PF_K_PUPDATE, &t0, &t1);
-        if (tm) PF_HIP(hipEventRecord(t0, c->stream));
-        hipLaunchKernelGGL(k_pupdate, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->p, p32, c->d, c->neq, c->state);
-        if (tm) PF_HIP(hipEventRecord(t1, c->stream));
+        if (!ranges || !pf_ranges_form_p(c)) {   // else the next range kernels form p
+            tm = pf_timing_begin(c, PF_K_PUPDATE, &t0, &t1);
+            if (tm) PF_HIP(hipEventRecord(t0, c->stream));
+            hipLaunchKernelGGL(k_pupdate, dim3(G), dim3(PF_BLOCK), 0, c->stream, c->p, p32, c->d, c->neq, c->state);
+            if (tm) PF_HIP(hipEventRecord(t1, c->stream));
+        }
         PF_HIP(hipGetLastError());
     }
     return PF_OK;

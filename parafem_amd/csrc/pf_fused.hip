@@ -19,7 +19,9 @@
 // summed per range over the range's equations (p(q) times its sum there) and
 // folded over the ranges in range order; a fixed freedom (p123's penalty
 // rows, p122's zero rows) is written by its first range as p*store (or 0),
-// contributing p*u once.
+// contributing p*u once.  On hex8 meshes the first range of each equation
+// also forms the iteration's p = d + beta p there (the p update of the other
+// paths), so p makes no separate pass and needs no FP32 copy.
 //
 // Reference: the product and SCATTER of p121.f90:89-97 (gather_scatter.f90:
 // 694-850) and p123.f90:135-145 (fixed rows).
@@ -41,16 +43,25 @@ template <int NT> struct RangeShape {
     static_assert(R * NT <= PF_BLOCK, "one thread per distinct equation of a range");
 };
 
+// Whether the range kernels form p = d + beta p themselves (hex8: it saves
+// the p pass and the FP32 copy of p that their gathers read).  For hex20 the
+// separate pass is cheaper: forming p in the ranges (an extra d read and a
+// scattered p write per equation) cost 0.11 ms per iteration at 1M elements,
+// the pass 0.058 ms.
+__host__ __device__ constexpr bool range_forms_p(int nt) { return nt <= 24; }
+
 // One workgroup per range of this colour (ranges[blockIdx.x]).
 template <int NT>
 __global__ __launch_bounds__(PF_BLOCK) void k_ebe_range(
     const float *__restrict__ Khi, const unsigned short *__restrict__ lmap, const unsigned char *__restrict__ lround,
     int nround, const int *__restrict__ rptr, const unsigned *__restrict__ ue, const int *__restrict__ ranges,
-    const double *__restrict__ p, const float *__restrict__ p32, double *__restrict__ u,
+    double *__restrict__ p, const double *__restrict__ d, double *__restrict__ u,
     const double *__restrict__ fixstore, int fix_zero, double *__restrict__ rpart, int nels,
     const pf_state *__restrict__ st)
 {
     if (skip(st, RUN_LIVE)) return;
+    const double beta = st->beta;
+    const bool pnew = range_forms_p(NT) && !st->pfresh;
     using L = Lanes<NT, KIND_F32>;
     constexpr int R = RangeShape<NT>::R;
     constexpr int D = pf_sym_diags(NT), Q = pf_sym_quads(NT), RM = pf_sym_rem(NT), H = NT / 2;
@@ -90,10 +101,18 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe_range(
     const int eq = (int)(uf & UE_EQ);
     double pd = 0.0, uold = 0.0, fsv = 0.0;
     if (t < nu) {
-        pd = p[eq];
-        pl[t] = p32 ? p32[eq] : (float)pd;
+        // every load before any result is used (one memory latency); then
+        // p = d + beta p (the reference's p update, p121.f90:102) by the
+        // equation's first range, stored with u at the end (a store here
+        // would hold the barrier below until it completes); the later
+        // colours read it
         if (!(uf & UE_FIXED) && !(uf & UE_FIRST)) uold = u[eq];
         if ((uf & UE_FIXED) && (uf & UE_FIRST) && !fix_zero) fsv = fixstore[eq];
+        const double pold = p[eq];
+        const bool form = pnew && (uf & UE_FIRST);
+        const double dq = form ? d[eq] : 0.0;
+        pd = form ? __dadd_rn(dq, __dmul_rn(pold, beta)) : pold;
+        pl[t] = (float)pd;
         pacc[t] = 0.0;
     }
     __syncthreads();
@@ -124,6 +143,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_ebe_range(
     }
     double dot[1] = {0.0};
     if (t < nu) {
+        if (pnew && (uf & UE_FIRST)) p[eq] = pd;
         const double s = pacc[t];
         if (uf & UE_FIXED) {
             if (uf & UE_FIRST) {
@@ -174,14 +194,14 @@ __global__ __launch_bounds__(PF_BLOCK) void k_fold_alpha_ranges(const double *fp
 }
 
 template <int NT>
-void launch_ranges_nt(pf_ctx *c, const float *p32)
+void launch_ranges_nt(pf_ctx *c)
 {
     const pf_ranges &F = *c->rng;
     for (int col = 0; col < F.ncol; ++col) {
         const int nb = F.cstart[col + 1] - F.cstart[col];
         if (nb <= 0) continue;
         hipLaunchKernelGGL(k_ebe_range<NT>, dim3(nb), dim3(PF_BLOCK), 0, c->stream, c->K32hi, F.lmap, F.lround,
-                           F.nround, F.rptr, F.ue, F.cperm + F.cstart[col], c->p, p32, c->u, c->fixstore,
+                           F.nround, F.rptr, F.ue, F.cperm + F.cstart[col], c->p, c->d, c->u, c->fixstore,
                            c->fix_zero ? 1 : 0, F.rpart, c->nels, c->state);
     }
 }
@@ -491,17 +511,19 @@ int pf_ranges_build(pf_ctx *c)
 
 // The mixed iteration's product, sums into u and alpha (replaces k_ebe,
 // k_scatter_pu and k_fold_alpha).
-int pf_ranges_iteration(pf_ctx *c, const float *p32)
+int pf_ranges_iteration(pf_ctx *c)
 {
     if (!c->rng) return PF_ERR_STATE;
     switch (c->ntot) {
-    case 60: launch_ranges_nt<60>(c, p32); break;
-    case 24: launch_ranges_nt<24>(c, p32); break;
-    default: launch_ranges_nt<8>(c, p32); break;
+    case 60: launch_ranges_nt<60>(c); break;
+    case 24: launch_ranges_nt<24>(c); break;
+    default: launch_ranges_nt<8>(c); break;
     }
     PF_HIP(hipGetLastError());
     return PF_OK;
 }
+
+bool pf_ranges_form_p(const pf_ctx *c) { return range_forms_p(c->ntot); }
 
 int pf_ranges_fold(pf_ctx *c)
 {

@@ -86,6 +86,8 @@ struct pf_state {
     int    refinements;
     int    refine_every;
     int    since_refine;
+    int    pfresh;    // p holds this iteration's direction (the start); otherwise the
+                      // range-fused products of hex8 meshes form p = d + beta p
 };
 
 // Multi-RHS block solve (pf_block_solve): up to PF_BLK right-hand sides as
@@ -218,8 +220,9 @@ struct pf_ctx {
 // u and the p.u partials; fold alpha from them.
 int pf_ranges_build(pf_ctx *c);
 void pf_ranges_drop(pf_ctx *c);
-int pf_ranges_iteration(pf_ctx *c, const float *p32);
+int pf_ranges_iteration(pf_ctx *c);
 int pf_ranges_fold(pf_ctx *c);
+bool pf_ranges_form_p(const pf_ctx *c);   // the range kernels form p (no p pass)
 
 // Exclusive scan of n integer counts on the context's stream (pf_setup.hip):
 // ptr[q + 1] = cnt[0] + ... + cnt[q] (ptr[0] is the caller's), the sum into
