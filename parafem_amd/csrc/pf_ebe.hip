@@ -400,12 +400,15 @@ __device__ __forceinline__ void update_block(int lb, int G, double *__restrict__
     for (int k = 0; k < NQ; ++k) v[k] = 0.0;
     const int stride = G * PF_BLOCK;
     for (int q = lb * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
-        const double xo = x[q];
+        // x, r and u are next touched a whole iteration later (streaming
+        // loads and stores); d is read again by the p update right after
+        const double xo = __builtin_nontemporal_load(&x[q]);
         const double xn = __dadd_rn(xo, __dmul_rn(p[q], alpha));
-        const double rq = __dsub_rn(r[q], __dmul_rn(u[q], alpha));
+        const double rq = __dsub_rn(__builtin_nontemporal_load(&r[q]),
+                                    __dmul_rn(__builtin_nontemporal_load(&u[q]), alpha));
         const double dq = __dmul_rn(dinv[q], rq);
-        x[q] = xn;
-        r[q] = rq;
+        __builtin_nontemporal_store(xn, &x[q]);
+        __builtin_nontemporal_store(rq, &r[q]);
         d[q] = dq;
         v[0] = __dadd_rn(v[0], __dmul_rn(rq, dq));
         if constexpr (CHECON) {
