@@ -160,3 +160,50 @@ def test_admission_counts_cached_blocks_of_slots_being_set_up(monkeypatch, lanes
     assert srv.stats["trims"] > 0          # the budget was reached and caches handed back
     # ... but only caches that hold something, and only when that makes room
     assert srv.stats["trims"] <= len(reqs)
+
+
+def test_server_and_one_by_one_use_the_same_replacement_threshold(monkeypatch):
+    """The server's mixed solves and serve_sequential's pass the same
+    residual-replacement threshold to the library (the serving tests compare
+    the two bit for bit)."""
+    seen = []
+
+    class Ctx:
+        neq = 4
+
+        def __init__(self, device=0):
+            pass
+
+        def mesh_set(self, g):
+            pass
+
+        def steer_build(self, g_num, rest, nn):
+            return nn, None
+
+        def assemble_elastic(self, e, v, ee, store="fp64"):
+            pass
+
+        def pcg_begin(self, r, limit, tol, **kw):
+            seen.append(("server", kw.get("refine_delta")))
+
+        def pcg_solve(self, r, limit, tol, **kw):
+            seen.append(("alone", kw.get("refine_delta")))
+            return np.zeros(4), _Res(3, True)
+
+        def reset(self):
+            pass
+
+        def close(self):
+            pass
+
+    class Cache(S.CubeCache):
+        def get(self, n):
+            return {"g_coord": None, "g_num": None, "rest": None, "nn": 4, "r_unit": np.ones(4)}
+
+    monkeypatch.setattr(S, "Context", Ctx)
+    reqs = S.make_stream(3, 0.0, seed=1, nmin=2, nmax=3)
+    S.serve_sequential(reqs, precision="mixed64", cache=Cache(), refine_delta=S.Server.refine_delta)
+    srv = S.Server.__new__(S.Server)                 # only _admit is exercised: no batch, no slots
+    srv.cache, srv.precision, srv.limit, srv.tol, srv.refine_delta = Cache(), "mixed64", 100, 1e-10, 0.1
+    srv._admit(S._Slot(Ctx()), reqs[0])
+    assert {d for _, d in seen} == {0.1} and {k for k, _ in seen} == {"server", "alone"}
