@@ -325,7 +325,7 @@ __device__ __forceinline__ void scatter_pu_block(int lb, int G, const int *__res
     for (int q0 = lb * PF_BLOCK + wave * 64; q0 < neq; q0 += stride) {
         const int q = q0 + lane;
         const bool live = q < neq;
-        const int a = live ? ptr[q] : 0, b = live ? ptr[q + 1] : 0;
+        const int a = live ? __builtin_nontemporal_load(&ptr[q]) : 0, b = live ? __builtin_nontemporal_load(&ptr[q + 1]) : 0;
         const int a0 = ptr[q0], a1 = ptr[q0 + 64 < neq ? q0 + 64 : neq];
         double s = 0.0;
         if (a1 - a0 <= SC_CAP) {
@@ -404,13 +404,14 @@ __device__ __forceinline__ void update_block(int lb, int G, double *__restrict__
     for (int k = 0; k < NQ; ++k) v[k] = 0.0;
     const int stride = G * PF_BLOCK;
     for (int q = lb * PF_BLOCK + threadIdx.x + 1; q <= neq; q += stride) {
-        // x, r and u are next touched a whole iteration later (streaming
-        // loads and stores); d is read again by the p update right after
+        // x, r, u and dinv are next touched a whole iteration later
+        // (streaming loads and stores); d is read again by the p update
+        // right after
         const double xo = __builtin_nontemporal_load(&x[q]);
         const double xn = __dadd_rn(xo, __dmul_rn(p[q], alpha));
         const double rq = __dsub_rn(__builtin_nontemporal_load(&r[q]),
                                     __dmul_rn(__builtin_nontemporal_load(&u[q]), alpha));
-        const double dq = __dmul_rn(dinv[q], rq);
+        const double dq = __dmul_rn(__builtin_nontemporal_load(&dinv[q]), rq);
         __builtin_nontemporal_store(xn, &x[q]);
         __builtin_nontemporal_store(rq, &r[q]);
         d[q] = dq;
